@@ -1,0 +1,162 @@
+// Device building blocks shared by every Monte Carlo kernel of the engine:
+//   * Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw,
+//     "Parallel random numbers: as easy as 1, 2, 3", SC'11),
+//   * the Box-Muller transform from two 32-bit words to two N(0,1) samples,
+//   * branch-free two-sum compensated FP32 accumulation,
+//   * wavefront (64-lane) and workgroup reductions in FP64.
+//
+// The counter layout and the uniform mapping below are a contract: the CPU
+// oracle (oracle/fsmc_oracle.c) reproduces them bit for bit, so a Philox-mode
+// run on the GPU and the oracle consume identical normals.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fsmc {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10
+// ---------------------------------------------------------------------------
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u;
+constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
+constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
+
+// Stream tags occupy the top 4 bits of counter word w so that the workloads
+// never share a counter even with the same seed.
+constexpr uint32_t kTagGbm = 1u;
+constexpr uint32_t kTagHeston = 2u;
+constexpr uint32_t kTagLibor = 3u;
+constexpr uint32_t kTagLsmc = 4u;
+
+__device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    // One v_mad_u64_u32 per product gives both halves.
+    const uint64_t p0 = (uint64_t)kPhiloxM0 * c.x;
+    const uint64_t p1 = (uint64_t)kPhiloxM1 * c.z;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c.x = hi1 ^ c.y ^ k0;
+    c.y = lo1;
+    c.z = hi0 ^ c.w ^ k1;
+    c.w = lo0;
+    k0 += kPhiloxW0;
+    k1 += kPhiloxW1;
+  }
+  return c;
+}
+
+// Counter word w: tag in bits 28..31, caller stream id in bits 0..27.
+__host__ __device__ __forceinline__ uint32_t counter_w(uint32_t tag, uint32_t stream) {
+  return (tag << 28) ^ (stream & 0x0FFFFFFFu);
+}
+
+// ---------------------------------------------------------------------------
+// Box-Muller.  u1 = ((a >> 8) + 1) * 2^-24 in (0, 1], u2 = (b >> 8) * 2^-24
+// in [0, 1): both are exact in FP32 and FP64, so the two precisions see the
+// same uniforms.  z0 = r cos(2 pi u2), z1 = r sin(2 pi u2), r = sqrt(-2 ln u1).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float& z0, float& z1) {
+  const float u1 = (float)((a >> 8) + 1u) * 0x1.0p-24f;
+  const float u2 = (float)(b >> 8) * 0x1.0p-24f;
+  // v_log_f32 is log2; -2 ln u = -2 ln2 log2 u.
+  const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+  // v_sin_f32 / v_cos_f32 take their argument in revolutions.
+  z0 = r * __builtin_amdgcn_cosf(u2);
+  z1 = r * __builtin_amdgcn_sinf(u2);
+}
+
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double& z0, double& z1) {
+  const double u1 = (double)((a >> 8) + 1u) * 0x1.0p-24;
+  const double u2 = (double)(b >> 8) * 0x1.0p-24;
+  const double r = sqrt(-2.0 * log(u1));
+  double s, c;
+  sincospi(2.0 * u2, &s, &c);
+  z0 = r * c;
+  z1 = r * s;
+}
+
+// Four normals from one Philox call.
+template <typename Real>
+__device__ __forceinline__ void normals4(U4 ctr, uint32_t k0, uint32_t k1, Real& n0, Real& n1, Real& n2,
+                                         Real& n3) {
+  const U4 r = philox4x32_10(ctr, k0, k1);
+  box_muller(r.x, r.y, n0, n1);
+  box_muller(r.z, r.w, n2, n3);
+}
+
+// ---------------------------------------------------------------------------
+// Compensated accumulation.  In FP32 mode a lane accumulates with a
+// branch-free two-sum (Knuth); the compensation is folded in before the
+// FP64 cross-lane combine.  Contraction into FMA is switched off locally so
+// that the error term survives -O3.
+// ---------------------------------------------------------------------------
+template <typename Real>
+struct CompSum {
+  Real s = 0, c = 0;
+  __device__ __forceinline__ void add(Real x) {
+#pragma clang fp contract(off)
+    const Real t = s + x;
+    const Real bp = t - s;
+    const Real e = (s - (t - bp)) + (x - bp);
+    s = t;
+    c += e;
+  }
+  __device__ __forceinline__ double value() const { return (double)s + (double)c; }
+};
+
+template <>
+struct CompSum<double> {
+  double s = 0;
+  __device__ __forceinline__ void add(double x) { s += x; }
+  __device__ __forceinline__ double value() const { return s; }
+};
+
+// ---------------------------------------------------------------------------
+// Reductions (fixed order => bitwise-reproducible for a fixed launch shape).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sums NV values across a workgroup of blockDim.x threads (multiple of 64).
+// Result valid in thread 0.  `lds` must hold NV * (blockDim.x / 64) doubles.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lds[k * nwaves + wave] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < nwaves; ++w) s += lds[k * nwaves + w];
+      v[k] = s;
+    }
+  }
+}
+
+// exp2 in the working precision: hardware v_exp_f32 for FP32.
+__device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ double fexp2(double x) { return exp2(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
+
+}  // namespace fsmc

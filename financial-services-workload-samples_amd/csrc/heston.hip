@@ -1,0 +1,458 @@
+// Heston arithmetic-average Asian call, one path per lane, the whole time
+// loop in registers.  Each step follows MCAsianOptions.cpp: correlate the
+// two normals (:153-158), advance the spot with the variance at the start of
+// the step (:51-62), then the full-truncation Euler variance (:65-74), and
+// accumulate the arithmetic average over steps 1..N (:168-182).
+//
+// Correct Heston uses sqrt(v+) as the volatility; FSMC_QUIRK_ASIAN_VAR_AS_VOL
+// reproduces the sample's v+ (:57-60).  The estimator is per-path sums
+// (sum, sum of squares); FSMC_QUIRK_ASIAN_RUNNING_VAL2 instead reproduces the
+// sample's chunk-of-64 running val2 (:184-201) from per-wavefront chunk sums.
+#include "fsmc_device.hpp"
+#include "fsmc_host.hpp"
+
+namespace fsmc {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kChunk = 64;  // VECLEN of MCAsianOptions.cpp:29 == one wavefront
+constexpr double kLog2e = 1.4426950408889634074;
+
+struct HestonArgs {
+  int n_paths, n_steps;
+  double r, rho;
+  const double *S, *X, *T, *v0, *kappa, *theta, *xi;
+  const double *z1, *z2;  // buffer mode
+  uint32_t k0, k1, w;
+  double* partials;  // [n_options][gridDim.x][2]
+  double* chunks;    // [n_options][n_paths / 64]  (running-val2 estimator)
+};
+
+template <typename Real>
+struct HestonConsts {
+  Real c_rdt, c_hdt, c_sdt, kdt, xsdt, theta, rho, srho;
+  __device__ HestonConsts(const HestonArgs& a, int o) {
+    const double dt = a.T[o] / a.n_steps;
+    const double sdt = sqrt(dt);
+    c_rdt = (Real)(a.r * dt * kLog2e);
+    c_hdt = (Real)(0.5 * dt * kLog2e);
+    c_sdt = (Real)(sdt * kLog2e);
+    kdt = (Real)(a.kappa[o] * dt);
+    xsdt = (Real)(a.xi[o] * sdt);
+    theta = (Real)a.theta[o];
+    rho = (Real)a.rho;
+    srho = (Real)sqrt(1.0 - a.rho * a.rho);
+  }
+};
+
+// One Euler step; returns nothing, updates S, v and the running average sum.
+template <typename Real, bool kVarAsVol>
+__device__ __forceinline__ void heston_step(const HestonConsts<Real>& c, Real& S, Real& v, CompSum<Real>& avg,
+                                            Real z1, Real z2) {
+  const Real vp = v > Real(0) ? v : Real(0);
+  const Real sq = fsqrt(vp);
+  const Real sv = kVarAsVol ? vp : sq;
+  const Real e = c.c_rdt - c.c_hdt * sv * sv + c.c_sdt * sv * z1;
+  S = S * fexp2(e);
+  avg.add(S);
+  const Real zc = c.rho * z1 + c.srho * z2;
+  v = v + c.kdt * (c.theta - vp) + c.xsdt * sq * zc;
+}
+
+template <typename Real, bool kBuffer, bool kVarAsVol, bool kChunks>
+__global__ __launch_bounds__(kBlock) void heston_paths_kernel(HestonArgs a) {
+  __shared__ double lds[2 * (kBlock / 64)];
+  const int o = blockIdx.y;
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  const HestonConsts<Real> c(a, o);
+  double pay = 0.0;
+  if (p < a.n_paths) {
+    Real S = (Real)a.S[o];
+    Real v = (Real)a.v0[o];
+    CompSum<Real> avg;
+    if (kBuffer) {
+      const int nchunks = a.n_paths / kChunk;
+      const size_t base = ((size_t)o * nchunks + (p / kChunk)) * (size_t)a.n_steps * kChunk + (p % kChunk);
+      const double* z1 = a.z1 + base;
+      const double* z2 = a.z2 + base;
+      for (int k = 0; k < a.n_steps; ++k)
+        heston_step<Real, kVarAsVol>(c, S, v, avg, (Real)z1[(size_t)k * kChunk], (Real)z2[(size_t)k * kChunk]);
+    } else {
+      const int nfull = a.n_steps & ~1;
+      for (int k = 0; k < nfull; k += 2) {
+        Real n0, n1, n2, n3;
+        normals4<Real>(U4{(uint32_t)p, (uint32_t)(k >> 1), (uint32_t)o, a.w}, a.k0, a.k1, n0, n1, n2, n3);
+        heston_step<Real, kVarAsVol>(c, S, v, avg, n0, n1);
+        heston_step<Real, kVarAsVol>(c, S, v, avg, n2, n3);
+      }
+      if (a.n_steps & 1) {
+        Real n0, n1, n2, n3;
+        normals4<Real>(U4{(uint32_t)p, (uint32_t)(nfull >> 1), (uint32_t)o, a.w}, a.k0, a.k1, n0, n1, n2, n3);
+        heston_step<Real, kVarAsVol>(c, S, v, avg, n0, n1);
+      }
+    }
+    const double mean = avg.value() / (double)a.n_steps;
+    const double x = mean - a.X[o];
+    pay = x > 0.0 ? x : 0.0;
+  }
+  if (kChunks) {
+    // One wavefront == one chunk of 64 paths (n_paths % 64 == 0 enforced).
+    const double cs = wave_sum(pay);
+    if ((threadIdx.x & 63) == 0 && p < a.n_paths) {
+      const int nchunks = a.n_paths / kChunk;
+      a.chunks[(size_t)o * nchunks + p / kChunk] = cs;
+    }
+  }
+  double v2[2] = {pay, pay * pay};
+  block_sum<2>(v2, lds);
+  if (threadIdx.x == 0) {
+    double* out = a.partials + ((size_t)o * gridDim.x + blockIdx.x) * 2;
+    out[0] = v2[0];
+    out[1] = v2[1];
+  }
+}
+
+__device__ __forceinline__ void mean_se(double s, double s2, double N, double f, double* mean, double* se) {
+  *mean = f * s / N;
+  const double var = (N * s2 - s * s) / (N * (N - 1.0));
+  *se = fabs(f) * sqrt(var > 0.0 ? var : 0.0) / sqrt(N);
+}
+
+__global__ void heston_finalize_kernel(int n_options, int nbx, int n_paths, double r, const double* T,
+                                       const double* partials, const double* chunks, int running_val2,
+                                       double* result, double* confidence) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n_options) return;
+  const double N = (double)n_paths;
+  const double exprt = exp(-r * T[o]);
+  if (running_val2) {
+    // MCAsianOptions.cpp:184-201 verbatim in meaning: val accumulates chunk
+    // means, val2 accumulates the square of the running val.
+    const int nchunks = n_paths / kChunk;
+    double val = 0.0, val2 = 0.0;
+    const double* cs = chunks + (size_t)o * nchunks;
+    for (int ci = 0; ci < nchunks; ++ci) {
+      val += cs[ci] / kChunk;
+      val2 += val * val;
+    }
+    result[o] = exprt * val * (1.0 / N) * kChunk;
+    const double stddev = sqrt((N * val2 - val * val) * (1.0 / (N * (N - 1.0))));
+    confidence[o] = exprt * 1.96 * stddev * (1.0 / sqrt(N));
+  } else {
+    double s0 = 0.0, s1 = 0.0;
+    const double* p = partials + (size_t)o * nbx * 2;
+    for (int b = 0; b < nbx; ++b) {
+      s0 += p[2 * b];
+      s1 += p[2 * b + 1];
+    }
+    result[o] = exprt * s0 / N;
+    const double stddev = sqrt((N * s1 - s0 * s0) * (1.0 / (N * (N - 1.0))));
+    confidence[o] = exprt * 1.96 * stddev * (1.0 / sqrt(N));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint (reverse sweep).  Pathwise derivatives of
+//   P = e^{-rT} max(A - K, 0),  A = (1/N) sum_{k=1..N} S_k,
+//   S_{k+1} = S_k exp(e_k),  e_k = r dt - vp_k dt / 2 + sqrt(vp_k dt) z1_k,
+//   v_{k+1} = v_k + kappa dt (theta - vp_k) + xi sqrt(vp_k dt) zc_k,
+// with vp = max(v, 0).  Going backwards with U_{k+1} = dP/de_k =
+// (e^{-rT}/N) sum_{j>k} S_j:
+//   vbar_k = vbar_{k+1} dv_{k+1}/dv_k + U_{k+1} de_k/dv_k,
+//   rbar   = -T P + dt sum_k U_{k+1},   dP/dS0 = U_1 / S0,
+// and vega = dP/dsqrt(v0) = 2 sqrt(v0) vbar_0.  The forward sweep stores
+// (S, v) every kSeg steps in the workspace; the reverse sweep recomputes one
+// segment at a time into registers, regenerating its normals from the Philox
+// counters, then walks it backwards.
+// ---------------------------------------------------------------------------
+constexpr int kSeg = 16;  // checkpoint interval; even, so segments start on a Philox pair
+
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void heston_adjoint_kernel(HestonArgs a, Real* ckpt) {
+  __shared__ double lds[8 * (kBlock / 64)];
+  const int o = blockIdx.y;
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  const HestonConsts<Real> c(a, o);
+  double out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (p < a.n_paths) {
+    const int N = a.n_steps;
+    const int nseg = (N + kSeg - 1) / kSeg;
+    const double Td = a.T[o];
+    const double dtd = Td / N;
+    const Real hdt = (Real)(0.5 * dtd), sdt = (Real)sqrt(dtd);
+    const Real S0 = (Real)a.S[o];
+    const Real v00 = (Real)a.v0[o];
+    Real* ck = ckpt + (size_t)o * nseg * 2 * a.n_paths + p;  // [o][seg][S|v][path]
+    Real S = S0, v = v00;
+    CompSum<Real> avg;
+    for (int seg = 0; seg < nseg; ++seg) {
+      ck[(size_t)(2 * seg) * a.n_paths] = S;
+      ck[(size_t)(2 * seg + 1) * a.n_paths] = v;
+#pragma unroll
+      for (int j = 0; j < kSeg; j += 2) {
+        const int k = seg * kSeg + j;
+        if (k < N) {
+          Real n0, n1, n2, n3;
+          normals4<Real>(U4{(uint32_t)p, (uint32_t)(k >> 1), (uint32_t)o, a.w}, a.k0, a.k1, n0, n1, n2, n3);
+          heston_step<Real, false>(c, S, v, avg, n0, n1);
+          if (k + 1 < N) heston_step<Real, false>(c, S, v, avg, n2, n3);
+        }
+      }
+    }
+    const double x = avg.value() / (double)N - a.X[o];
+    if (x > 0.0) {
+      const double disc = exp(-a.r * Td);
+      const Real abar = (Real)(disc / N);
+      Real U = 0, vbar = 0, ebar_sum = 0;
+      Real Snext = S;  // S_N
+      for (int seg = nseg - 1; seg >= 0; --seg) {
+        Real Ss[kSeg], vs[kSeg], z1s[kSeg], zcs[kSeg];
+        Real Sr = ck[(size_t)(2 * seg) * a.n_paths];
+        Real vr = ck[(size_t)(2 * seg + 1) * a.n_paths];
+        CompSum<Real> dummy;
+#pragma unroll
+        for (int j = 0; j < kSeg; j += 2) {
+          const int k = seg * kSeg + j;
+          if (k < N) {
+            Real n0, n1, n2, n3;
+            normals4<Real>(U4{(uint32_t)p, (uint32_t)(k >> 1), (uint32_t)o, a.w}, a.k0, a.k1, n0, n1, n2, n3);
+            Ss[j] = Sr;
+            vs[j] = vr;
+            z1s[j] = n0;
+            zcs[j] = c.rho * n0 + c.srho * n1;
+            heston_step<Real, false>(c, Sr, vr, dummy, n0, n1);
+            Ss[j + 1] = Sr;
+            vs[j + 1] = vr;
+            z1s[j + 1] = n2;
+            zcs[j + 1] = c.rho * n2 + c.srho * n3;
+            if (k + 1 < N) heston_step<Real, false>(c, Sr, vr, dummy, n2, n3);
+          }
+        }
+#pragma unroll
+        for (int j = kSeg - 1; j >= 0; --j) {
+          const int k = seg * kSeg + j;
+          if (k < N) {
+            const Real Skp1 = (j + 1 < kSeg && k + 1 < N) ? Ss[j + 1] : Snext;
+            U += abar * Skp1;
+            ebar_sum += U;
+            const Real vk = vs[j];
+            if (vk > Real(0)) {
+              const Real h = Real(0.5) / fsqrt(vk);
+              const Real dedv = sdt * z1s[j] * h - hdt;
+              const Real dvdv = Real(1) - c.kdt + c.xsdt * zcs[j] * h;
+              vbar = vbar * dvdv + U * dedv;
+            }
+          }
+        }
+        Snext = Ss[0];
+      }
+      const double delta = (double)U / a.S[o];
+      const double vega = (double)vbar * 2.0 * sqrt(a.v0[o]);
+      const double rho = -Td * disc * x + dtd * (double)ebar_sum;
+      out[0] = x;
+      out[1] = x * x;
+      out[2] = delta;
+      out[3] = delta * delta;
+      out[4] = vega;
+      out[5] = vega * vega;
+      out[6] = rho;
+      out[7] = rho * rho;
+    }
+  }
+  block_sum<8>(out, lds);
+  if (threadIdx.x == 0) {
+    double* dst = a.partials + ((size_t)o * gridDim.x + blockIdx.x) * 8;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) dst[k] = out[k];
+  }
+}
+
+__global__ void heston_greeks_finalize_kernel(int n_options, int nbx, int n_paths, double r, const double* T,
+                                              const double* partials, double* result, double* confidence,
+                                              double* greeks) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n_options) return;
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const double* p = partials + (size_t)o * nbx * 8;
+  for (int b = 0; b < nbx; ++b)
+    for (int k = 0; k < 8; ++k) s[k] += p[8 * b + k];
+  const double N = (double)n_paths;
+  const double exprt = exp(-r * T[o]);
+  double se;
+  mean_se(s[0], s[1], N, exprt, result + o, &se);
+  confidence[o] = 1.96 * se;
+  double* g = greeks + 6 * (size_t)o;
+  for (int k = 0; k < 3; ++k) mean_se(s[2 + 2 * k], s[3 + 2 * k], N, 1.0, g + k, g + 3 + k);
+}
+
+struct HestonPlan {
+  int nbx;
+  size_t partial_bytes, chunk_bytes, ws;  // forward estimator
+  size_t gpartial_bytes, ckpt_bytes, gws;  // adjoint
+};
+
+HestonPlan heston_plan(const fsmc_heston_params* p, const fsmc_mode* m) {
+  HestonPlan h;
+  h.nbx = (p->n_paths + kBlock - 1) / kBlock;
+  h.partial_bytes = align_up((size_t)p->n_options * h.nbx * 2 * sizeof(double), 256);
+  h.chunk_bytes = align_up((size_t)p->n_options * (p->n_paths / kChunk + 1) * sizeof(double), 256);
+  h.ws = h.partial_bytes + h.chunk_bytes;
+  const size_t real = m->precision == FSMC_FP64 ? sizeof(double) : sizeof(float);
+  const size_t nseg = (size_t)(p->n_steps + kSeg - 1) / kSeg;
+  h.gpartial_bytes = align_up((size_t)p->n_options * h.nbx * 8 * sizeof(double), 256);
+  h.ckpt_bytes = align_up((size_t)p->n_options * nseg * 2 * (size_t)p->n_paths * real, 256);
+  h.gws = h.gpartial_bytes + h.ckpt_bytes;
+  return h;
+}
+
+int heston_validate(const fsmc_heston_params* p, const fsmc_mode* m) {
+  FSMC_REQUIRE(p != nullptr, "params is NULL");
+  FSMC_TRY(validate_mode(m, true));
+  FSMC_REQUIRE(p->n_options >= 1 && p->n_options <= 65535, "n_options %d outside [1, 65535]", p->n_options);
+  FSMC_REQUIRE(p->n_paths >= 2, "n_paths %d < 2", p->n_paths);
+  FSMC_REQUIRE(p->n_steps >= 1, "n_steps %d < 1", p->n_steps);
+  FSMC_REQUIRE(p->rho >= -1.0 && p->rho <= 1.0, "rho %g outside [-1, 1]", p->rho);
+  const bool needs64 = m->rng == FSMC_RNG_BUFFER || (m->quirks & FSMC_QUIRK_ASIAN_RUNNING_VAL2);
+  FSMC_REQUIRE(!needs64 || p->n_paths % kChunk == 0,
+               "buffer mode and the running-val2 estimator need n_paths %% 64 == 0 (got %d)", p->n_paths);
+  return FSMC_OK;
+}
+
+template <typename Real, bool B, bool Q4, bool CH>
+void launch(const HestonArgs& a, int n_options, int nbx, hipStream_t s) {
+  hipLaunchKernelGGL((heston_paths_kernel<Real, B, Q4, CH>), dim3(nbx, n_options), dim3(kBlock), 0, s, a);
+}
+
+template <typename Real, bool B>
+void launch_q(const HestonArgs& a, bool q4, bool ch, int n_options, int nbx, hipStream_t s) {
+  if (q4) ch ? launch<Real, B, true, true>(a, n_options, nbx, s) : launch<Real, B, true, false>(a, n_options, nbx, s);
+  else ch ? launch<Real, B, false, true>(a, n_options, nbx, s) : launch<Real, B, false, false>(a, n_options, nbx, s);
+}
+
+}  // namespace
+
+}  // namespace fsmc
+
+using namespace fsmc;
+
+extern "C" {
+
+size_t fsmc_heston_asian_workspace(const fsmc_heston_params* p, const fsmc_mode* m) {
+  if (heston_validate(p, m) != FSMC_OK) return 0;
+  const HestonPlan h = heston_plan(p, m);
+  // Philox mode without quirks may be asked for Greeks: size for the adjoint.
+  const bool greeks_possible = m->rng == FSMC_RNG_PHILOX && m->quirks == 0;
+  return greeks_possible && h.gws > h.ws ? h.gws : h.ws;
+}
+
+int fsmc_heston_asian_async(const fsmc_heston_params* p, const fsmc_mode* m, const double* d_S,
+                            const double* d_X, const double* d_T, const double* d_v0, const double* d_kappa,
+                            const double* d_theta, const double* d_xi, const double* d_z1, const double* d_z2,
+                            double* d_result, double* d_conf, double* d_greeks, void* d_ws, size_t ws_bytes,
+                            void* stream) {
+  FSMC_TRY(heston_validate(p, m));
+  FSMC_REQUIRE(d_S && d_X && d_T && d_v0 && d_kappa && d_theta && d_xi && d_result && d_conf,
+               "NULL device pointer");
+  const bool buffer = m->rng == FSMC_RNG_BUFFER;
+  FSMC_REQUIRE(!buffer || (d_z1 && d_z2), "buffer mode needs z1 and z2");
+  FSMC_REQUIRE(!d_greeks || (!buffer && m->quirks == 0), "Greeks need Philox mode without quirks");
+  const HestonPlan h = heston_plan(p, m);
+  const size_t need = fsmc_heston_asian_workspace(p, m);
+  if (d_ws == nullptr || ws_bytes < need) {
+    set_error("workspace %zu bytes < required %zu", ws_bytes, need);
+    return FSMC_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  HestonArgs a;
+  a.n_paths = p->n_paths;
+  a.n_steps = p->n_steps;
+  a.r = p->riskfree;
+  a.rho = p->rho;
+  a.S = d_S;
+  a.X = d_X;
+  a.T = d_T;
+  a.v0 = d_v0;
+  a.kappa = d_kappa;
+  a.theta = d_theta;
+  a.xi = d_xi;
+  a.z1 = d_z1;
+  a.z2 = d_z2;
+  a.k0 = (uint32_t)m->seed;
+  a.k1 = (uint32_t)(m->seed >> 32);
+  a.w = counter_w(kTagHeston, m->stream);
+  a.partials = (double*)d_ws;
+  a.chunks = (double*)((char*)d_ws + h.partial_bytes);
+  if (d_greeks) {
+    // One launch: the adjoint kernel's forward sweep also yields the price.
+    a.partials = (double*)d_ws;
+    if (m->precision == FSMC_FP64)
+      hipLaunchKernelGGL(heston_adjoint_kernel<double>, dim3(h.nbx, p->n_options), dim3(kBlock), 0, s, a,
+                         (double*)((char*)d_ws + h.gpartial_bytes));
+    else
+      hipLaunchKernelGGL(heston_adjoint_kernel<float>, dim3(h.nbx, p->n_options), dim3(kBlock), 0, s, a,
+                         (float*)((char*)d_ws + h.gpartial_bytes));
+    FSMC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(heston_greeks_finalize_kernel, dim3((p->n_options + 63) / 64), dim3(64), 0, s,
+                       p->n_options, h.nbx, p->n_paths, p->riskfree, d_T, a.partials, d_result, d_conf, d_greeks);
+    FSMC_HIP(hipGetLastError());
+    return FSMC_OK;
+  }
+  const bool q4 = (m->quirks & FSMC_QUIRK_ASIAN_VAR_AS_VOL) != 0;
+  const bool ch = (m->quirks & FSMC_QUIRK_ASIAN_RUNNING_VAL2) != 0;
+  if (m->precision == FSMC_FP64) {
+    buffer ? launch_q<double, true>(a, q4, ch, p->n_options, h.nbx, s)
+           : launch_q<double, false>(a, q4, ch, p->n_options, h.nbx, s);
+  } else {
+    buffer ? launch_q<float, true>(a, q4, ch, p->n_options, h.nbx, s)
+           : launch_q<float, false>(a, q4, ch, p->n_options, h.nbx, s);
+  }
+  FSMC_HIP(hipGetLastError());
+  const int fb = 64;
+  hipLaunchKernelGGL(heston_finalize_kernel, dim3((p->n_options + fb - 1) / fb), dim3(fb), 0, s, p->n_options,
+                     h.nbx, p->n_paths, p->riskfree, d_T, a.partials, a.chunks, ch ? 1 : 0, d_result, d_conf);
+  FSMC_HIP(hipGetLastError());
+  return FSMC_OK;
+}
+
+int fsmc_heston_asian(fsmc_ctx* ctx, const fsmc_heston_params* p, const fsmc_mode* m, const double* S,
+                      const double* X, const double* T, const double* v0, const double* kappa, const double* theta,
+                      const double* xi, const double* z1, const double* z2, double* call_result,
+                      double* call_confidence, double* greeks) {
+  FSMC_REQUIRE(ctx != nullptr, "ctx is NULL");
+  FSMC_TRY(heston_validate(p, m));
+  FSMC_REQUIRE(S && X && T && v0 && kappa && theta && xi && call_result && call_confidence, "NULL host pointer");
+  const bool buffer = m->rng == FSMC_RNG_BUFFER;
+  FSMC_REQUIRE(!buffer || (z1 && z2), "buffer mode needs z1 and z2");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const size_t n = (size_t)p->n_options;
+  const size_t zn = buffer ? n * (size_t)p->n_paths * (size_t)p->n_steps : 0;
+  FSMC_TRY(ctx_prepare(ctx, 7 * n * sizeof(double), 2 * zn * sizeof(double), 8 * n * sizeof(double),
+                       fsmc_heston_asian_workspace(p, m)));
+  double* d = (double*)ctx->inputs.ptr;
+  const double* host[7] = {S, X, T, v0, kappa, theta, xi};
+  hipStream_t s = ctx->stream;
+  for (int k = 0; k < 7; ++k)
+    FSMC_HIP(hipMemcpyAsync(d + k * n, host[k], n * sizeof(double), hipMemcpyHostToDevice, s));
+  double* dz1 = (double*)ctx->normals.ptr;
+  double* dz2 = dz1 ? dz1 + zn : nullptr;
+  if (buffer) {
+    FSMC_HIP(hipMemcpyAsync(dz1, z1, zn * sizeof(double), hipMemcpyHostToDevice, s));
+    FSMC_HIP(hipMemcpyAsync(dz2, z2, zn * sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  double* dres = (double*)ctx->outputs.ptr;
+  double* dconf = dres + n;
+  double* dgr = greeks ? dconf + n : nullptr;
+  int st = fsmc_heston_asian_async(p, m, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, d + 6 * n, dz1, dz2,
+                                   dres, dconf, dgr, ctx->workspace.ptr, ctx->workspace.bytes, s);
+  if (st != FSMC_OK) {
+    (void)hipStreamSynchronize(s);
+    return st;
+  }
+  FSMC_HIP(hipMemcpyAsync(call_result, dres, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  FSMC_HIP(hipMemcpyAsync(call_confidence, dconf, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (greeks) FSMC_HIP(hipMemcpyAsync(greeks, dgr, 6 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+  FSMC_HIP(hipStreamSynchronize(s));
+  return FSMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+/*
+ * fsmc.h - C ABI of the MI355X (gfx950) Monte Carlo pricing engine.
+ *
+ * Every entry point takes plain pointers and sizes, returns an int status
+ * (FSMC_OK == 0, negative on failure) and never exits the process.  Two
+ * flavours exist for each workload:
+ *
+ *   fsmc_X(ctx, ...)          synchronous; host pointers in and out; the
+ *                             context owns and reuses the device buffers.
+ *   fsmc_X_async(..., stream) device pointers; the caller provides a device
+ *                             workspace of fsmc_X_workspace(...) bytes and a
+ *                             hipStream_t (passed as void*); nothing is
+ *                             allocated and nothing synchronises.
+ *
+ * Normals come either from the in-kernel Philox4x32-10 generator
+ * (FSMC_RNG_PHILOX) or from a caller buffer laid out exactly as the reference
+ * sample consumes its MKL stream (FSMC_RNG_BUFFER, "oracle mode").
+ *
+ * Philox contract (mirrored bit for bit by the CPU oracle): key =
+ * (seed & 0xffffffff, seed >> 32); counter = (c0, c1, c2, tag << 28 ^ stream)
+ * with tag 1 GBM, 2 Heston, 3 LIBOR, 4 LSMC.  Each call yields four words
+ * (a, b, c, d) -> Box-Muller pairs (a, b) and (c, d): u1 = ((w1 >> 8) + 1)
+ * 2^-24, u2 = (w2 >> 8) 2^-24, z = sqrt(-2 ln u1) (cos 2 pi u2, sin 2 pi u2).
+ *   GBM     c0 = path / 4,  c1 = 0,        c2 = option; normal j -> path 4 c0 + j
+ *   Heston  c0 = path,      c1 = step / 2, c2 = option; (z1, z2) of step 2 c1
+ *           are normals 0, 1 and of step 2 c1 + 1 normals 2, 3
+ *   LIBOR   c0 = path,      c1 = step / 4, c2 = 0;      normal j -> step 4 c1 + j
+ *   LSMC    c0 = path,      c1 = step / 4, c2 = option; normal j -> step 4 c1 + j
+ */
+#ifndef FSMC_H_
+#define FSMC_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define FSMC_OK 0
+#define FSMC_ERR_INVALID_ARG (-1) /* bad size, pointer, mode or parameter   */
+#define FSMC_ERR_DEVICE (-2)      /* HIP runtime error                      */
+#define FSMC_ERR_ALLOC (-3)       /* device allocation failed               */
+#define FSMC_ERR_WORKSPACE (-4)   /* async workspace smaller than required  */
+
+/* ---- mode --------------------------------------------------------------- */
+#define FSMC_FP32 0 /* FP32 path arithmetic, compensated FP32 sums, FP64 combine */
+#define FSMC_FP64 1 /* plain FP64 throughout                                      */
+
+#define FSMC_RNG_PHILOX 0 /* in-kernel Philox4x32-10 + Box-Muller          */
+#define FSMC_RNG_BUFFER 1 /* caller-supplied normals (reference layout)     */
+
+/* Reference-compatibility switches ("refcompat"): each reproduces one
+ * documented behaviour of the reference sample bit for bit. */
+#define FSMC_QUIRK_ASIAN_VAR_AS_VOL (1u << 0)   /* MCAsianOptions.cpp:57-60: sigma = v+ */
+#define FSMC_QUIRK_ASIAN_RUNNING_VAL2 (1u << 1) /* MCAsianOptions.cpp:191-192,199        */
+#define FSMC_QUIRK_LIBOR_INPLACE_ADJ (1u << 2)  /* liborSwaptionGreeks.cpp:173-184       */
+
+typedef struct fsmc_mode {
+  int precision;   /* FSMC_FP32 | FSMC_FP64                               */
+  int rng;         /* FSMC_RNG_PHILOX | FSMC_RNG_BUFFER                   */
+  uint64_t seed;   /* Philox key                                          */
+  uint32_t stream; /* Philox sub-stream id (28 bits), for independent runs */
+  uint32_t quirks; /* FSMC_QUIRK_* bit mask                               */
+} fsmc_mode;
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct fsmc_ctx fsmc_ctx;
+
+int fsmc_ctx_create(int device, fsmc_ctx** out);
+int fsmc_ctx_destroy(fsmc_ctx* ctx);
+const char* fsmc_status_string(int status);
+/* Detail of the last failure on the calling thread ("" if none). */
+const char* fsmc_last_error(void);
+
+/* ---- GBM European call ---------------------------------------------------
+ * Replaces the option loop inlined in main, MonteCarloInsideBlockingDP.cpp:
+ * 237-270 (per option: paths S_T = S*2^(MuByT + VBySqrtT*z), payoff sums
+ * v0, v1, then CallResult = e^{-rT} v0/N and CallConfidence = 1.96 * SE,
+ * :266-269).  Buffer mode consumes the reference's float samples
+ * s = MuByT + VBySqrtT*z (log2 domain, vsRngGaussian output, :249) laid out
+ * [n_options][n_paths].  greeks (optional, [n_options][6]) receives the
+ * pathwise delta, vega, rho followed by their standard errors. */
+typedef struct fsmc_gbm_params {
+  int n_options;
+  int n_paths; /* per option */
+  double riskfree;
+  double volatility;
+} fsmc_gbm_params;
+
+int fsmc_gbm_european(fsmc_ctx* ctx, const fsmc_gbm_params* p, const fsmc_mode* m, const double* S,
+                      const double* X, const double* T, const float* samples, double* call_result,
+                      double* call_confidence, double* greeks);
+size_t fsmc_gbm_european_workspace(const fsmc_gbm_params* p, const fsmc_mode* m);
+int fsmc_gbm_european_async(const fsmc_gbm_params* p, const fsmc_mode* m, const double* d_S,
+                            const double* d_X, const double* d_T, const float* d_samples,
+                            double* d_call_result, double* d_call_confidence, double* d_greeks,
+                            void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* ---- Heston arithmetic-average Asian call --------------------------------
+ * Replaces MonteCarloTimeStep, MCAsianOptions.cpp:76-86 (same nine SoA
+ * arrays: results, confidences, S, X, T, initialVol, kappa, theta, xi;
+ * RISKFREE and RHO of :39-41 become riskfree/rho; SIMSTEPS and RAND_N of
+ * :28-33 become n_steps/n_paths).  Buffer mode consumes two normal streams
+ * z1, z2 in the reference consumption order
+ * [n_options][n_paths/64][n_steps][64] (get_gaussian blocks of 64 lanes x
+ * n_steps, mkl_rng.h:57-62, indexed z[step*64+lane] at :151,166); n_paths
+ * must then be a multiple of 64.  greeks (optional, [n_options][6], Philox
+ * mode without quirks) receives delta dV/dS0, vega dV/dsqrt(v0) and rho dV/dr
+ * from the adjoint (reverse-sweep) kernel, followed by their standard
+ * errors; the price then comes from the same kernel's forward sweep. */
+typedef struct fsmc_heston_params {
+  int n_options;
+  int n_paths; /* per option */
+  int n_steps;
+  double riskfree;
+  double rho;
+} fsmc_heston_params;
+
+int fsmc_heston_asian(fsmc_ctx* ctx, const fsmc_heston_params* p, const fsmc_mode* m, const double* S,
+                      const double* X, const double* T, const double* v0, const double* kappa,
+                      const double* theta, const double* xi, const double* z1, const double* z2,
+                      double* call_result, double* call_confidence, double* greeks);
+size_t fsmc_heston_asian_workspace(const fsmc_heston_params* p, const fsmc_mode* m);
+int fsmc_heston_asian_async(const fsmc_heston_params* p, const fsmc_mode* m, const double* d_S,
+                            const double* d_X, const double* d_T, const double* d_v0,
+                            const double* d_kappa, const double* d_theta, const double* d_xi,
+                            const double* d_z1, const double* d_z2, double* d_call_result,
+                            double* d_call_confidence, double* d_greeks, void* d_workspace,
+                            size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FSMC_H_ */

@@ -1,0 +1,60 @@
+/*
+ * CPU oracle of the Monte Carlo engine.  TEST INFRASTRUCTURE ONLY: this code
+ * is the checker the GPU results are compared against (and the CPU baseline
+ * that bench.py times); no product code path links or calls it.
+ *
+ * Every function restates the reference algorithm in plain FP64 C, citing
+ * the reference file:line it follows.  The Philox/Box-Muller contract is the
+ * one documented in include/fsmc.h.
+ */
+#ifndef FSMC_ORACLE_H_
+#define FSMC_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- generator ---------------------------------------------------------- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void orc_box_muller(uint32_t a, uint32_t b, double* z0, double* z1);
+/* Four normals of the counter (c0, c1, c2, tag<<28 ^ stream) under key seed. */
+void orc_normals4(uint64_t seed, uint32_t stream, uint32_t tag, uint32_t c0, uint32_t c1, uint32_t c2,
+                  double out[4]);
+
+/* ---- analytic pins -------------------------------------------------------- */
+double orc_bs_call(double S, double X, double T, double r, double sigma);
+/* delta, vega, rho of the Black-Scholes call. */
+void orc_bs_call_greeks(double S, double X, double T, double r, double sigma, double out[3]);
+double orc_crr_european_call(double S, double X, double T, double r, double sigma, int steps);
+double orc_crr_american_put(double S, double K, double T, double r, double sigma, int steps);
+/* Geometric-average Asian call with discrete monitoring t_k = k T/N, k=1..N,
+ * under GBM with volatility sigma (closed form; the xi = 0, v0 = theta
+ * limit of the Heston model with sigma = sqrt(v0)). */
+double orc_geometric_asian_call(double S, double X, double T, double r, double sigma, int N);
+
+/* ---- GBM European ----------------------------------------------------------- */
+void orc_gbm_buffer(int n_opt, int n_paths, double r, double sigma, const double* S, const double* X,
+                    const double* T, const float* samples, double* res, double* conf);
+void orc_gbm_philox(int n_opt, int n_paths, double r, double sigma, const double* S, const double* X,
+                    const double* T, uint64_t seed, uint32_t stream, double* res, double* conf,
+                    double* greeks6);
+
+/* ---- Heston Asian --------------------------------------------------------- */
+#define ORC_Q_VAR_AS_VOL 1u
+#define ORC_Q_RUNNING_VAL2 2u
+void orc_heston_buffer(int n_opt, int n_paths, int n_steps, double r, double rho, const double* S,
+                       const double* X, const double* T, const double* v0, const double* kappa,
+                       const double* theta, const double* xi, const double* z1, const double* z2,
+                       uint32_t quirks, double* res, double* conf);
+void orc_heston_philox(int n_opt, int n_paths, int n_steps, double r, double rho, const double* S,
+                       const double* X, const double* T, const double* v0, const double* kappa,
+                       const double* theta, const double* xi, uint64_t seed, uint32_t stream,
+                       uint32_t quirks, double* res, double* conf, double* greeks6);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

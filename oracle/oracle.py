@@ -1,0 +1,170 @@
+"""ctypes binding of the CPU oracle (oracle/fsmc_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+as the checker, and by bench.py's ``cpu_baseline`` leg.  The product path
+never imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+_F = ctypes.POINTER(ctypes.c_float)
+_U32 = ctypes.POINTER(ctypes.c_uint32)
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc, full-precision libm)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_philox4x32_10.argtypes = [_U32, _U32, _U32]
+        L.orc_box_muller.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _D, _D]
+        L.orc_normals4.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                   ctypes.c_uint32, ctypes.c_uint32, _D]
+        for name in ("orc_bs_call", "orc_geometric_asian_call"):
+            getattr(L, name).restype = ctypes.c_double
+        L.orc_bs_call.argtypes = [ctypes.c_double] * 5
+        L.orc_bs_call_greeks.argtypes = [ctypes.c_double] * 5 + [_D]
+        L.orc_geometric_asian_call.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int]
+        for name in ("orc_crr_european_call", "orc_crr_american_put"):
+            getattr(L, name).restype = ctypes.c_double
+            getattr(L, name).argtypes = [ctypes.c_double] * 5 + [ctypes.c_int]
+        L.orc_gbm_buffer.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     _D, _D, _D, _F, _D, _D]
+        L.orc_gbm_philox.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     _D, _D, _D, ctypes.c_uint64, ctypes.c_uint32, _D, _D, _D]
+        L.orc_heston_buffer.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 2 + [_D] * 7 +
+                                        [_D, _D, ctypes.c_uint32, _D, _D])
+        L.orc_heston_philox.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 2 + [_D] * 7 +
+                                        [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _D, _D, _D])
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_D)
+
+
+def _f(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(_F)
+
+
+# ---- generator ---------------------------------------------------------------
+def philox4x32_10(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*[int(x) & 0xFFFFFFFF for x in ctr])
+    k = (ctypes.c_uint32 * 2)(*[int(x) & 0xFFFFFFFF for x in key])
+    out = (ctypes.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, out)
+    return [int(x) for x in out]
+
+
+def box_muller(a, b):
+    z0, z1 = ctypes.c_double(), ctypes.c_double()
+    lib().orc_box_muller(a, b, ctypes.byref(z0), ctypes.byref(z1))
+    return z0.value, z1.value
+
+
+def normals4(seed, stream, tag, c0, c1, c2):
+    out = (ctypes.c_double * 4)()
+    lib().orc_normals4(seed, stream, tag, c0, c1, c2, out)
+    return list(out)
+
+
+# ---- analytic ----------------------------------------------------------------
+def bs_call(S, X, T, r, sigma):
+    return lib().orc_bs_call(S, X, T, r, sigma)
+
+
+def bs_call_greeks(S, X, T, r, sigma):
+    out = (ctypes.c_double * 3)()
+    lib().orc_bs_call_greeks(S, X, T, r, sigma, out)
+    return list(out)
+
+
+def crr_european_call(S, X, T, r, sigma, steps=2048):
+    return lib().orc_crr_european_call(S, X, T, r, sigma, steps)
+
+
+def crr_american_put(S, K, T, r, sigma, steps=4096):
+    return lib().orc_crr_american_put(S, K, T, r, sigma, steps)
+
+
+def geometric_asian_call(S, X, T, r, sigma, N):
+    return lib().orc_geometric_asian_call(S, X, T, r, sigma, N)
+
+
+# ---- GBM ---------------------------------------------------------------------
+def gbm_buffer(S, X, T, r, sigma, samples):
+    n_opt = len(S)
+    n_paths = samples.shape[-1]
+    S, pS = _d(S)
+    X, pX = _d(X)
+    T, pT = _d(T)
+    sm, psm = _f(samples.reshape(n_opt, n_paths))
+    res, pres = _d(np.zeros(n_opt))
+    conf, pconf = _d(np.zeros(n_opt))
+    lib().orc_gbm_buffer(n_opt, n_paths, r, sigma, pS, pX, pT, psm, pres, pconf)
+    return res, conf
+
+
+def gbm_philox(S, X, T, r, sigma, n_paths, seed, stream=0, greeks=False):
+    n_opt = len(S)
+    S, pS = _d(S)
+    X, pX = _d(X)
+    T, pT = _d(T)
+    res, pres = _d(np.zeros(n_opt))
+    conf, pconf = _d(np.zeros(n_opt))
+    g, pg = _d(np.zeros((n_opt, 6)))
+    lib().orc_gbm_philox(n_opt, n_paths, r, sigma, pS, pX, pT, seed, stream, pres, pconf,
+                         pg if greeks else None)
+    return (res, conf, g) if greeks else (res, conf)
+
+
+# ---- Heston ------------------------------------------------------------------
+Q_VAR_AS_VOL = 1
+Q_RUNNING_VAL2 = 2
+
+
+def _heston_arrays(S, X, T, v0, kappa, theta, xi):
+    arrs = [_d(a) for a in (S, X, T, v0, kappa, theta, xi)]
+    return [a for a, _ in arrs], [p for _, p in arrs]
+
+
+def heston_buffer(S, X, T, v0, kappa, theta, xi, r, rho, n_paths, n_steps, z1, z2, quirks=0):
+    n_opt = len(S)
+    keep, ptrs = _heston_arrays(S, X, T, v0, kappa, theta, xi)
+    z1, pz1 = _d(z1)
+    z2, pz2 = _d(z2)
+    assert z1.size == n_opt * n_paths * n_steps and z2.size == z1.size
+    res, pres = _d(np.zeros(n_opt))
+    conf, pconf = _d(np.zeros(n_opt))
+    lib().orc_heston_buffer(n_opt, n_paths, n_steps, r, rho, *ptrs, pz1, pz2, quirks, pres, pconf)
+    return res, conf
+
+
+def heston_philox(S, X, T, v0, kappa, theta, xi, r, rho, n_paths, n_steps, seed, stream=0, quirks=0,
+                  greeks=False):
+    n_opt = len(S)
+    keep, ptrs = _heston_arrays(S, X, T, v0, kappa, theta, xi)
+    res, pres = _d(np.zeros(n_opt))
+    conf, pconf = _d(np.zeros(n_opt))
+    g, pg = _d(np.zeros((n_opt, 6)))
+    lib().orc_heston_philox(n_opt, n_paths, n_steps, r, rho, *ptrs, seed, stream, quirks, pres, pconf,
+                            pg if greeks else None)
+    return (res, conf, g) if greeks else (res, conf)

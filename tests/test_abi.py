@@ -1,0 +1,78 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry
+point declared in include/*.h, and rejects bad arguments before touching the
+device."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+import mcengine
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b(fsmc_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(mcengine.LIB_PATH):
+        import build
+        build.build()
+    return mcengine.lib()
+
+
+def test_declarations_found():
+    assert len(_declared()) >= 10
+
+
+@pytest.mark.parametrize("name", _declared())
+def test_symbol_exported(lib, name):
+    assert hasattr(lib, name), f"{name} declared in include/ but not exported"
+
+
+def test_status_strings(lib):
+    assert lib.fsmc_status_string(0) == b"ok"
+    assert lib.fsmc_status_string(-1) == b"invalid argument"
+    assert lib.fsmc_status_string(-4) == b"workspace too small"
+
+
+def test_invalid_arguments_rejected_without_device(lib):
+    m = mcengine.mode()
+    p = mcengine.GbmParams(0, 1024, 0.02, 0.3)  # zero options
+    st = lib.fsmc_gbm_european_async(ctypes.byref(p), ctypes.byref(m), *([None] * 7), None, 0, None)
+    assert st == -1
+    assert b"n_options" in lib.fsmc_last_error()
+    assert lib.fsmc_gbm_european_workspace(ctypes.byref(p), ctypes.byref(m)) == 0
+    p = mcengine.GbmParams(4, 1024, 0.02, 0.3)
+    bad = mcengine.mode(precision=7)
+    st = lib.fsmc_gbm_european_async(ctypes.byref(p), ctypes.byref(bad), *([None] * 7), None, 0, None)
+    assert st == -1 and b"precision" in lib.fsmc_last_error()
+    # NULL device pointers are rejected before any launch
+    st = lib.fsmc_gbm_european_async(ctypes.byref(p), ctypes.byref(m), *([None] * 7), None, 0, None)
+    assert st == -1
+    h = mcengine.HestonParams(2, 100, 12, 0.03, -0.7)  # buffer mode needs n_paths % 64 == 0
+    mb = mcengine.mode(rng=mcengine.RNG_BUFFER)
+    st = lib.fsmc_heston_asian_async(ctypes.byref(h), ctypes.byref(mb), *([None] * 12), None, 0, None)
+    assert st == -1 and b"64" in lib.fsmc_last_error()
+    h = mcengine.HestonParams(2, 128, 12, 0.03, -1.5)
+    st = lib.fsmc_heston_asian_async(ctypes.byref(h), ctypes.byref(m), *([None] * 12), None, 0, None)
+    assert st == -1 and b"rho" in lib.fsmc_last_error()
+
+
+def test_workspace_sizes(lib):
+    m = mcengine.mode()
+    p = mcengine.GbmParams(1024, 262144, 0.02, 0.3)
+    ws = lib.fsmc_gbm_european_workspace(ctypes.byref(p), ctypes.byref(m))
+    assert 0 < ws < 64 << 20
+    h = mcengine.HestonParams(64, 131072, 365, 0.03, -0.7)
+    ws = lib.fsmc_heston_asian_workspace(ctypes.byref(h), ctypes.byref(m))
+    assert ws > 0

@@ -1,0 +1,242 @@
+"""Pins of the CPU oracle (no GPU).
+
+The reference ships no golden vectors or tests, so each oracle function is
+pinned here by what the mathematics fixes: published known-answer vectors,
+textbook values, closed forms, exact martingale identities and conditional
+closed-form quadrature -- each chosen so that a dropped term, a wrong sign or
+a swapped operand in the oracle fails at least one check.
+"""
+import math
+
+import numpy as np
+import pytest
+from numpy.polynomial.hermite_e import hermegauss
+from scipy.stats import norm
+
+import oracle as orc
+
+
+# ---- Philox4x32-10: Random123 known-answer vectors (kat_vectors) --------------
+@pytest.mark.parametrize("ctr,key,expect", [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+])
+def test_philox_kat(ctr, key, expect):
+    assert orc.philox4x32_10(ctr, key) == expect
+
+
+def test_box_muller_identities():
+    # u2 = 0 -> angle 0: z0 = r, z1 = 0; u1 = 1 (a >> 8 == 2^24 - 1) -> r = 0.
+    z0, z1 = orc.box_muller(0, 0)
+    assert z1 == 0.0 and z0 == pytest.approx(math.sqrt(-2 * math.log(2.0 ** -24)), rel=1e-15)
+    z0, z1 = orc.box_muller(0xFFFFFFFF, 0x40000000)  # u1 = 1, u2 = 1/4
+    assert z0 == 0.0 and z1 == 0.0
+    # u2 = 1/4 -> (cos, sin) = (0, 1): all radius in z1.
+    z0, z1 = orc.box_muller(0x7FFFFFFF, 0x40000000)
+    r = math.sqrt(-2 * math.log(((0x7FFFFFFF >> 8) + 1) * 2.0 ** -24))
+    assert abs(z0) < 1e-15 * r and z1 == pytest.approx(r, rel=1e-15)
+
+
+def test_normals_moments():
+    z = np.array([orc.normals4(7, 0, 2, p, s, 0) for p in range(2000) for s in range(8)]).ravel()
+    n = z.size
+    assert abs(z.mean()) < 4 / math.sqrt(n)
+    assert abs(z.var() - 1) < 4 * math.sqrt(2 / n)
+    assert abs(np.mean(z ** 3)) < 4 * math.sqrt(15 / n)
+    assert abs(np.mean(z ** 4) - 3) < 4 * math.sqrt(96 / n)
+    # different tags / streams are different draws
+    assert orc.normals4(7, 0, 1, 0, 0, 0) != orc.normals4(7, 0, 2, 0, 0, 0)
+    assert orc.normals4(7, 0, 1, 0, 0, 0) != orc.normals4(7, 1, 1, 0, 0, 0)
+
+
+# ---- Black-Scholes and lattices --------------------------------------------------
+def test_bs_textbook():
+    # Hull, Options Futures and Other Derivatives, example 15.6: c = 4.76.
+    assert orc.bs_call(42, 40, 0.5, 0.1, 0.2) == pytest.approx(4.7594, abs=1e-4)
+    assert orc.bs_call(100, 100, 1, 0.05, 0.2) == pytest.approx(10.450583572185565, rel=1e-13)
+    d1 = (math.log(42 / 40) + (0.1 + 0.02) * 0.5) / (0.2 * math.sqrt(0.5))
+    g = orc.bs_call_greeks(42, 40, 0.5, 0.1, 0.2)
+    assert g[0] == pytest.approx(norm.cdf(d1), rel=1e-13)
+    # Greeks against central differences of the price
+    h = 1e-5
+    fd = [(orc.bs_call(42 + h, 40, .5, .1, .2) - orc.bs_call(42 - h, 40, .5, .1, .2)) / (2 * h),
+          (orc.bs_call(42, 40, .5, .1, .2 + h) - orc.bs_call(42, 40, .5, .1, .2 - h)) / (2 * h),
+          (orc.bs_call(42, 40, .5, .1 + h, .2) - orc.bs_call(42, 40, .5, .1 - h, .2)) / (2 * h)]
+    assert np.allclose(g, fd, rtol=1e-7)
+
+
+def test_crr_european_converges_to_bs():
+    for S, X, T, r, s in [(42, 40, .5, .1, .2), (20, 80, 10, .02, .3), (30, 1, .25, .02, .3)]:
+        bs = orc.bs_call(S, X, T, r, s)
+        c2 = orc.crr_european_call(S, X, T, r, s, 2048)
+        c3 = orc.crr_european_call(S, X, T, r, s, 2049)
+        assert abs(c2 - bs) < 2e-3 * max(bs, 1e-2)
+        # CRR error oscillates between odd and even step counts; their mean
+        # converges much faster.
+        assert abs(0.5 * (c2 + c3) - bs) < 5e-4 * max(bs, 1.0)
+
+
+def test_crr_american_put():
+    # Longstaff & Schwartz (2001) Table 1, finite-difference column.
+    assert orc.crr_american_put(36, 40, 1, .06, .2, 4096) == pytest.approx(4.478, abs=1e-2)
+    assert orc.crr_american_put(40, 40, 1, .06, .2, 4096) == pytest.approx(2.314, abs=1e-2)
+    assert orc.crr_american_put(44, 40, 2, .06, .4, 4096) == pytest.approx(5.647, abs=2e-2)
+    # converged to 1e-3 between 4096 and 8192 steps
+    a = orc.crr_american_put(36, 40, 1, .06, .2, 4096)
+    b = orc.crr_american_put(36, 40, 1, .06, .2, 8192)
+    assert abs(a - b) < 1e-3
+    # r = 0: early exercise is worthless, American put = European put (parity)
+    eu_put = orc.bs_call(36, 40, 1, 0.0, .2) - 36 + 40
+    assert orc.crr_american_put(36, 40, 1, 0.0, .2, 4096) == pytest.approx(eu_put, abs=2e-3)
+    # American >= European
+    eu_put6 = orc.bs_call(36, 40, 1, .06, .2) - 36 + 40 * math.exp(-.06)
+    assert orc.crr_american_put(36, 40, 1, .06, .2, 4096) > eu_put6 + 0.5
+
+
+def test_geometric_asian_closed_form():
+    # N = 1 monitoring date is a European call.
+    assert orc.geometric_asian_call(100, 95, 1, .03, .25, 1) == pytest.approx(
+        orc.bs_call(100, 95, 1, .03, .25), rel=1e-13)
+
+
+# ---- GBM oracle --------------------------------------------------------------------
+def test_gbm_oracle_vs_black_scholes():
+    rng = np.random.default_rng(1)
+    n = 16
+    S = rng.uniform(5, 30, n)
+    X = rng.uniform(1, 100, n)
+    T = rng.uniform(.25, 10, n)
+    res, conf, g = orc.gbm_philox(S, X, T, .02, .3, 1 << 16, seed=11, greeks=True)
+    se = conf / 1.96
+    for i in range(n):
+        bs = orc.bs_call(S[i], X[i], T[i], .02, .3)
+        assert abs(res[i] - bs) < 4 * se[i] + 1e-8
+        gb = orc.bs_call_greeks(S[i], X[i], T[i], .02, .3)
+        for k in range(3):
+            assert abs(g[i, k] - gb[k]) < 4.5 * g[i, 3 + k] + 1e-7
+
+
+def test_gbm_buffer_oracle_matches_philox_oracle():
+    # The buffer estimator fed with the Philox normals (converted to log2
+    # samples) reproduces the Philox estimator: same arithmetic, same draws.
+    S, X, T = np.array([20.0, 7.0]), np.array([18.0, 9.0]), np.array([2.0, .5])
+    n_paths, r, sig = 4096, .02, .3
+    z = np.array([[orc.normals4(5, 0, 1, q, 0, o)[j] for q in range(n_paths // 4) for j in range(4)]
+                  for o in range(2)])
+    L2E = 1.4426950408889634074
+    mu = (L2E * (r - .5 * sig * sig) * T)[:, None]
+    vol = (L2E * sig * np.sqrt(T))[:, None]
+    samples = (mu + vol * z).astype(np.float32)
+    rb, cb = orc.gbm_buffer(S, X, T, r, sig, samples)
+    rp, cp = orc.gbm_philox(S, X, T, r, sig, n_paths, seed=5)
+    # only the float rounding of the samples differs
+    assert np.allclose(rb, rp, rtol=2e-6)
+    assert np.allclose(cb, cp, rtol=2e-5)
+
+
+# ---- Heston oracle -------------------------------------------------------------------
+HP = dict(S=[100.0], X=[100.0], T=[1.0], v0=[0.09], kappa=[2.0], theta=[0.04], xi=[1.0])
+
+
+def _heston_n2_quadrature(S0, K, T, r, v0, kappa, theta, xi, rho, var_as_vol=False, n=160):
+    """E[e^{-rT} max((S1+S2)/2 - K, 0)] for two Euler steps: the inner
+    expectation over the second spot normal is a Black-Scholes-type closed
+    form; the outer one over (z1_0, z2_0) is 2-D Gauss-Hermite."""
+    x, w = hermegauss(n)
+    w = w / math.sqrt(2 * math.pi)
+    a0, b0 = np.meshgrid(x, x, indexing="ij")
+    W = np.outer(w, w)
+    dt = T / 2
+    s0 = v0 if var_as_vol else math.sqrt(v0)
+    S1 = S0 * np.exp((r - .5 * s0 * s0) * dt + s0 * math.sqrt(dt) * a0)
+    zc = rho * a0 + math.sqrt(1 - rho * rho) * b0
+    v1 = v0 + kappa * dt * (theta - v0) + xi * math.sqrt(v0 * dt) * zc
+    v1p = np.maximum(v1, 0.0)
+    s1 = v1p if var_as_vol else np.sqrt(v1p)
+    m = (r - .5 * s1 * s1) * dt
+    s = s1 * math.sqrt(dt)
+    Kp = 2 * K - S1
+    fwd = S1 * np.exp(r * dt)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        d2 = (np.log(S1 / np.where(Kp > 0, Kp, 1.0)) + m) / np.where(s > 0, s, 1.0)
+        bs = fwd * norm.cdf(d2 + s) - Kp * norm.cdf(d2)
+    inner = np.where(Kp <= 0, fwd - Kp, np.where(s > 0, bs, np.maximum(fwd - Kp, 0.0)))
+    return math.exp(-r * T) * 0.5 * float(np.sum(W * inner))
+
+
+@pytest.mark.parametrize("var_as_vol", [False, True])
+def test_heston_oracle_two_step_quadrature(var_as_vol):
+    r, rho = 0.03, -0.7
+    q = orc.Q_VAR_AS_VOL if var_as_vol else 0
+    res, conf = orc.heston_philox(r=r, rho=rho, n_paths=1 << 22, n_steps=2, seed=3, quirks=q, **HP)
+    se = conf[0] / 1.96
+    exact = _heston_n2_quadrature(100, 100, 1, r, .09, 2, .04, 1.0, rho, var_as_vol)
+    assert abs(res[0] - exact) < 4 * se
+    # the pin discriminates the correlation sign and the variance coefficients:
+    # each alternative lies more than 8 SE away, so it would fail the 4-SE check
+    for alt in (dict(rho=+0.7), dict(kappa=0.0), dict(xi=0.0)):
+        kw = dict(S0=100, K=100, T=1, r=r, v0=.09, kappa=2, theta=.04, xi=1.0, rho=rho)
+        kw.update(alt)
+        assert abs(_heston_n2_quadrature(var_as_vol=var_as_vol, **kw) - exact) > 8 * se
+
+
+def test_heston_oracle_martingale_mean():
+    # K = 0: the call pays the average, whose expectation is exact:
+    # e^{-rT} S0 (1/N) sum_k e^{r k dt} (the log-Euler step is a martingale).
+    N, r = 12, 0.05
+    for q in (0, orc.Q_VAR_AS_VOL):
+        res, conf = orc.heston_philox([100.0], [0.0], [2.0], [.04], [2.0], [.04], [.3], r, -.7, 1 << 17, N,
+                                      seed=9, quirks=q)
+        dt = 2.0 / N
+        exact = math.exp(-r * 2) * 100 * np.mean(np.exp(r * dt * np.arange(1, N + 1)))
+        assert abs(res[0] - exact) < 4 * conf[0] / 1.96
+
+
+def test_heston_oracle_one_step_is_black_scholes():
+    # one monitoring date and xi = 0: a Black-Scholes call with sigma = sqrt(v0)
+    res, conf = orc.heston_philox([100.0], [95.0], [1.0], [.0625], [3.0], [.5], [0.0], .03, -.7, 1 << 20, 1,
+                                  seed=4)
+    assert abs(res[0] - orc.bs_call(100, 95, 1, .03, .25)) < 4 * conf[0] / 1.96
+
+
+def test_heston_buffer_oracle_matches_philox_oracle():
+    # Buffer (reference-layout) estimator fed with the Philox draws equals the
+    # Philox estimator to rounding: checks the [opt][chunk][step][lane] layout.
+    n_paths, n_steps = 128, 6
+    P = dict(S=[100.0, 90.0], X=[100.0, 95.0], T=[1.0, .5], v0=[.04, .05], kappa=[2.0, 1.0],
+             theta=[.04, .06], xi=[.3, .5])
+    z1 = np.zeros((2, n_paths // 64, n_steps, 64))
+    z2 = np.zeros_like(z1)
+    for o in range(2):
+        for p in range(n_paths):
+            for k in range(0, n_steps, 2):
+                n = orc.normals4(21, 0, 2, p, k // 2, o)
+                z1[o, p // 64, k, p % 64], z2[o, p // 64, k, p % 64] = n[0], n[1]
+                z1[o, p // 64, k + 1, p % 64], z2[o, p // 64, k + 1, p % 64] = n[2], n[3]
+    rb, cb = orc.heston_buffer(r=.03, rho=-.7, n_paths=n_paths, n_steps=n_steps, z1=z1, z2=z2, **P)
+    rp, cp = orc.heston_philox(r=.03, rho=-.7, n_paths=n_paths, n_steps=n_steps, seed=21, **P)
+    assert np.allclose(rb, rp, rtol=1e-12) and np.allclose(cb, cp, rtol=1e-10)
+
+
+def test_heston_oracle_greeks_vs_fd():
+    # Tangent-mode pathwise Greeks against central differences of the same
+    # common-random-number estimator.
+    n_paths, N, r = 1 << 14, 24, 0.03
+    P = dict(S=[100.0], X=[100.0], T=[1.0], v0=[.04], kappa=[2.0], theta=[.04], xi=[.3])
+    res, conf, g = orc.heston_philox(r=r, rho=-.7, n_paths=n_paths, n_steps=N, seed=5, greeks=True, **P)
+
+    def price(**kw):
+        q = dict(P, r=r)
+        q.update(kw)
+        return orc.heston_philox(rho=-.7, n_paths=n_paths, n_steps=N, seed=5, **q)[0][0]
+
+    h = 1e-4
+    fd_delta = (price(S=[100 + h]) - price(S=[100 - h])) / (2 * h)
+    s0 = 0.2
+    fd_vega = (price(v0=[(s0 + h) ** 2]) - price(v0=[(s0 - h) ** 2])) / (2 * h)
+    fd_rho = (price(r=r + h) - price(r=r - h)) / (2 * h)
+    assert g[0, 0] == pytest.approx(fd_delta, rel=2e-3)
+    assert g[0, 1] == pytest.approx(fd_vega, rel=2e-3)
+    assert g[0, 2] == pytest.approx(fd_rho, rel=2e-3)
