@@ -1,6 +1,8 @@
-"""Builds libfsmc.so: every HIP translation unit under csrc/ (kernels + C ABI)
-and the C++ drop-in entry points under host/, compiled for gfx950 with hipcc
-and linked into one shared library in lib/.
+"""Builds libfsmc.so: every HIP translation unit under csrc/ (kernels + C ABI,
+hipcc for gfx950) and the C++ drop-in entry points under host/dropin_*.cpp
+(g++), linked into one shared library in lib/; then the sample-compatible
+command lines under host/cli_*.cpp into bin/, each in a double (_dp, the
+samples' -DUSE_DOUBLE) and a float (_sp) build.
 
 Run directly (``python build.py``) or through ``__graft_entry__.build()``.
 """
@@ -16,6 +18,9 @@ HOST = os.path.join(HERE, "host")
 OBJ = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libfsmc.so")
+BIN_DIR = os.path.join(HERE, "bin")
+GXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{os.path.join(os.path.dirname(HERE), 'include')}"]
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -29,7 +34,7 @@ def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
     cmd = [HIPCC] + COMMON + ["-c", src, "-o", obj]
     if src.endswith(".cpp"):
-        cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", src, "-o", obj]
+        cmd = [GXX] + HOST_FLAGS + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     # hipcc's host pass warns that it does not know the packed-fp32 feature.
     noise = "packed-fp32-ops' is not a recognized feature"
@@ -42,7 +47,17 @@ def _compile(src):
 
 
 def sources():
-    return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(HOST, "*.cpp")))
+    return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(HOST, "dropin_*.cpp")))
+
+
+def _cli(src, suffix, defines):
+    name = os.path.basename(src)[len("cli_"):-len(".cpp")]
+    out = os.path.join(BIN_DIR, f"{name}_{suffix}")
+    cmd = [GXX] + HOST_FLAGS + defines + [src, "-o", out, f"-L{LIB_DIR}", "-lfsmc", "-Wl,-rpath,$ORIGIN/../lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src}:\n{r.stderr}")
+    return out
 
 
 def build(verbose=False):
@@ -57,8 +72,13 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, LIB)
+    os.makedirs(BIN_DIR, exist_ok=True)
+    clis = []
+    for src in sorted(glob.glob(os.path.join(HOST, "cli_*.cpp"))):
+        clis.append(_cli(src, "dp", ["-DUSE_DOUBLE"]))
+        clis.append(_cli(src, "sp", []))
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources")
+        print(f"built {LIB} from {len(srcs)} sources and {len(clis)} command lines")
     return LIB
 
 

@@ -151,6 +151,25 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
   }
 }
 
+// Precision-dependent exponential and division.  FP32 uses the hardware
+// v_exp_f32 (base 2, so exponents are pre-scaled by kE = log2 e) and
+// v_rcp_f32; FP64 uses the full-precision library exp and IEEE division, so
+// its results follow the CPU oracle to rounding.
+template <typename Real>
+struct MathOps;
+template <>
+struct MathOps<float> {
+  static constexpr float kE = 1.4426950408889634f;
+  __device__ static float ex(float x) { return __builtin_amdgcn_exp2f(x); }
+  __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+};
+template <>
+struct MathOps<double> {
+  static constexpr double kE = 1.0;
+  __device__ static double ex(double x) { return exp(x); }
+  __device__ static double div(double a, double b) { return a / b; }
+};
+
 // exp2 in the working precision: hardware v_exp_f32 for FP32.
 __device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ double fexp2(double x) { return exp2(x); }

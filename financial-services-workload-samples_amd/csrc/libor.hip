@@ -1,0 +1,366 @@
+// LIBOR market model swaption portfolio with pathwise adjoint deltas
+// (the liborSwaptionGreeks sample, after Giles).  One path per lane:
+//
+//  forward   path_calc_b1, liborSwaptionGreeks.cpp:34-82 -- 40 log-Euler steps
+//            of the 80 forward rates under the spot measure, rates in
+//            registers; every updated rate is written to the trajectory
+//            buffer (triangular rows, [slot][path] so that a wavefront's
+//            stores and loads are coalesced);
+//  portfolio portfolio_b, :120-211 -- 15 payer swaptions and the reverse
+//            sweep of the annuity/discount recursion (in registers);
+//  adjoint   path_calc_b2, :86-117 -- 40 reverse steps reading the
+//            trajectory back one row at a time (the whole row is prefetched
+//            before the step so its latency is paid once per row).
+//
+// Each path writes its value and 80 deltas to a [81][path] buffer; a second
+// kernel reduces them in FP64 (mean and standard error per quantity).
+//
+// FSMC_QUIRK_LIBOR_INPLACE_ADJ reproduces the sample computing L_b in place
+// over L (:173-184): B_b[m-1] is then divided by 1 + delta*L_b[n] instead of
+// 1 + delta*L[n].
+#include "fsmc_device.hpp"
+#include "fsmc_host.hpp"
+
+namespace fsmc {
+namespace {
+
+constexpr int NN = 80;    // forward rates           (params.h:27)
+constexpr int NMAT = 40;  // simulation steps        (params.h:28)
+constexpr int NOPT = 15;  // swaptions               (params.h:30)
+constexpr int kBlock = 128;
+constexpr double kLog2e = 1.4426950408889634074;
+
+// Portfolio of params.h:38-40.  The sample initialises its REAL swap rates
+// from float literals, so the FP64 build sees the float-rounded values.
+__host__ __device__ constexpr int mat_of(int j) {
+  return j < 3 ? 4 : j < 6 ? 8 : j < 9 ? 20 : j < 12 ? 28 : 40;
+}
+__host__ __device__ constexpr float rate_of(int j) { return (j % 3) == 0 ? .045f : (j % 3) == 1 ? .05f : .055f; }
+
+// Trajectory row r (1..40) holds L(r, i) for i = r..79; its first slot:
+__host__ __device__ constexpr int row_off(int r) { return 80 * (r - 1) - (r - 1) * r / 2; }
+constexpr int kSlots = row_off(NMAT + 1);  // 2380
+static_assert(kSlots == 2380, "trajectory size");
+
+struct LiborArgs {
+  int n_paths;
+  double L0, lambda;
+  const double* z;  // buffer mode: [n_paths][40] (liborSwaptionGreeks.cpp:386, z[path*NMAT + n])
+  uint32_t k0, k1, w;
+  void* traj;  // Real [kSlots][n_paths]
+  void* out;   // Real [81][n_paths]: value, then dV/dL0_i
+};
+
+template <typename Real, bool kBuffer>
+__device__ __forceinline__ Real normal_at(const LiborArgs& a, int p, int n, Real (&zq)[4]) {
+  if (kBuffer) return (Real)a.z[(size_t)p * NMAT + n];
+  if ((n & 3) == 0) normals4<Real>(U4{(uint32_t)p, (uint32_t)(n >> 2), 0u, a.w}, a.k0, a.k1, zq[0], zq[1], zq[2], zq[3]);
+  const int j = n & 3;
+  return j == 0 ? zq[0] : j == 1 ? zq[1] : j == 2 ? zq[2] : zq[3];
+}
+
+template <typename Real, bool kBuffer, bool kInplace>
+__global__ __launch_bounds__(kBlock) void libor_path_kernel(LiborArgs a) {
+  using O = MathOps<Real>;
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= a.n_paths) return;  // no workgroup synchronisation below
+  const size_t P = (size_t)a.n_paths;
+  Real* traj = (Real*)a.traj + p;
+  const Real delta = Real(0.25), sqrt_delta = Real(0.5);
+  const Real lam = (Real)a.lambda;
+  const Real con1 = delta * lam;
+  const Real L0 = (Real)a.L0;
+
+  // ---- forward: path_calc_b1 ------------------------------------------------
+  Real L[NN];
+#pragma unroll
+  for (int i = 0; i < NN; ++i) L[i] = L0;
+  Real zq[4] = {0, 0, 0, 0};
+  for (int n = 0; n < NMAT; ++n) {
+    const Real sqez = sqrt_delta * normal_at<Real, kBuffer>(a, p, n, zq);
+    // vrat = exp(con1*v + lam*(sqez - con1/2)), exponent pre-scaled for exp2
+    const Real ce = con1 * O::kE;
+    const Real c2 = lam * (sqez - Real(0.5) * con1) * O::kE;
+    Real v = 0;
+    const int rb = row_off(n + 1) - (n + 1);  // slot of entry i is rb + i (i > n)
+#pragma unroll
+    for (int i = 1; i < NN; ++i) {
+      if (i > n) {
+        const Real Li = L[i];
+        v += con1 * O::div(Li, Real(1) + delta * Li);
+        L[i] = Li * O::ex(ce * v + c2);
+        traj[(size_t)(rb + i) * P] = L[i];
+      }
+    }
+  }
+
+  // ---- portfolio_b ---------------------------------------------------------------
+  Real B[NMAT], Bb[NMAT], Sb[NMAT];
+  Real b = 1, s = 0, val = 0;
+#pragma unroll
+  for (int m = 0; m < NMAT; ++m) {
+    b = O::div(b, Real(1) + delta * L[m + NMAT]);
+    s = s + delta * b;
+    B[m] = b;
+    Bb[m] = 0;
+    Sb[m] = 0;
+#pragma unroll
+    for (int j = 0; j < NOPT; ++j) {
+      if (mat_of(j) - 1 == m) {
+        const Real K = (Real)rate_of(j);
+        const Real sw = b + K * s - Real(1);
+        if (sw < Real(0)) {
+          val += Real(-100) * sw;
+          Sb[m] += Real(-100) * K;
+          Bb[m] += Real(-100);
+        }
+      }
+    }
+  }
+  Real Lb[NN];
+#pragma unroll
+  for (int m = NMAT - 1; m >= 0; --m) {
+    const int n = m + NMAT;
+    Bb[m] += delta * Sb[m];
+    const Real dn = Real(1) + delta * L[n];
+    Lb[n] = O::div(-Bb[m] * B[m] * delta, dn);
+    if (m > 0) {
+      Sb[m - 1] += Sb[m];
+      Bb[m - 1] += O::div(Bb[m], kInplace ? Real(1) + delta * Lb[n] : dn);
+    }
+  }
+  b = 1;
+#pragma unroll
+  for (int n = 0; n < NMAT; ++n) b = O::div(b, Real(1) + delta * L[n]);
+  val = b * val;
+#pragma unroll
+  for (int n = 0; n < NMAT; ++n) Lb[n] = O::div(-val * delta, Real(1) + delta * L[n]);
+#pragma unroll
+  for (int n = NMAT; n < NN; ++n) Lb[n] = b * Lb[n];
+
+  // ---- adjoint: path_calc_b2 (L holds row n+1, Lb the adjoints) ---------------------
+  for (int n = NMAT - 1; n >= 0; --n) {
+    Real R[NN];  // row n, prefetched: entries i > n (clamped addresses below)
+    if (n > 0) {
+      // row n holds entries i >= n at slots row_off(n) + i - n; entries
+      // i < n are not needed and read the (valid) slot of i = n instead
+      const int rb = row_off(n) - n;
+#pragma unroll
+      for (int i = 1; i < NN; ++i) R[i] = traj[(size_t)(rb + (i > n ? i : n)) * P];
+    } else {
+#pragma unroll
+      for (int i = 1; i < NN; ++i) R[i] = L0;
+    }
+    Real v1 = 0;
+#pragma unroll
+    for (int i = NN - 1; i >= 1; --i) {
+      if (i > n) {
+        const Real Ln = L[i], Lc = R[i];
+        v1 += lam * Ln * Lb[i];
+        const Real faci = O::div(delta, Real(1) + delta * Lc);
+        Lb[i] = Lb[i] * O::div(Ln, Lc) + v1 * lam * faci * faci;
+        L[i] = Lc;
+      }
+    }
+  }
+
+  Real* out = (Real*)a.out + p;
+  out[0] = val;
+#pragma unroll
+  for (int i = 0; i < NN; ++i) out[(size_t)(i + 1) * P] = Lb[i];
+}
+
+// Sum and sum of squares of quantity q over a strided set of paths, in FP64.
+template <typename Real>
+__global__ __launch_bounds__(256) void libor_reduce_kernel(int n_paths, const Real* out, double* partials) {
+  __shared__ double lds[2 * 4];
+  const int q = blockIdx.y;
+  const Real* src = out + (size_t)q * n_paths;
+  double s[2] = {0.0, 0.0};
+  for (int p = blockIdx.x * 256 + threadIdx.x; p < n_paths; p += gridDim.x * 256) {
+    const double x = (double)src[p];
+    s[0] += x;
+    s[1] += x * x;
+  }
+  block_sum<2>(s, lds);
+  if (threadIdx.x == 0) {
+    double* d = partials + ((size_t)q * gridDim.x + blockIdx.x) * 2;
+    d[0] = s[0];
+    d[1] = s[1];
+  }
+}
+
+__global__ void libor_finalize_kernel(int n_paths, int nchunks, const double* partials, double* mean, double* se) {
+  const int q = threadIdx.x;
+  if (q > NN) return;
+  double s0 = 0.0, s1 = 0.0;
+  for (int c = 0; c < nchunks; ++c) {
+    s0 += partials[((size_t)q * nchunks + c) * 2];
+    s1 += partials[((size_t)q * nchunks + c) * 2 + 1];
+  }
+  const double N = (double)n_paths;
+  mean[q] = s0 / N;
+  double var = (N * s1 - s0 * s0) / (N * (N - 1.0));
+  se[q] = sqrt(var > 0.0 ? var : 0.0) / sqrt(N);
+}
+
+// Normals of the LIBOR Philox layout written out as a [n_paths][40] buffer,
+// scaled to N(mean, stddev): the device side of the generateRand seam
+// (mklRng.h:9).  A buffer-mode run on this z equals the Philox-mode run.
+__global__ void libor_normals_kernel(int n_paths, double mean, double stddev, uint32_t k0, uint32_t k1, uint32_t w,
+                                     double* z) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;  // one Philox call each
+  const int p = t / (NMAT / 4), q = t % (NMAT / 4);
+  if (p >= n_paths) return;
+  double n[4];
+  normals4<double>(U4{(uint32_t)p, (uint32_t)q, 0u, w}, k0, k1, n[0], n[1], n[2], n[3]);
+  double* dst = z + (size_t)p * NMAT + 4 * q;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) dst[j] = mean + stddev * n[j];
+}
+
+constexpr int kReduceChunks = 64;
+
+struct LiborPlan {
+  size_t real, traj_bytes, out_bytes, part_bytes, res_bytes, ws;
+};
+
+LiborPlan libor_plan(const fsmc_libor_params* p, const fsmc_mode* m) {
+  LiborPlan l;
+  l.real = m->precision == FSMC_FP64 ? sizeof(double) : sizeof(float);
+  const size_t P = (size_t)p->n_paths;
+  l.traj_bytes = align_up((size_t)kSlots * P * l.real, 256);
+  l.out_bytes = align_up((size_t)(NN + 1) * P * l.real, 256);
+  l.part_bytes = align_up((size_t)(NN + 1) * kReduceChunks * 2 * sizeof(double), 256);
+  l.res_bytes = align_up(2 * (NN + 1) * sizeof(double), 256);
+  l.ws = l.traj_bytes + l.out_bytes + l.part_bytes + l.res_bytes;
+  return l;
+}
+
+int libor_validate(const fsmc_libor_params* p, const fsmc_mode* m) {
+  FSMC_REQUIRE(p != nullptr, "params is NULL");
+  FSMC_TRY(validate_mode(m, true));
+  FSMC_REQUIRE(p->n_paths >= 2, "n_paths %d < 2", p->n_paths);
+  FSMC_REQUIRE(p->L0 > 0.0, "L0 must be > 0");
+  FSMC_REQUIRE(p->lambda >= 0.0, "lambda must be >= 0");
+  return FSMC_OK;
+}
+
+template <typename Real, bool B, bool Q>
+void launch(const LiborArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((libor_path_kernel<Real, B, Q>), dim3((a.n_paths + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     a);
+}
+
+template <typename Real>
+void launch_all(const LiborArgs& a, bool buffer, bool q, hipStream_t s) {
+  if (buffer) q ? launch<Real, true, true>(a, s) : launch<Real, true, false>(a, s);
+  else q ? launch<Real, false, true>(a, s) : launch<Real, false, false>(a, s);
+}
+
+}  // namespace
+}  // namespace fsmc
+
+using namespace fsmc;
+
+extern "C" {
+
+size_t fsmc_libor_workspace(const fsmc_libor_params* p, const fsmc_mode* m) {
+  if (libor_validate(p, m) != FSMC_OK) return 0;
+  return libor_plan(p, m).ws;
+}
+
+int fsmc_libor_swaption_greeks_async(const fsmc_libor_params* p, const fsmc_mode* m, const double* d_z,
+                                     double* d_value, double* d_deltas, void* d_ws, size_t ws_bytes,
+                                     void* stream) {
+  FSMC_TRY(libor_validate(p, m));
+  FSMC_REQUIRE(d_value && d_deltas, "NULL device pointer");
+  const bool buffer = m->rng == FSMC_RNG_BUFFER;
+  FSMC_REQUIRE(!buffer || d_z, "buffer mode needs z");
+  const LiborPlan l = libor_plan(p, m);
+  if (d_ws == nullptr || ws_bytes < l.ws) {
+    set_error("workspace %zu bytes < required %zu", ws_bytes, l.ws);
+    return FSMC_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)d_ws;
+  LiborArgs a;
+  a.n_paths = p->n_paths;
+  a.L0 = p->L0;
+  a.lambda = p->lambda;
+  a.z = d_z;
+  a.k0 = (uint32_t)m->seed;
+  a.k1 = (uint32_t)(m->seed >> 32);
+  a.w = counter_w(kTagLibor, m->stream);
+  a.traj = ws;
+  a.out = ws + l.traj_bytes;
+  double* part = (double*)(ws + l.traj_bytes + l.out_bytes);
+  double* res = (double*)(ws + l.traj_bytes + l.out_bytes + l.part_bytes);
+  const bool q = (m->quirks & FSMC_QUIRK_LIBOR_INPLACE_ADJ) != 0;
+  const bool f64 = m->precision == FSMC_FP64;
+  if (f64) launch_all<double>(a, buffer, q, s);
+  else launch_all<float>(a, buffer, q, s);
+  FSMC_HIP(hipGetLastError());
+  if (f64)
+    hipLaunchKernelGGL(libor_reduce_kernel<double>, dim3(kReduceChunks, NN + 1), dim3(256), 0, s, p->n_paths,
+                       (const double*)a.out, part);
+  else
+    hipLaunchKernelGGL(libor_reduce_kernel<float>, dim3(kReduceChunks, NN + 1), dim3(256), 0, s, p->n_paths,
+                       (const float*)a.out, part);
+  FSMC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(libor_finalize_kernel, dim3(1), dim3(128), 0, s, p->n_paths, kReduceChunks, part, res,
+                     res + NN + 1);
+  FSMC_HIP(hipGetLastError());
+  // value, its standard error, the 80 deltas and their standard errors
+  FSMC_HIP(hipMemcpyAsync(d_value, res, sizeof(double), hipMemcpyDeviceToDevice, s));
+  FSMC_HIP(hipMemcpyAsync(d_value + 1, res + NN + 1, sizeof(double), hipMemcpyDeviceToDevice, s));
+  FSMC_HIP(hipMemcpyAsync(d_deltas, res + 1, NN * sizeof(double), hipMemcpyDeviceToDevice, s));
+  FSMC_HIP(hipMemcpyAsync(d_deltas + NN, res + NN + 2, NN * sizeof(double), hipMemcpyDeviceToDevice, s));
+  return FSMC_OK;
+}
+
+int fsmc_libor_normals(fsmc_ctx* ctx, int n_paths, const fsmc_mode* m, double mean, double stddev, double* z) {
+  FSMC_REQUIRE(ctx != nullptr && z != nullptr, "NULL argument");
+  FSMC_TRY(validate_mode(m, false));
+  FSMC_REQUIRE(n_paths >= 1, "n_paths %d < 1", n_paths);
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const size_t bytes = (size_t)n_paths * NMAT * sizeof(double);
+  FSMC_TRY(ctx_prepare(ctx, 0, bytes, 0, 0));
+  hipStream_t s = ctx->stream;
+  const int threads = n_paths * (NMAT / 4);
+  hipLaunchKernelGGL(libor_normals_kernel, dim3((threads + 255) / 256), dim3(256), 0, s, n_paths, mean, stddev,
+                     (uint32_t)m->seed, (uint32_t)(m->seed >> 32), counter_w(kTagLibor, m->stream),
+                     (double*)ctx->normals.ptr);
+  FSMC_HIP(hipGetLastError());
+  FSMC_HIP(hipMemcpyAsync(z, ctx->normals.ptr, bytes, hipMemcpyDeviceToHost, s));
+  FSMC_HIP(hipStreamSynchronize(s));
+  return FSMC_OK;
+}
+
+int fsmc_libor_swaption_greeks(fsmc_ctx* ctx, const fsmc_libor_params* p, const fsmc_mode* m, const double* z,
+                               double* value, double* deltas) {
+  FSMC_REQUIRE(ctx != nullptr, "ctx is NULL");
+  FSMC_TRY(libor_validate(p, m));
+  FSMC_REQUIRE(value && deltas, "NULL host pointer");
+  const bool buffer = m->rng == FSMC_RNG_BUFFER;
+  FSMC_REQUIRE(!buffer || z, "buffer mode needs z");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const size_t zn = buffer ? (size_t)p->n_paths * NMAT : 0;
+  FSMC_TRY(ctx_prepare(ctx, 0, zn * sizeof(double), (2 + 2 * NN) * sizeof(double), libor_plan(p, m).ws));
+  hipStream_t s = ctx->stream;
+  if (buffer) FSMC_HIP(hipMemcpyAsync(ctx->normals.ptr, z, zn * sizeof(double), hipMemcpyHostToDevice, s));
+  double* dv = (double*)ctx->outputs.ptr;
+  double* dd = dv + 2;
+  int st = fsmc_libor_swaption_greeks_async(p, m, (const double*)ctx->normals.ptr, dv, dd, ctx->workspace.ptr,
+                                            ctx->workspace.bytes, s);
+  if (st != FSMC_OK) {
+    (void)hipStreamSynchronize(s);
+    return st;
+  }
+  FSMC_HIP(hipMemcpyAsync(value, dv, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  FSMC_HIP(hipMemcpyAsync(deltas, dd, 2 * NN * sizeof(double), hipMemcpyDeviceToHost, s));
+  FSMC_HIP(hipStreamSynchronize(s));
+  return FSMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,751 @@
+// Longstaff-Schwartz American puts.
+//
+// Batched "correct" LSMC (many options, uneven step and path counts):
+//   * lsmc_paths_kernel: one path per lane per round, GBM in the working
+//     precision, the whole path stored to a per-option FP32/FP64 matrix
+//     [date][path] in HBM (the backward sweep revisits every date); the
+//     terminal cash flow e^{-rT} (K - S_T)^+ is kept discounted to time 0.
+//   * lsmc_pass_kernel, once per backward date: applies the exercise decision
+//     of date d with the coefficients fitted at d, then accumulates the
+//     regression of the discounted cash flow on {1, x, x^2, x^3} over the
+//     in-the-money paths of date d-1.  x = 1 + (S/K - 1)/(sigma sqrt(t)) is
+//     an affine map of S/K (same fitted values as the plain monomials of S/K)
+//     that keeps the Gram matrix well conditioned.  In FP32 the Gram matrix
+//     and right-hand side come from v_mfma_f32_4x4x1f32 outer products
+//     (a = (1, x, x^2, x^3), b = (1, x^3, Y, 0) gives all seven moments and
+//     the four right-hand sides); FP64 accumulates on the VALU.  The last
+//     workgroup of each option (device-scope ticket, no waiting) combines the
+//     partial sums in FP64 and solves the 4x4 system by Cholesky in FP64.
+//   * lsmc_final_kernel: price = max(K - S0, mean discounted cash flow).
+// Options are sorted by step count (descending) so the options still active
+// at backward pass j are a prefix of the workgroup table: pass j launches
+// exactly their workgroups.
+//
+// Reference algorithm ("refcompat", MonteCarloAmericanOptions/main.cpp:115-279
+// and utilities.cpp:42-123) for one option: path matrix, lagged quadratic
+// regression with ridge 0.01, cofactor 3x3 inverse, no discounting; same
+// ticket scheme for the per-step solve.
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "fsmc_device.hpp"
+#include "fsmc_host.hpp"
+
+namespace fsmc {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kPPT = 4;  // paths per thread per workgroup
+constexpr int kChunk = kBlock * kPPT;
+constexpr int kNS = 12;  // partial sums: m0..m6, c0..c3, spare
+constexpr double kLog2e = 1.4426950408889634074;
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+struct OptMeta {
+  long long mat_off;  // first element of the option's path matrix (row k-1 = date k)
+  long long cf_off;   // first cash flow
+  int first_block, nblk, steps, paths;
+  double S0, K, sigma, r, T;
+};
+
+struct LsmcArgs {
+  const OptMeta* meta;
+  const int2* table;  // workgroup -> (option, chunk)
+  void* mat;
+  void* cf;
+  double* partials;    // [workgroup][kNS]
+  unsigned* tickets;   // [option]
+  double* beta;        // [2][option][5]: coefficients, valid flag (double-buffered by pass)
+  double* eur;         // [workgroup] European payoff sums
+  int n_options;
+  int pass;
+  uint32_t k0, k1, w;
+};
+
+// Ticket of a finishing workgroup for option o: returns true in every thread
+// of the workgroup that finishes last (its partials and everyone else's are
+// visible to it).  The last workgroup resets the ticket for the next launch.
+__device__ __forceinline__ bool last_of_option(unsigned* ticket, int nblk) {
+  __shared__ unsigned s_ticket;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0)
+    s_ticket = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const bool last = s_ticket == (unsigned)(nblk - 1);
+  if (last) __threadfence();
+  return last;
+}
+
+template <typename Real>
+__device__ __forceinline__ Real pick4(const Real (&z)[4], int j) {
+  return j == 0 ? z[0] : j == 1 ? z[1] : j == 2 ? z[2] : z[3];
+}
+
+// ---------------------------------------------------------------------------
+// Batched correct LSMC
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void lsmc_paths_kernel(LsmcArgs a) {
+  using O = MathOps<Real>;
+  __shared__ double lds[kBlock / 64];
+  const int2 e = a.table[blockIdx.x];
+  const OptMeta m = a.meta[e.x];
+  const double dt = m.T / m.steps;
+  const Real drift = (Real)((m.r - 0.5 * m.sigma * m.sigma) * dt * O::kE);
+  const Real vol = (Real)(m.sigma * sqrt(dt) * O::kE);
+  const Real discT = (Real)exp(-m.r * m.T);
+  const Real K = (Real)m.K;
+  Real* mat = (Real*)a.mat + m.mat_off;
+  Real* cf = (Real*)a.cf + m.cf_off;
+  const size_t P = (size_t)m.paths;
+  double eu[1] = {0.0};
+  for (int q = 0; q < kPPT; ++q) {
+    const int p = e.y * kChunk + q * kBlock + threadIdx.x;
+    if (p >= m.paths) continue;
+    Real S = (Real)m.S0;
+    Real z[4] = {0, 0, 0, 0};
+    for (int j = 0; j < m.steps; ++j) {
+      if ((j & 3) == 0)
+        normals4<Real>(U4{(uint32_t)p, (uint32_t)(j >> 2), (uint32_t)e.x, a.w}, a.k0, a.k1, z[0], z[1], z[2], z[3]);
+      S = S * O::ex(drift + vol * pick4(z, j & 3));
+      mat[(size_t)j * P + p] = S;
+    }
+    const Real pay = K - S > Real(0) ? K - S : Real(0);
+    const Real c = discT * pay;
+    cf[p] = c;
+    eu[0] += (double)c;
+  }
+  block_sum<1>(eu, lds);
+  if (threadIdx.x == 0) a.eur[blockIdx.x] = eu[0];
+}
+
+// Moments of one path for the FP64 (VALU) accumulation.
+__device__ __forceinline__ void accum_valu(double (&s)[kNS], bool itm, double x, double y) {
+  if (!itm) return;
+  double xp = 1.0;
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    s[q] += xp;
+    if (q < 4) s[7 + q] += xp * y;
+    xp *= x;
+  }
+}
+
+// FP32: 64 paths of a wavefront through four v_mfma_f32_4x4x1f32.  In round
+// t the 16 4-lane groups each take the path held by their lane t; lane i of
+// the group supplies a_i = x^i and b_i = (1, x^3, y, 0)[i] of that path, so
+// group g accumulates C_g[i][j] += a_i b_j.
+__device__ __forceinline__ void accum_mfma(v4f& acc, bool itm, float x, float y) {
+  const int lane = threadIdx.x & 63;
+  const int i = lane & 3;
+  const int qb = lane & ~3;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const float xt = __shfl(x, qb + t, 64);
+    const float yt = __shfl(y, qb + t, 64);
+    const int it = __shfl(itm ? 1 : 0, qb + t, 64);
+    const float x2 = xt * xt, x3 = x2 * xt;
+    float av = i == 0 ? 1.0f : i == 1 ? xt : i == 2 ? x2 : x3;
+    float bv = i == 0 ? 1.0f : i == 1 ? x3 : i == 2 ? yt : 0.0f;
+    if (!it) av = 0.0f;
+    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(av, bv, acc, 0, 0, 0);
+  }
+}
+
+// Solves the 4x4 normal equations of option o from its workgroups' partials.
+__device__ void lsmc_solve(const LsmcArgs& a, const OptMeta& m, double* beta_out) {
+  double s[kNS];
+#pragma unroll
+  for (int q = 0; q < kNS; ++q) s[q] = 0.0;
+  for (int b = 0; b < m.nblk; ++b) {
+#pragma unroll
+    for (int q = 0; q < kNS; ++q) s[q] += a.partials[(size_t)(m.first_block + b) * kNS + q];
+  }
+  // Cholesky G = L L^T of G_ij = m_{i+j}; a pivot that is not clearly
+  // positive (fewer than 4 ITM paths, or a singular basis) disables exercise
+  // at this date.  Fully unrolled so L stays in registers.
+  double L[4][4] = {{0}};
+  bool ok = s[0] >= 4.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      double v = s[i + j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
+      if (i == j) {
+        ok = ok && v > 1e-13 * s[2 * i];
+        L[i][i] = ok ? sqrt(v) : 1.0;
+      } else {
+        L[i][j] = v / L[j][j];
+      }
+    }
+  }
+  double y[4], x[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double v = s[7 + i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) v -= L[i][k] * y[k];
+    y[i] = v / L[i][i];
+  }
+#pragma unroll
+  for (int i = 3; i >= 0; --i) {
+    double v = y[i];
+#pragma unroll
+    for (int k = i + 1; k < 4; ++k) v -= L[k][i] * x[k];
+    x[i] = v / L[i][i];
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) beta_out[i] = ok ? x[i] : 0.0;
+  beta_out[4] = ok ? 1.0 : 0.0;
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
+  constexpr bool kMfma = sizeof(Real) == 4;
+  __shared__ double lds[kNS * (kBlock / 64)];
+  const int2 e = a.table[blockIdx.x];
+  const int o = e.x;
+  const OptMeta m = a.meta[o];
+  const int d = m.steps - a.pass;  // decision date
+  const int k = d - 1;             // regression date
+  const double dt = m.T / m.steps;
+  const double* bin = a.beta + (size_t)((a.pass & 1) * a.n_options + o) * 5;
+  double* bout = a.beta + (size_t)(((a.pass + 1) & 1) * a.n_options + o) * 5;
+  const bool decide = a.pass >= 1 && bin[4] != 0.0;
+  const Real b0 = (Real)bin[0], b1 = (Real)bin[1], b2 = (Real)bin[2], b3 = (Real)bin[3];
+  const Real K = (Real)m.K, invK = (Real)(1.0 / m.K);
+  const Real isw_d = (Real)(1.0 / (m.sigma * sqrt(d * dt))), disc_d = (Real)exp(-m.r * d * dt);
+  const Real isw_k = k >= 1 ? (Real)(1.0 / (m.sigma * sqrt(k * dt))) : Real(0);
+  Real* mat = (Real*)a.mat + m.mat_off;
+  Real* cf = (Real*)a.cf + m.cf_off;
+  const size_t P = (size_t)m.paths;
+  double s[kNS];
+#pragma unroll
+  for (int q = 0; q < kNS; ++q) s[q] = 0.0;
+  v4f acc = {0.f, 0.f, 0.f, 0.f};
+  for (int q = 0; q < kPPT; ++q) {
+    const int p = e.y * kChunk + q * kBlock + threadIdx.x;
+    const bool live = p < m.paths;
+    Real c = live ? cf[p] : Real(0);
+    if (live && decide) {
+      const Real Sd = mat[(size_t)(d - 1) * P + p];
+      if (K - Sd > Real(0)) {
+        const Real x = Real(1) + (Sd * invK - Real(1)) * isw_d;
+        const Real cont = b0 + x * (b1 + x * (b2 + x * b3));
+        const Real ex = (K - Sd) * disc_d;
+        if (ex > cont) {
+          c = ex;
+          cf[p] = c;
+        }
+      }
+    }
+    if (k >= 1) {  // uniform per workgroup
+      bool itm = false;
+      Real x = 0;
+      if (live) {
+        const Real Sk = mat[(size_t)(k - 1) * P + p];
+        itm = K - Sk > Real(0);
+        x = Real(1) + (Sk * invK - Real(1)) * isw_k;
+      }
+      if constexpr (kMfma)
+        accum_mfma(acc, itm, (float)x, (float)c);
+      else
+        accum_valu(s, itm, (double)x, (double)c);
+    }
+  }
+  if (k < 1) return;  // last pass of this option: decision only
+  if constexpr (kMfma) {
+    // lane l holds C_g[r][l & 3] (g = l / 4) in acc[r]; sum the 16 groups
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float v = acc[r];
+      for (int off = 4; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+      acc[r] = v;
+    }
+    const int lane = threadIdx.x & 63;
+    // column 0: m_r; column 1: m_{r+3}; column 2: c_r (lanes 0, 1, 2)
+    double col[4] = {(double)acc[0], (double)acc[1], (double)acc[2], (double)acc[3]};
+    double c0[4], c1[4], c2[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      c0[r] = __shfl(col[r], 0, 64);
+      c1[r] = __shfl(col[r], 1, 64);
+      c2[r] = __shfl(col[r], 2, 64);
+    }
+    if (lane == 0) {
+      for (int r = 0; r < 4; ++r) s[r] = c0[r];
+      for (int r = 1; r < 4; ++r) s[3 + r] = c1[r];
+      for (int r = 0; r < 4; ++r) s[7 + r] = c2[r];
+    }
+  }
+  block_sum<kNS>(s, lds);
+  if (threadIdx.x == 0) {
+    double* dst = a.partials + (size_t)blockIdx.x * kNS;
+    for (int q = 0; q < kNS; ++q) dst[q] = s[q];
+  }
+  if (last_of_option(a.tickets + o, m.nblk)) {
+    if (threadIdx.x == 0) {
+      lsmc_solve(a, m, bout);
+      a.tickets[o] = 0u;
+    }
+  }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void lsmc_final_kernel(LsmcArgs a, double* price2, double* european) {
+  __shared__ double lds[2 * (kBlock / 64)];
+  const int o = blockIdx.x;
+  const OptMeta m = a.meta[o];
+  const Real* cf = (const Real*)a.cf + m.cf_off;
+  double s[2] = {0.0, 0.0};
+  for (int p = threadIdx.x; p < m.paths; p += kBlock) {
+    const double c = (double)cf[p];
+    s[0] += c;
+    s[1] += c * c;
+  }
+  block_sum<2>(s, lds);
+  if (threadIdx.x == 0) {
+    const double N = (double)m.paths;
+    const double mean = s[0] / N;
+    double var = (N * s[1] - s[0] * s[0]) / (N * (N - 1.0));
+    const double now = m.K - m.S0 > 0.0 ? m.K - m.S0 : 0.0;
+    price2[2 * o] = mean > now ? mean : now;
+    price2[2 * o + 1] = sqrt(var > 0.0 ? var : 0.0) / sqrt(N);
+    if (european) {
+      double eu = 0.0;
+      for (int b = 0; b < m.nblk; ++b) eu += a.eur[m.first_block + b];
+      european[o] = eu / N;
+    }
+  }
+}
+
+__global__ void zero_u32_kernel(unsigned* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+__global__ void zero_f64_kernel(double* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Reference algorithm, one option
+// ---------------------------------------------------------------------------
+struct RefArgs {
+  int n_paths, n_steps;
+  double spot, strike;
+  double fwd_l2e, vsdt_l2e;  // main.cpp:55-59, log2 domain as perform_sims
+  const double* z;           // buffer mode: [n_steps][n_paths]
+  uint32_t k0, k1, w;
+  void* a;                   // Real [n_steps + 1][n_paths]
+  void* cf;                  // Real [n_paths]
+  double* partials;          // [workgroup][8]
+  unsigned* ticket;
+  double* beta;              // [2][3] double-buffered by step parity
+  double* out;               // [2]: american, european
+};
+
+template <typename Real, bool kBuffer>
+__global__ __launch_bounds__(kBlock) void ref_paths_kernel(RefArgs r) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= r.n_paths) return;
+  Real* a = (Real*)r.a;
+  const size_t N = (size_t)r.n_paths;
+  Real x = (Real)r.spot;
+  a[p] = x;
+  Real z4[4] = {0, 0, 0, 0};
+  for (int s = 1; s <= r.n_steps; ++s) {
+    Real z;
+    if (kBuffer) {
+      z = (Real)r.z[(size_t)(s - 1) * N + p];
+    } else {
+      const int j = s - 1;
+      if ((j & 3) == 0)
+        normals4<Real>(U4{(uint32_t)p, (uint32_t)(j >> 2), 0u, r.w}, r.k0, r.k1, z4[0], z4[1], z4[2], z4[3]);
+      z = pick4(z4, j & 3);
+    }
+    // utilities.cpp:83: a[s] = a[s-1] * exp2(fwdFactor + volSqrtdt * z)
+    x = x * fexp2((Real)r.fwd_l2e + (Real)r.vsdt_l2e * z);
+    a[(size_t)s * N + p] = x;
+  }
+}
+
+// Step s of main.cpp:131-266: decision with the previous step's betas (or
+// the 1/N-scaled terminal cash flow at s = n_steps - 1), then the eight ITM
+// sums; the last workgroup inverts the ridge-corrected 3x3 moment matrix.
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void ref_step_kernel(RefArgs r, int s) {
+  __shared__ double lds[8 * (kBlock / 64)];
+  const size_t N = (size_t)r.n_paths;
+  const Real* ap = (const Real*)r.a + (size_t)s * N;
+  Real* cf = (Real*)r.cf;
+  const double* bin = r.beta + (s & 1) * 3;
+  double* bout = r.beta + ((s + 1) & 1) * 3;
+  const Real b0 = (Real)bin[0], b1 = (Real)bin[1], b2 = (Real)bin[2];
+  const Real K = (Real)r.strike;
+  const Real avg = (Real)(1.0 / r.n_paths);
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (size_t p = (size_t)blockIdx.x * kBlock + threadIdx.x; p < N; p += (size_t)gridDim.x * kBlock) {
+    const Real x = ap[p];
+    Real c = cf[p];
+    if (s < r.n_steps - 1) {
+      if (x <= K) {
+        const Real reg = b0 + b1 * x + b2 * (x * x);
+        const Real payoff = K - x;
+        if (payoff > reg) c = payoff;
+      }
+    } else {
+      c = (K - x > Real(0) ? K - x : Real(0)) * avg;
+    }
+    cf[p] = c;
+    if (x <= K) {
+      const double xd = (double)x, cd = (double)c, x2 = xd * xd;
+      v[0] += 1.0;
+      v[1] += xd;
+      v[2] += x2;
+      v[3] += x2 * xd;
+      v[4] += x2 * x2;
+      v[5] += cd;
+      v[6] += cd * xd;
+      v[7] += cd * x2;
+    }
+  }
+  block_sum<8>(v, lds);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 8; ++q) r.partials[(size_t)blockIdx.x * 8 + q] = v[q];
+  if (last_of_option(r.ticket, gridDim.x) && threadIdx.x == 0) {
+    double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (unsigned b = 0; b < gridDim.x; ++b)
+      for (int q = 0; q < 8; ++q) t[q] += r.partials[(size_t)b * 8 + q];
+    const double ridge = 0.01;  // main.cpp:244-246
+    const double s0 = t[0] + ridge, s1 = t[1] + ridge, s2 = t[2] + ridge, s3 = t[3] + ridge, s4 = t[4] + ridge;
+    const double x[3][3] = {{s0, s1, s2}, {s1, s2, s3}, {s2, s3, s4}};
+    // utilities.cpp:90-123 cofactor inverse
+    const double det = x[0][0] * x[1][1] * x[2][2] + x[1][0] * x[2][1] * x[0][2] + x[2][0] * x[0][1] * x[1][2] -
+                       x[2][0] * x[1][1] * x[0][2] - x[1][0] * x[0][1] * x[2][2] - x[0][0] * x[2][1] * x[1][2];
+    double inv[3][3];
+    inv[0][0] = (x[1][1] * x[2][2] - x[2][1] * x[1][2]) / det;
+    inv[0][1] = (x[0][2] * x[2][1] - x[0][1] * x[2][2]) / det;
+    inv[0][2] = (x[0][1] * x[1][2] - x[0][2] * x[1][1]) / det;
+    inv[1][0] = (x[1][2] * x[2][0] - x[1][0] * x[2][2]) / det;
+    inv[1][1] = (x[0][0] * x[2][2] - x[0][2] * x[2][0]) / det;
+    inv[1][2] = (x[0][2] * x[1][0] - x[0][0] * x[1][2]) / det;
+    inv[2][0] = (x[1][0] * x[2][1] - x[1][1] * x[2][0]) / det;
+    inv[2][1] = (x[0][1] * x[2][0] - x[0][0] * x[2][1]) / det;
+    inv[2][2] = (x[0][0] * x[1][1] - x[0][1] * x[1][0]) / det;
+    // main.cpp:259-261
+    bout[0] = t[5] * inv[0][0] + t[6] * inv[1][0] + t[7] * inv[2][0];
+    bout[1] = t[5] * inv[0][1] + t[6] * inv[1][1] + t[7] * inv[2][1];
+    bout[2] = t[5] * inv[0][2] + t[6] * inv[1][2] + t[7] * inv[2][2];
+    *r.ticket = 0u;
+  }
+}
+
+// main.cpp:269-279: American = sum(cf)/N, European = sum((K - S_T)^+)/N.
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void ref_final_kernel(RefArgs r) {
+  __shared__ double lds[2 * (kBlock / 64)];
+  const size_t N = (size_t)r.n_paths;
+  const Real* aT = (const Real*)r.a + (size_t)r.n_steps * N;
+  const Real* cf = (const Real*)r.cf;
+  double v[2] = {0.0, 0.0};
+  for (size_t p = threadIdx.x; p < N; p += kBlock) {
+    v[0] += (double)cf[p];
+    const double e = r.strike - (double)aT[p];
+    v[1] += e > 0.0 ? e : 0.0;
+  }
+  block_sum<2>(v, lds);
+  if (threadIdx.x == 0) {
+    r.out[0] = v[0] / r.n_paths;
+    r.out[1] = v[1] / r.n_paths;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct LsmcPlanImpl {
+  int n;
+  int precision;
+  fsmc_mode mode;
+  std::vector<OptMeta> meta;
+  std::vector<int2> table;
+  std::vector<int> grid_of_pass;  // workgroups launched by backward pass j
+  int max_steps = 0;
+  size_t real = 4, mat_elems = 0, cf_elems = 0;
+  size_t off_meta, off_table, off_mat, off_cf, off_part, off_tick, off_beta, off_eur, off_out, ws = 0;
+};
+
+int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K, const double* sigma,
+                   const double* r, const double* T, const int* steps, const int* paths, LsmcPlanImpl* pl) {
+  FSMC_REQUIRE(n >= 1 && n <= (1 << 20), "n_options %d outside [1, 2^20]", n);
+  FSMC_TRY(validate_mode(m, false));
+  FSMC_REQUIRE(S0 && K && sigma && r && T && steps && paths, "NULL host pointer");
+  for (int o = 0; o < n; ++o) {
+    FSMC_REQUIRE(steps[o] >= 1 && paths[o] >= 2, "option %d: steps %d / paths %d out of range", o, steps[o],
+                 paths[o]);
+    FSMC_REQUIRE(S0[o] > 0 && K[o] > 0 && sigma[o] > 0 && T[o] > 0, "option %d: S0, K, sigma, T must be > 0", o);
+  }
+  pl->n = n;
+  pl->mode = *m;
+  pl->precision = m->precision;
+  pl->real = m->precision == FSMC_FP64 ? 8 : 4;
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return steps[x] > steps[y]; });
+  pl->meta.resize(n);
+  pl->table.clear();
+  long long mat = 0, cfo = 0;
+  for (int idx = 0; idx < n; ++idx) {
+    const int o = order[idx];
+    OptMeta& q = pl->meta[o];
+    q.mat_off = mat;
+    q.cf_off = cfo;
+    q.steps = steps[o];
+    q.paths = paths[o];
+    q.nblk = (paths[o] + kChunk - 1) / kChunk;
+    q.first_block = (int)pl->table.size();
+    q.S0 = S0[o];
+    q.K = K[o];
+    q.sigma = sigma[o];
+    q.r = r[o];
+    q.T = T[o];
+    for (int c = 0; c < q.nblk; ++c) pl->table.push_back(make_int2(o, c));
+    mat += (long long)steps[o] * paths[o];
+    cfo += paths[o];
+  }
+  pl->mat_elems = (size_t)mat;
+  pl->cf_elems = (size_t)cfo;
+  pl->max_steps = steps[order[0]];
+  pl->grid_of_pass.assign(pl->max_steps, 0);
+  // options sorted by steps: those with steps > j are a prefix of the table
+  for (int j = 0; j < pl->max_steps; ++j) {
+    int blocks = 0;
+    for (int idx = 0; idx < n && steps[order[idx]] > j; ++idx) blocks += pl->meta[order[idx]].nblk;
+    pl->grid_of_pass[j] = blocks;
+  }
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = align_up(off + bytes, 256);
+    return at;
+  };
+  pl->off_meta = take(sizeof(OptMeta) * n);
+  pl->off_table = take(sizeof(int2) * pl->table.size());
+  pl->off_mat = take(pl->real * pl->mat_elems);
+  pl->off_cf = take(pl->real * pl->cf_elems);
+  pl->off_part = take(sizeof(double) * kNS * pl->table.size());
+  pl->off_tick = take(sizeof(unsigned) * n);
+  pl->off_beta = take(sizeof(double) * 10 * n);
+  pl->off_eur = take(sizeof(double) * pl->table.size());
+  pl->off_out = take(sizeof(double) * 3 * n);
+  pl->ws = off;
+  return FSMC_OK;
+}
+
+template <typename Real>
+int lsmc_enqueue(const LsmcPlanImpl& pl, char* ws, hipStream_t s) {
+  LsmcArgs a;
+  a.meta = (const OptMeta*)(ws + pl.off_meta);
+  a.table = (const int2*)(ws + pl.off_table);
+  a.mat = ws + pl.off_mat;
+  a.cf = ws + pl.off_cf;
+  a.partials = (double*)(ws + pl.off_part);
+  a.tickets = (unsigned*)(ws + pl.off_tick);
+  a.beta = (double*)(ws + pl.off_beta);
+  a.eur = (double*)(ws + pl.off_eur);
+  a.n_options = pl.n;
+  a.k0 = (uint32_t)pl.mode.seed;
+  a.k1 = (uint32_t)(pl.mode.seed >> 32);
+  a.w = counter_w(kTagLsmc, pl.mode.stream);
+  a.pass = 0;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3((pl.n + 255) / 256), dim3(256), 0, s, a.tickets, pl.n);
+  hipLaunchKernelGGL(zero_f64_kernel, dim3((10 * pl.n + 255) / 256), dim3(256), 0, s, a.beta, 10 * pl.n);
+  const int nb = (int)pl.table.size();
+  hipLaunchKernelGGL(lsmc_paths_kernel<Real>, dim3(nb), dim3(kBlock), 0, s, a);
+  FSMC_HIP(hipGetLastError());
+  for (int j = 0; j < pl.max_steps; ++j) {
+    a.pass = j;
+    hipLaunchKernelGGL(lsmc_pass_kernel<Real>, dim3(pl.grid_of_pass[j]), dim3(kBlock), 0, s, a);
+  }
+  FSMC_HIP(hipGetLastError());
+  double* out = (double*)(ws + pl.off_out);
+  hipLaunchKernelGGL(lsmc_final_kernel<Real>, dim3(pl.n), dim3(kBlock), 0, s, a, out, out + 2 * pl.n);
+  FSMC_HIP(hipGetLastError());
+  return FSMC_OK;
+}
+
+}  // namespace fsmc
+
+using namespace fsmc;
+
+struct fsmc_lsmc_plan {
+  fsmc_ctx* ctx;
+  LsmcPlanImpl impl;
+  DeviceBuffer ws;
+};
+
+extern "C" {
+
+int fsmc_lsmc_plan_create(fsmc_ctx* ctx, int n_options, const fsmc_mode* m, const double* S0, const double* K,
+                          const double* sigma, const double* r, const double* T, const int* steps,
+                          const int* paths, fsmc_lsmc_plan** out) {
+  FSMC_REQUIRE(ctx && out, "NULL argument");
+  *out = nullptr;
+  fsmc_lsmc_plan* pl = new fsmc_lsmc_plan();
+  pl->ctx = ctx;
+  int st = lsmc_make_plan(n_options, m, S0, K, sigma, r, T, steps, paths, &pl->impl);
+  if (st == FSMC_OK) {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    st = hipSetDevice(ctx->device) == hipSuccess ? FSMC_OK : FSMC_ERR_DEVICE;
+    if (st == FSMC_OK) st = pl->ws.reserve(pl->impl.ws);
+    if (st == FSMC_OK) {
+      char* ws = (char*)pl->ws.ptr;
+      if (hipMemcpyAsync(ws + pl->impl.off_meta, pl->impl.meta.data(), sizeof(OptMeta) * n_options,
+                         hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+          hipMemcpyAsync(ws + pl->impl.off_table, pl->impl.table.data(), sizeof(int2) * pl->impl.table.size(),
+                         hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+          hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        set_error("upload of the LSMC plan failed");
+        st = FSMC_ERR_DEVICE;
+      }
+    }
+  }
+  if (st != FSMC_OK) {
+    pl->ws.release();
+    delete pl;
+    return st;
+  }
+  *out = pl;
+  return FSMC_OK;
+}
+
+size_t fsmc_lsmc_plan_workspace(const fsmc_lsmc_plan* pl) { return pl ? pl->impl.ws : 0; }
+
+int fsmc_lsmc_plan_run(fsmc_lsmc_plan* pl, void* stream) {
+  FSMC_REQUIRE(pl != nullptr, "plan is NULL");
+  FSMC_HIP(hipSetDevice(pl->ctx->device));
+  hipStream_t s = stream ? (hipStream_t)stream : pl->ctx->stream;
+  char* ws = (char*)pl->ws.ptr;
+  return pl->impl.precision == FSMC_FP64 ? lsmc_enqueue<double>(pl->impl, ws, s)
+                                         : lsmc_enqueue<float>(pl->impl, ws, s);
+}
+
+int fsmc_lsmc_plan_results(fsmc_lsmc_plan* pl, void* stream, double* price, double* european) {
+  FSMC_REQUIRE(pl && price, "NULL argument");
+  FSMC_HIP(hipSetDevice(pl->ctx->device));
+  hipStream_t s = stream ? (hipStream_t)stream : pl->ctx->stream;
+  const double* out = (const double*)((char*)pl->ws.ptr + pl->impl.off_out);
+  const int n = pl->impl.n;
+  FSMC_HIP(hipMemcpyAsync(price, out, 2 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (european) FSMC_HIP(hipMemcpyAsync(european, out + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  FSMC_HIP(hipStreamSynchronize(s));
+  return FSMC_OK;
+}
+
+int fsmc_lsmc_plan_destroy(fsmc_lsmc_plan* pl) {
+  if (!pl) return FSMC_OK;
+  (void)hipSetDevice(pl->ctx->device);
+  (void)hipDeviceSynchronize();
+  pl->ws.release();
+  delete pl;
+  return FSMC_OK;
+}
+
+int fsmc_american_lsmc(fsmc_ctx* ctx, int n_options, const fsmc_mode* m, const double* S0, const double* K,
+                       const double* sigma, const double* r, const double* T, const int* steps, const int* paths,
+                       double* price, double* european) {
+  fsmc_lsmc_plan* pl = nullptr;
+  FSMC_TRY(fsmc_lsmc_plan_create(ctx, n_options, m, S0, K, sigma, r, T, steps, paths, &pl));
+  int st = fsmc_lsmc_plan_run(pl, nullptr);
+  if (st == FSMC_OK) st = fsmc_lsmc_plan_results(pl, nullptr, price, european);
+  fsmc_lsmc_plan_destroy(pl);
+  return st;
+}
+
+int fsmc_american_lsmc_ref(fsmc_ctx* ctx, const fsmc_lsmc_ref_params* p, const fsmc_mode* m, const double* z,
+                           double* american, double* european) {
+  FSMC_REQUIRE(ctx && p && american && european, "NULL argument");
+  FSMC_TRY(validate_mode(m, true));
+  FSMC_REQUIRE(p->n_paths >= 2 && p->n_steps >= 2, "n_paths %d / n_steps %d too small", p->n_paths, p->n_steps);
+  const bool buffer = m->rng == FSMC_RNG_BUFFER;
+  FSMC_REQUIRE(!buffer || z, "buffer mode needs z");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const size_t N = (size_t)p->n_paths;
+  const size_t real = m->precision == FSMC_FP64 ? 8 : 4;
+  const int nb = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 1024);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = align_up(off + bytes, 256);
+    return at;
+  };
+  const size_t o_a = take(real * (size_t)(p->n_steps + 1) * N), o_cf = take(real * N),
+               o_part = take(sizeof(double) * 8 * nb), o_tick = take(sizeof(unsigned)),
+               o_beta = take(sizeof(double) * 6), o_out = take(sizeof(double) * 2);
+  const size_t zn = buffer ? N * (size_t)p->n_steps : 0;
+  FSMC_TRY(ctx_prepare(ctx, 0, zn * sizeof(double), 0, off));
+  hipStream_t s = ctx->stream;
+  if (buffer) FSMC_HIP(hipMemcpyAsync(ctx->normals.ptr, z, zn * sizeof(double), hipMemcpyHostToDevice, s));
+  char* ws = (char*)ctx->workspace.ptr;
+  RefArgs r;
+  r.n_paths = p->n_paths;
+  r.n_steps = p->n_steps;
+  r.spot = p->spot;
+  r.strike = p->strike;
+  // main.cpp:55-59
+  const double dt = p->maturity / p->n_steps;
+  r.vsdt_l2e = p->vol * sqrt(dt) * kLog2e;
+  r.fwd_l2e = p->vol * p->vol * dt * -0.5 * kLog2e;
+  r.z = (const double*)ctx->normals.ptr;
+  r.k0 = (uint32_t)m->seed;
+  r.k1 = (uint32_t)(m->seed >> 32);
+  r.w = counter_w(kTagLsmc, m->stream);
+  r.a = ws + o_a;
+  r.cf = ws + o_cf;
+  r.partials = (double*)(ws + o_part);
+  r.ticket = (unsigned*)(ws + o_tick);
+  r.beta = (double*)(ws + o_beta);
+  r.out = (double*)(ws + o_out);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, r.ticket, 1);
+  hipLaunchKernelGGL(zero_f64_kernel, dim3(1), dim3(64), 0, s, r.beta, 6);
+  const int pb = (int)((N + kBlock - 1) / kBlock);
+  const bool f64 = m->precision == FSMC_FP64;
+  if (f64 && buffer)
+    hipLaunchKernelGGL((ref_paths_kernel<double, true>), dim3(pb), dim3(kBlock), 0, s, r);
+  else if (f64)
+    hipLaunchKernelGGL((ref_paths_kernel<double, false>), dim3(pb), dim3(kBlock), 0, s, r);
+  else if (buffer)
+    hipLaunchKernelGGL((ref_paths_kernel<float, true>), dim3(pb), dim3(kBlock), 0, s, r);
+  else
+    hipLaunchKernelGGL((ref_paths_kernel<float, false>), dim3(pb), dim3(kBlock), 0, s, r);
+  FSMC_HIP(hipGetLastError());
+  for (int st = p->n_steps - 1; st > 0; --st) {
+    if (f64)
+      hipLaunchKernelGGL(ref_step_kernel<double>, dim3(nb), dim3(kBlock), 0, s, r, st);
+    else
+      hipLaunchKernelGGL(ref_step_kernel<float>, dim3(nb), dim3(kBlock), 0, s, r, st);
+  }
+  FSMC_HIP(hipGetLastError());
+  if (f64)
+    hipLaunchKernelGGL(ref_final_kernel<double>, dim3(1), dim3(kBlock), 0, s, r);
+  else
+    hipLaunchKernelGGL(ref_final_kernel<float>, dim3(1), dim3(kBlock), 0, s, r);
+  FSMC_HIP(hipGetLastError());
+  double res[2];
+  FSMC_HIP(hipMemcpyAsync(res, r.out, sizeof(res), hipMemcpyDeviceToHost, s));
+  FSMC_HIP(hipStreamSynchronize(s));
+  *american = res[0];
+  *european = res[1];
+  return FSMC_OK;
+}
+
+}  // extern "C"

@@ -48,6 +48,15 @@ class GbmParams(ctypes.Structure):
                 ("volatility", ctypes.c_double)]
 
 
+class LiborParams(ctypes.Structure):
+    _fields_ = [("n_paths", ctypes.c_int), ("L0", ctypes.c_double), ("lam", ctypes.c_double)]
+
+
+class LsmcRefParams(ctypes.Structure):
+    _fields_ = [("n_paths", ctypes.c_int), ("n_steps", ctypes.c_int), ("spot", ctypes.c_double),
+                ("strike", ctypes.c_double), ("vol", ctypes.c_double), ("maturity", ctypes.c_double)]
+
+
 class HestonParams(ctypes.Structure):
     _fields_ = [("n_options", ctypes.c_int), ("n_paths", ctypes.c_int), ("n_steps", ctypes.c_int),
                 ("riskfree", ctypes.c_double), ("rho", ctypes.c_double)]
@@ -87,6 +96,21 @@ def lib():
         L.fsmc_heston_asian_workspace.argtypes = [ctypes.POINTER(HestonParams), PM]
         L.fsmc_heston_asian_async.argtypes = [ctypes.POINTER(HestonParams), PM] + [_VP] * 12 + [
             _VP, ctypes.c_size_t, _VP]
+        PL = ctypes.POINTER(LiborParams)
+        L.fsmc_libor_swaption_greeks.argtypes = [_VP, PL, PM, _D, _D, _D]
+        L.fsmc_libor_workspace.restype = ctypes.c_size_t
+        L.fsmc_libor_workspace.argtypes = [PL, PM]
+        L.fsmc_libor_swaption_greeks_async.argtypes = [PL, PM, _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]
+        L.fsmc_libor_normals.argtypes = [_VP, ctypes.c_int, PM, ctypes.c_double, ctypes.c_double, _D]
+        _I = ctypes.POINTER(ctypes.c_int)
+        L.fsmc_lsmc_plan_create.argtypes = [_VP, ctypes.c_int, PM] + [_D] * 5 + [_I, _I, ctypes.POINTER(_VP)]
+        L.fsmc_lsmc_plan_workspace.restype = ctypes.c_size_t
+        L.fsmc_lsmc_plan_workspace.argtypes = [_VP]
+        L.fsmc_lsmc_plan_run.argtypes = [_VP, _VP]
+        L.fsmc_lsmc_plan_results.argtypes = [_VP, _VP, _D, _D]
+        L.fsmc_lsmc_plan_destroy.argtypes = [_VP]
+        L.fsmc_american_lsmc.argtypes = [_VP, ctypes.c_int, PM] + [_D] * 5 + [_I, _I, _D, _D]
+        L.fsmc_american_lsmc_ref.argtypes = [_VP, ctypes.POINTER(LsmcRefParams), PM, _D, _D, _D]
         _lib = L
     return _lib
 
@@ -169,6 +193,92 @@ class Engine:
         return (res, conf, g) if greeks else (res, conf)
 
 
+    # -- LIBOR swaption portfolio -------------------------------------------
+    def libor(self, n_paths, m, L0=0.05, lam=0.2, z=None):
+        """Returns (value, value_se, deltas[80], deltas_se[80])."""
+        p = LiborParams(n_paths, L0, lam)
+        pz = None
+        if z is not None:
+            z, pz = _d(np.asarray(z).reshape(-1))
+        val, pval = _d(np.zeros(2))
+        dl, pdl = _d(np.zeros(160))
+        _check(lib().fsmc_libor_swaption_greeks(self._ctx, ctypes.byref(p), ctypes.byref(m), pz, pval, pdl),
+               "fsmc_libor_swaption_greeks")
+        return val[0], val[1], dl[:80].copy(), dl[80:].copy()
+
+    # -- Longstaff-Schwartz --------------------------------------------------
+    def american_lsmc(self, S0, K, sigma, r, T, steps, paths, m):
+        """Returns (price[n], se[n], european[n])."""
+        n = len(S0)
+        arrs = [_d(a) for a in (S0, K, sigma, r, T)]
+        st = np.ascontiguousarray(steps, dtype=np.int32)
+        pa = np.ascontiguousarray(paths, dtype=np.int32)
+        _I = ctypes.POINTER(ctypes.c_int)
+        pr, ppr = _d(np.zeros((n, 2)))
+        eu, peu = _d(np.zeros(n))
+        _check(lib().fsmc_american_lsmc(self._ctx, n, ctypes.byref(m), *[q for _, q in arrs], st.ctypes.data_as(_I),
+                                        pa.ctypes.data_as(_I), ppr, peu), "fsmc_american_lsmc")
+        return pr[:, 0].copy(), pr[:, 1].copy(), eu
+
+    def lsmc_plan(self, S0, K, sigma, r, T, steps, paths, m):
+        return LsmcPlan(self, S0, K, sigma, r, T, steps, paths, m)
+
+    def american_lsmc_ref(self, n_paths, n_steps, spot, strike, vol, maturity, m, z=None):
+        """The sample's own LSMC algorithm; returns (american, european)."""
+        p = LsmcRefParams(n_paths, n_steps, spot, strike, vol, maturity)
+        pz = None
+        if z is not None:
+            z, pz = _d(np.asarray(z).reshape(-1))
+        am, eu = ctypes.c_double(), ctypes.c_double()
+        _check(lib().fsmc_american_lsmc_ref(self._ctx, ctypes.byref(p), ctypes.byref(m), pz, ctypes.byref(am),
+                                            ctypes.byref(eu)), "fsmc_american_lsmc_ref")
+        return am.value, eu.value
+
+    def libor_normals(self, n_paths, m, mean=0.0, stddev=1.0):
+        z, pz = _d(np.zeros(n_paths * 40))
+        _check(lib().fsmc_libor_normals(self._ctx, n_paths, ctypes.byref(m), mean, stddev, pz),
+               "fsmc_libor_normals")
+        return z.reshape(n_paths, 40)
+
+
+class LsmcPlan:
+    """A batched LSMC plan: option table on the device, workspace owned."""
+
+    def __init__(self, engine, S0, K, sigma, r, T, steps, paths, m):
+        self.n = len(S0)
+        self._p = _VP()
+        arrs = [_d(a) for a in (S0, K, sigma, r, T)]
+        st = np.ascontiguousarray(steps, dtype=np.int32)
+        pa = np.ascontiguousarray(paths, dtype=np.int32)
+        _I = ctypes.POINTER(ctypes.c_int)
+        _check(lib().fsmc_lsmc_plan_create(engine._ctx, self.n, ctypes.byref(m), *[q for _, q in arrs],
+                                           st.ctypes.data_as(_I), pa.ctypes.data_as(_I), ctypes.byref(self._p)),
+               "fsmc_lsmc_plan_create")
+
+    def workspace_bytes(self):
+        return lib().fsmc_lsmc_plan_workspace(self._p)
+
+    def run(self, stream=None):
+        _check(lib().fsmc_lsmc_plan_run(self._p, stream), "fsmc_lsmc_plan_run")
+
+    def results(self, stream=None):
+        pr, ppr = _d(np.zeros((self.n, 2)))
+        eu, peu = _d(np.zeros(self.n))
+        _check(lib().fsmc_lsmc_plan_results(self._p, stream, ppr, peu), "fsmc_lsmc_plan_results")
+        return pr[:, 0].copy(), pr[:, 1].copy(), eu
+
+    def close(self):
+        if self._p:
+            lib().fsmc_lsmc_plan_destroy(self._p)
+            self._p = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---- asynchronous entry points on torch device tensors ------------------------
 def _ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
@@ -206,3 +316,17 @@ def heston_asian_async(params, n_paths, n_steps, r, rho, m, out_res, out_conf, w
                                          _ptr(z1), _ptr(z2), _ptr(out_res), _ptr(out_conf), _ptr(out_greeks),
                                          _ptr(workspace), workspace.numel() * workspace.element_size(),
                                          _stream()), "fsmc_heston_asian_async")
+
+
+def libor_workspace(n_paths, m, L0=0.05, lam=0.2):
+    p = LiborParams(n_paths, L0, lam)
+    return lib().fsmc_libor_workspace(ctypes.byref(p), ctypes.byref(m))
+
+
+def libor_async(n_paths, m, out_value, out_deltas, workspace, L0=0.05, lam=0.2, z=None):
+    """out_value: 2 doubles, out_deltas: 160 doubles (device tensors)."""
+    p = LiborParams(n_paths, L0, lam)
+    _check(lib().fsmc_libor_swaption_greeks_async(ctypes.byref(p), ctypes.byref(m), _ptr(z), _ptr(out_value),
+                                                  _ptr(out_deltas), _ptr(workspace),
+                                                  workspace.numel() * workspace.element_size(), _stream()),
+           "fsmc_libor_swaption_greeks_async")

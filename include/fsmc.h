@@ -130,6 +130,73 @@ int fsmc_heston_asian_async(const fsmc_heston_params* p, const fsmc_mode* m, con
                             double* d_call_confidence, double* d_greeks, void* d_workspace,
                             size_t workspace_bytes, void* stream);
 
+/* ---- LIBOR market model swaption portfolio + pathwise adjoint deltas -------
+ * Replaces the per-path body of liborSwaptionGreeks, liborSwaptionGreeks.cpp:
+ * 241-356 (declared liborSwaptionGreeks.hpp:12-13): 80 forward rates, 40
+ * log-Euler steps of tenor 0.25 (params.h:27-34), the 15 payer swaptions of
+ * params.h:38-40, and the adjoint sweep path_calc_b2.  L0 and lambda are the
+ * flat initial rate and volatility of :327-329 (0.051 and 0.2 there).
+ * Buffer mode consumes z laid out [n_paths][40] (z[path*40 + n], :386).
+ * Outputs: value[2] = {mean discounted portfolio value, its standard error};
+ * deltas[160] = {dV/dL0_i for i = 0..79, then their standard errors}.  The
+ * means are per path (the sample prints 10x these, NLOOPS = 10, :355-356). */
+typedef struct fsmc_libor_params {
+  int n_paths;
+  double L0;
+  double lambda;
+} fsmc_libor_params;
+
+int fsmc_libor_swaption_greeks(fsmc_ctx* ctx, const fsmc_libor_params* p, const fsmc_mode* m, const double* z,
+                               double* value, double* deltas);
+size_t fsmc_libor_workspace(const fsmc_libor_params* p, const fsmc_mode* m);
+/* Fills z[n_paths][40] with mean + stddev * (the normals Philox mode uses),
+ * the device-side replacement of generateRand, mklRng.h:9 / mklRng.cpp:44. */
+int fsmc_libor_normals(fsmc_ctx* ctx, int n_paths, const fsmc_mode* m, double mean, double stddev, double* z);
+int fsmc_libor_swaption_greeks_async(const fsmc_libor_params* p, const fsmc_mode* m, const double* d_z,
+                                     double* d_value, double* d_deltas, void* d_workspace,
+                                     size_t workspace_bytes, void* stream);
+
+/* ---- Longstaff-Schwartz American puts -------------------------------------
+ * Batched LSMC for many options with uneven step and path counts: one
+ * exercise date per time step, cash flows discounted at r, regression of the
+ * discounted cash flow on {1, x, x^2, x^3} of the in-the-money paths with x an
+ * affine rescaling of S/K (Gram matrix on FP32 MFMA in FP32 mode, solved per
+ * date by FP64 Cholesky), Philox normals.  It replaces the sweep inlined in
+ * MonteCarloAmericanOptions/main.cpp:131-279 (with perform_sims,
+ * utilities.cpp:42-88) for the configurations that sample cannot express
+ * (rates, many options).  price[n][2] = {price, standard error}; european
+ * (optional, [n]) = the discounted European put on the same paths.
+ * A plan uploads the option table once and owns its device workspace; run it
+ * repeatedly on a stream (it enqueues only kernels), then read results. */
+typedef struct fsmc_lsmc_plan fsmc_lsmc_plan;
+
+int fsmc_lsmc_plan_create(fsmc_ctx* ctx, int n_options, const fsmc_mode* m, const double* S0, const double* K,
+                          const double* sigma, const double* r, const double* T, const int* steps,
+                          const int* paths, fsmc_lsmc_plan** out);
+size_t fsmc_lsmc_plan_workspace(const fsmc_lsmc_plan* plan);
+int fsmc_lsmc_plan_run(fsmc_lsmc_plan* plan, void* stream);
+int fsmc_lsmc_plan_results(fsmc_lsmc_plan* plan, void* stream, double* price, double* european);
+int fsmc_lsmc_plan_destroy(fsmc_lsmc_plan* plan);
+int fsmc_american_lsmc(fsmc_ctx* ctx, int n_options, const fsmc_mode* m, const double* S0, const double* K,
+                       const double* sigma, const double* r, const double* T, const int* steps, const int* paths,
+                       double* price, double* european);
+
+/* The sample's own algorithm for one option (MonteCarloAmericanOptions/
+ * main.cpp:41-296, utilities.cpp:42-123): GBM without a rate in the log2
+ * domain, quadratic regression with ridge 0.01 and a cofactor 3x3 inverse,
+ * the betas of step s applied at step s-1, the terminal cash flow taken at
+ * step n_steps-1 and scaled by 1/N.  Buffer mode consumes
+ * z[(s-1)*n_paths + p] for step s = 1..n_steps of path p.  Outputs the
+ * sample's "American" and "European" prices (main.cpp:294-295). */
+typedef struct fsmc_lsmc_ref_params {
+  int n_paths;
+  int n_steps;
+  double spot, strike, vol, maturity;
+} fsmc_lsmc_ref_params;
+
+int fsmc_american_lsmc_ref(fsmc_ctx* ctx, const fsmc_lsmc_ref_params* p, const fsmc_mode* m, const double* z,
+                           double* american, double* european);
+
 #ifdef __cplusplus
 }
 #endif

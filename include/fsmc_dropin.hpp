@@ -1,0 +1,26 @@
+// fsmc_dropin.hpp - C++ drop-in entry points with the reference samples' own
+// names and signatures, implemented on the engine's C ABI (fsmc.h).  Each is
+// overloaded for REAL = double and REAL = float, so a sample built with or
+// without -DUSE_DOUBLE links against libfsmc.so instead of its CPU source.
+// They run synchronously on device 0, write the caller's host buffers and
+// keep the sample's observable semantics; they never exit the process (on a
+// failure they print to stderr and write NaN results).
+#ifndef FSMC_DROPIN_HPP_
+#define FSMC_DROPIN_HPP_
+
+// LiborMarketModel/mklRng.h:9 -- z[0..num) = mean + stddev * N(0,1) normals
+// (the engine's Philox stream keyed by seed instead of MKL MRG32k3a).
+void generateRand(double* z, int num, double mean, double stddev, int seed = 12345);
+void generateRand(float* z, int num, float mean, float stddev, int seed = 12345);
+
+// LiborMarketModel/liborSwaptionGreeks.hpp:12-13 -- regenerates z (seed
+// argument ignored, as liborSwaptionGreeks.cpp:319 does), prices the swaption
+// portfolio for numPaths paths NLOOPS = 10 times and returns the sums divided
+// by numPaths: *d_v = 10 x mean value, *d_Lb = 10 x mean dV/dL0_79
+// (liborSwaptionGreeks.cpp:336-356).
+void liborSwaptionGreeks(double* z, int num, double mean, double std_dev, int seed, double* d_v, double* d_Lb,
+                         int numPaths);
+void liborSwaptionGreeks(float* z, int num, float mean, float std_dev, int seed, float* d_v, float* d_Lb,
+                         int numPaths);
+
+#endif

@@ -135,6 +135,37 @@ double orc_crr_american_put(double S, double K, double T, double r, double sigma
   return res;
 }
 
+double orc_crr_bermudan_put(double S, double K, double T, double r, double sigma, int dates, int steps_per_date) {
+  const int steps = dates * steps_per_date;
+  const double dt = T / steps;
+  const double vdt = sigma * sqrt(dt);
+  const double If = exp(r * dt), Df = exp(-r * dt);
+  const double u = exp(vdt), d = exp(-vdt);
+  const double pu = (If - d) / (u - d), pd = 1.0 - pu;
+  const double puDf = pu * Df, pdDf = pd * Df;
+  double* put = (double*)malloc(sizeof(double) * (steps + 1));
+  for (int i = 0; i <= steps; ++i) {
+    const double v = K - S * exp(vdt * (2.0 * i - steps));
+    put[i] = v > 0.0 ? v : 0.0;
+  }
+  for (int i = steps; i > 0; --i) {
+    const int layer = i - 1; /* time layer*dt of the nodes being computed */
+    const int can_exercise = (layer % steps_per_date) == 0;
+    for (int j = 0; j <= i - 1; ++j) {
+      const double cont = puDf * put[j + 1] + pdDf * put[j];
+      if (can_exercise) {
+        const double ex = K - S * exp(vdt * (2.0 * j - layer));
+        put[j] = cont > ex ? cont : ex;
+      } else {
+        put[j] = cont;
+      }
+    }
+  }
+  const double res = put[0];
+  free(put);
+  return res;
+}
+
 double orc_geometric_asian_call(double S, double X, double T, double r, double sigma, int N) {
   const double mu = log(S) + (r - 0.5 * sigma * sigma) * T * (N + 1.0) / (2.0 * N);
   const double var = sigma * sigma * T * (N + 1.0) * (2.0 * N + 1.0) / (6.0 * N * (double)N);
@@ -358,4 +389,318 @@ void orc_heston_philox(int n_opt, int n_paths, int n_steps, double r, double rho
     if (greeks6)
       for (int t = 0; t < 3; ++t) mean_se(g[t], g2[t], N, 1.0, &greeks6[6 * opt + t], &greeks6[6 * opt + 3 + t]);
   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* LIBOR market model, liborSwaptionGreeks.cpp.  NN = 80 rates, NMAT = 40
+ * steps, delta = 0.25 (params.h:27-34); portfolio of params.h:38-40 with the
+ * swap rates rounded to float as the sample initialises them. */
+#define ORC_NN 80
+#define ORC_NMAT 40
+static const int orc_mat[15] = {4, 4, 4, 8, 8, 8, 20, 20, 20, 28, 28, 28, 40, 40, 40};
+static const float orc_rate[15] = {.045f, .05f, .055f, .045f, .05f, .055f, .045f, .05f,
+                                   .055f, .045f, .05f, .055f, .045f, .05f, .055f};
+
+double orc_libor_path(const double* z, const double* L0, double lambda, uint32_t quirks, double* Lb) {
+  const double delta = 0.25, sqrt_delta = 0.5;
+  double L[ORC_NN], L2[(ORC_NMAT + 1) * ORC_NN];
+  for (int i = 0; i < ORC_NN; ++i) L[i] = L0[i];
+  memset(L2, 0, sizeof(L2));
+  /* path_calc_b1, :34-82 */
+  for (int i = 0; i < ORC_NN; ++i) L2[i] = L[i];
+  for (int n = 0; n < ORC_NMAT; ++n) {
+    const double sqez = sqrt_delta * z[n];
+    double v = 0.0;
+    for (int i = n + 1; i < ORC_NN; ++i) {
+      const double lam = lambda, con1 = delta * lam;
+      v += con1 * L[i] / (1.0 + delta * L[i]);
+      const double vrat = exp(con1 * v + lam * (sqez - 0.5 * con1));
+      L[i] = L[i] * vrat;
+      L2[i + (n + 1) * ORC_NN] = L[i];
+    }
+  }
+  /* portfolio_b, :120-211; with ORC_Q_LIBOR_INPLACE the adjoint overwrites L
+   * exactly as the sample does (:178, read back at :181). */
+  double B[ORC_NMAT], S[ORC_NMAT], B_b[ORC_NMAT], S_b[ORC_NMAT];
+  double b = 1.0, s = 0.0, v = 0.0;
+  double Lw[ORC_NN];
+  memcpy(Lw, L, sizeof(L));
+  for (int m = 0; m < ORC_NN - ORC_NMAT; ++m) {
+    const int n = m + ORC_NMAT;
+    b = b / (1.0 + delta * Lw[n]);
+    s = s + delta * b;
+    B[m] = b;
+    S[m] = s;
+  }
+  for (int i = 0; i < ORC_NMAT; ++i) B_b[i] = S_b[i] = 0.0;
+  for (int n = 0; n < 15; ++n) {
+    const int m = orc_mat[n] - 1;
+    const double K = (double)orc_rate[n];
+    const double swapval = B[m] + K * S[m] - 1.0;
+    if (swapval < 0.0) {
+      v += -100.0 * swapval;
+      S_b[m] += -100.0 * K;
+      B_b[m] += -100.0;
+    }
+  }
+  double Lbar[ORC_NN];
+  for (int m = ORC_NN - ORC_NMAT - 1; m >= 0; --m) {
+    const int n = m + ORC_NMAT;
+    B_b[m] += delta * S_b[m];
+    Lbar[n] = -B_b[m] * B[m] * delta / (1.0 + delta * Lw[n]);
+    if (quirks & ORC_Q_LIBOR_INPLACE) Lw[n] = Lbar[n];
+    if (m > 0) {
+      S_b[m - 1] += S_b[m];
+      B_b[m - 1] += B_b[m] / (1.0 + delta * Lw[n]);
+    }
+  }
+  b = 1.0;
+  for (int n = 0; n < ORC_NMAT; ++n) b = b / (1.0 + delta * Lw[n]);
+  v = b * v;
+  for (int n = 0; n < ORC_NMAT; ++n) Lbar[n] = -v * delta / (1.0 + delta * Lw[n]);
+  for (int n = ORC_NMAT; n < ORC_NN; ++n) Lbar[n] = b * Lbar[n];
+  /* path_calc_b2, :86-117 */
+  for (int n = ORC_NMAT - 1; n >= 0; --n) {
+    double v1 = 0.0;
+    for (int i = ORC_NN - 1; i > n; --i) {
+      const double lam = lambda;
+      const double l2n = L2[i + n * ORC_NN], l2n1 = L2[i + (n + 1) * ORC_NN];
+      v1 += lam * l2n1 * Lbar[i];
+      const double faci = delta / (1.0 + delta * l2n);
+      Lbar[i] = Lbar[i] * l2n1 / l2n + v1 * lam * faci * faci;
+    }
+  }
+  memcpy(Lb, Lbar, sizeof(Lbar));
+  return v;
+}
+
+void orc_libor(int n_paths, double L0, double lambda, const double* z, uint64_t seed, uint32_t stream,
+               uint32_t quirks, double* value, double* deltas) {
+  double s0 = 0, s1 = 0, d0[ORC_NN], d1[ORC_NN], curve[ORC_NN];
+  for (int i = 0; i < ORC_NN; ++i) {
+    d0[i] = d1[i] = 0.0;
+    curve[i] = L0;
+  }
+#pragma omp parallel
+  {
+    double t0 = 0, t1 = 0, e0[ORC_NN], e1[ORC_NN], zp[ORC_NMAT], Lb[ORC_NN];
+    for (int i = 0; i < ORC_NN; ++i) e0[i] = e1[i] = 0.0;
+#pragma omp for schedule(static)
+    for (int p = 0; p < n_paths; ++p) {
+      if (z) {
+        memcpy(zp, z + (size_t)p * ORC_NMAT, sizeof(zp));
+      } else {
+        for (int q = 0; q < ORC_NMAT / 4; ++q) orc_normals4(seed, stream, 3u, (uint32_t)p, (uint32_t)q, 0u, zp + 4 * q);
+      }
+      const double v = orc_libor_path(zp, curve, lambda, quirks, Lb);
+      t0 += v;
+      t1 += v * v;
+      for (int i = 0; i < ORC_NN; ++i) {
+        e0[i] += Lb[i];
+        e1[i] += Lb[i] * Lb[i];
+      }
+    }
+#pragma omp critical
+    {
+      s0 += t0;
+      s1 += t1;
+      for (int i = 0; i < ORC_NN; ++i) {
+        d0[i] += e0[i];
+        d1[i] += e1[i];
+      }
+    }
+  }
+  const double N = (double)n_paths;
+  mean_se(s0, s1, N, 1.0, &value[0], &value[1]);
+  for (int i = 0; i < ORC_NN; ++i) mean_se(d0[i], d1[i], N, 1.0, &deltas[i], &deltas[ORC_NN + i]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Longstaff-Schwartz, reference algorithm: MonteCarloAmericanOptions/main.cpp
+ * :46-60 (constants), utilities.cpp:42-88 (perform_sims, log2 domain),
+ * main.cpp:131-279 (backward sweep) and utilities.cpp:90-123 (invert3_arr).
+ * z[(s-1)*n_paths + p] drives step s = 1..n_steps of path p. */
+static double det3(double a, double b, double c, double d, double e, double f, double g, double h, double i) {
+  return a * e * i + d * h * c + g * b * f - g * e * c - d * b * i - a * h * f;
+}
+
+static void invert3(double a[3][3], double x[3][3]) {
+  const double det = det3(x[0][0], x[0][1], x[0][2], x[1][0], x[1][1], x[1][2], x[2][0], x[2][1], x[2][2]);
+  a[0][0] = (x[1][1] * x[2][2] - x[2][1] * x[1][2]) / det;
+  a[0][1] = (x[0][2] * x[2][1] - x[0][1] * x[2][2]) / det;
+  a[0][2] = (x[0][1] * x[1][2] - x[0][2] * x[1][1]) / det;
+  a[1][0] = (x[1][2] * x[2][0] - x[1][0] * x[2][2]) / det;
+  a[1][1] = (x[0][0] * x[2][2] - x[0][2] * x[2][0]) / det;
+  a[1][2] = (x[0][2] * x[1][0] - x[0][0] * x[1][2]) / det;
+  a[2][0] = (x[1][0] * x[2][1] - x[1][1] * x[2][0]) / det;
+  a[2][1] = (x[0][1] * x[2][0] - x[0][0] * x[2][1]) / det;
+  a[2][2] = (x[0][0] * x[1][1] - x[0][1] * x[1][0]) / det;
+}
+
+void orc_lsmc_ref(int n_paths, int n_steps, double spot, double strike, double vol, double maturity,
+                  const double* z, double* american, double* european) {
+  const double dt = maturity / n_steps;
+  const double volSqrtdt = vol * sqrt(dt);
+  const double fwdFactor = vol * vol * dt * -0.5;
+  const double vsqrtdt_log2e = volSqrtdt * ORC_LOG2E;
+  const double fwdFactor_log2e = fwdFactor * ORC_LOG2E;
+  const double avgPathFactor = 1.0 / n_paths;
+  const size_t N = (size_t)n_paths;
+  double* a = (double*)malloc(sizeof(double) * (size_t)(n_steps + 1) * N);
+  double* cf = (double*)calloc(N, sizeof(double));
+  for (size_t p = 0; p < N; ++p) a[p] = spot;
+  for (int s = 1; s <= n_steps; ++s)
+    for (size_t p = 0; p < N; ++p)
+      a[s * N + p] = a[(s - 1) * N + p] * exp2(fwdFactor_log2e + vsqrtdt_log2e * z[(size_t)(s - 1) * N + p]);
+  double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+  for (int s = n_steps - 1; s > 0; --s) {
+    double sum0 = 0, sum1 = 0, sum2 = 0, sum3 = 0, sum4 = 0, x0 = 0, x1 = 0, x2 = 0;
+    const double* ap = a + (size_t)s * N;
+    for (size_t p = 0; p < N; ++p) {
+      const double x = ap[p];
+      if (s < n_steps - 1) {
+        if (x <= strike) {
+          const double reg = b0 + b1 * x + b2 * (x * x);
+          const double payoff = strike - x;
+          if (payoff > reg) cf[p] = payoff;
+        }
+      } else {
+        cf[p] = (strike - x > 0.0 ? strike - x : 0.0) * avgPathFactor;
+      }
+      if (x <= strike) {
+        const double x2v = x * x;
+        sum0 += 1;
+        sum1 += x;
+        sum2 += x2v;
+        sum3 += x2v * x;
+        sum4 += x2v * x2v;
+        x0 += cf[p];
+        x1 += cf[p] * x;
+        x2 += cf[p] * x2v;
+      }
+    }
+    const double ridge = 0.01;
+    sum0 += ridge;
+    sum1 += ridge;
+    sum2 += ridge;
+    sum3 += ridge;
+    sum4 += ridge;
+    double m[3][3] = {{sum0, sum1, sum2}, {sum1, sum2, sum3}, {sum2, sum3, sum4}}, inv[3][3];
+    invert3(inv, m);
+    b0 = x0 * inv[0][0] + x1 * inv[1][0] + x2 * inv[2][0];
+    b1 = x0 * inv[0][1] + x1 * inv[1][1] + x2 * inv[2][1];
+    b2 = x0 * inv[0][2] + x1 * inv[1][2] + x2 * inv[2][2];
+  }
+  double am = 0.0, eu = 0.0;
+  for (size_t p = 0; p < N; ++p) {
+    am += cf[p];
+    const double e = strike - a[(size_t)n_steps * N + p];
+    eu += e > 0.0 ? e : 0.0;
+  }
+  *american = am / n_paths;
+  *european = eu / n_paths;
+  free(a);
+  free(cf);
+}
+
+int orc_chol4_solve(const double* G, const double* rhs, double* beta) {
+  double L[4][4] = {{0}};
+  for (int i = 0; i < 4; ++i) {
+    for (int j = 0; j <= i; ++j) {
+      double s = G[4 * i + j];
+      for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
+      if (i == j) {
+        if (!(s > 1e-13 * G[4 * i + i])) return 0;
+        L[i][i] = sqrt(s);
+      } else {
+        L[i][j] = s / L[j][j];
+      }
+    }
+  }
+  double y[4];
+  for (int i = 0; i < 4; ++i) {
+    double s = rhs[i];
+    for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
+    y[i] = s / L[i][i];
+  }
+  for (int i = 3; i >= 0; --i) {
+    double s = y[i];
+    for (int k = i + 1; k < 4; ++k) s -= L[k][i] * beta[k];
+    beta[i] = s / L[i][i];
+  }
+  return 1;
+}
+
+/* Correct LSMC for one option.  Normals: counter (path, j/4, option) of the
+ * LSMC tag, normal j%4 drives the step from S_j to S_{j+1}. */
+static void lsmc_one(int o, double S0, double K, double sigma, double r, double T, int n, int P, uint64_t seed,
+                     uint32_t stream, double* price2, double* european) {
+  const double dt = T / n;
+  const double drift = (r - 0.5 * sigma * sigma) * dt, vol = sigma * sqrt(dt);
+  double* S = (double*)malloc(sizeof(double) * (size_t)n * P); /* row j = S_{j+1} */
+  double* cf = (double*)malloc(sizeof(double) * (size_t)P);
+  const double discT = exp(-r * T);
+  double eu = 0.0;
+  for (int p = 0; p < P; ++p) {
+    double x = S0, zq[4] = {0, 0, 0, 0};
+    for (int j = 0; j < n; ++j) {
+      if ((j & 3) == 0) orc_normals4(seed, stream, 4u, (uint32_t)p, (uint32_t)(j >> 2), (uint32_t)o, zq);
+      x = x * exp(drift + vol * zq[j & 3]);
+      S[(size_t)j * P + p] = x;
+    }
+    const double pay = K - x > 0.0 ? K - x : 0.0;
+    cf[p] = discT * pay;
+    eu += cf[p];
+  }
+  for (int k = n - 1; k >= 1; --k) {
+    const double tk = k * dt, disc = exp(-r * tk);
+    const double inv_sw = 1.0 / (sigma * sqrt(tk));
+    const double* row = S + (size_t)(k - 1) * P;
+    double mo[7] = {0}, c[4] = {0};
+    for (int p = 0; p < P; ++p) {
+      if (K - row[p] > 0.0) {
+        const double x = 1.0 + (row[p] / K - 1.0) * inv_sw;
+        double xp = 1.0;
+        for (int q = 0; q < 7; ++q) {
+          mo[q] += xp;
+          if (q < 4) c[q] += xp * cf[p];
+          xp *= x;
+        }
+      }
+    }
+    double G[16], beta[4];
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) G[4 * i + j] = mo[i + j];
+    if (mo[0] < 4.0 || !orc_chol4_solve(G, c, beta)) continue; /* no exercise at this date */
+    for (int p = 0; p < P; ++p) {
+      if (K - row[p] > 0.0) {
+        const double x = 1.0 + (row[p] / K - 1.0) * inv_sw;
+        const double cont = beta[0] + x * (beta[1] + x * (beta[2] + x * beta[3]));
+        const double ex = (K - row[p]) * disc;
+        if (ex > cont) cf[p] = ex;
+      }
+    }
+  }
+  double s0 = 0.0, s1 = 0.0;
+  for (int p = 0; p < P; ++p) {
+    s0 += cf[p];
+    s1 += cf[p] * cf[p];
+  }
+  double mean, se;
+  mean_se(s0, s1, (double)P, 1.0, &mean, &se);
+  const double now = K - S0 > 0.0 ? K - S0 : 0.0;
+  price2[0] = mean > now ? mean : now;
+  price2[1] = se;
+  if (european) *european = eu / P;
+  free(S);
+  free(cf);
+}
+
+void orc_lsmc(int n_opt, const double* S0, const double* K, const double* sigma, const double* r,
+              const double* T, const int* steps, const int* paths, uint64_t seed, uint32_t stream,
+              double* price2, double* european) {
+#pragma omp parallel for schedule(dynamic)
+  for (int o = 0; o < n_opt; ++o)
+    lsmc_one(o, S0[o], K[o], sigma[o], r[o], T[o], steps[o], paths[o], seed, stream, price2 + 2 * o,
+             european ? european + o : NULL);
 }

@@ -29,6 +29,9 @@ double orc_bs_call(double S, double X, double T, double r, double sigma);
 void orc_bs_call_greeks(double S, double X, double T, double r, double sigma, double out[3]);
 double orc_crr_european_call(double S, double X, double T, double r, double sigma, int steps);
 double orc_crr_american_put(double S, double K, double T, double r, double sigma, int steps);
+/* Bermudan put on the same lattice: exercise only at t = k T / dates,
+ * k = 0..dates, with steps_per_date lattice steps between dates. */
+double orc_crr_bermudan_put(double S, double K, double T, double r, double sigma, int dates, int steps_per_date);
 /* Geometric-average Asian call with discrete monitoring t_k = k T/N, k=1..N,
  * under GBM with volatility sigma (closed form; the xi = 0, v0 = theta
  * limit of the Heston model with sigma = sqrt(v0)). */
@@ -52,6 +55,31 @@ void orc_heston_philox(int n_opt, int n_paths, int n_steps, double r, double rho
                        const double* X, const double* T, const double* v0, const double* kappa,
                        const double* theta, const double* xi, uint64_t seed, uint32_t stream,
                        uint32_t quirks, double* res, double* conf, double* greeks6);
+
+/* ---- LIBOR market model ------------------------------------------------- */
+#define ORC_Q_LIBOR_INPLACE 4u
+/* One path: forward rates from the initial curve L0[80] and z[40]; returns
+ * the discounted portfolio value and writes dV/dL0 into Lb[80]. */
+double orc_libor_path(const double* z, const double* L0, double lambda, uint32_t quirks, double* Lb);
+/* value[2] = {mean, se}; deltas[160] = {means, ses}.  z: [n_paths][40]
+ * (buffer) or NULL for the engine's Philox normals. */
+void orc_libor(int n_paths, double L0, double lambda, const double* z, uint64_t seed, uint32_t stream,
+               uint32_t quirks, double* value, double* deltas);
+
+/* ---- Longstaff-Schwartz American put ------------------------------------ */
+/* Reference algorithm (refcompat): one option, z[n_steps][n_paths] normals,
+ * quadratic basis, ridge 0.01, cofactor inverse, lagged betas, no rate. */
+void orc_lsmc_ref(int n_paths, int n_steps, double spot, double strike, double vol, double maturity,
+                  const double* z, double* american, double* european);
+/* Correct LSMC: cash flows discounted to 0, cubic basis in the standardized
+ * moneyness x = 1 + (S/K - 1)/(sigma sqrt(t)), FP64 Cholesky per date.
+ * price2 = [n][2] {price, se}; european may be NULL. */
+void orc_lsmc(int n_opt, const double* S0, const double* K, const double* sigma, const double* r,
+              const double* T, const int* steps, const int* paths, uint64_t seed, uint32_t stream,
+              double* price2, double* european);
+/* 4x4 SPD solve used by both the oracle and the kernels; returns 0 on a
+ * non-positive pivot. */
+int orc_chol4_solve(const double* G, const double* rhs, double* beta);
 
 #ifdef __cplusplus
 }

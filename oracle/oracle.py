@@ -51,6 +51,17 @@ def lib():
                                         [_D, _D, ctypes.c_uint32, _D, _D])
         L.orc_heston_philox.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 2 + [_D] * 7 +
                                         [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _D, _D, _D])
+        L.orc_libor_path.restype = ctypes.c_double
+        L.orc_libor_path.argtypes = [_D, _D, ctypes.c_double, ctypes.c_uint32, _D]
+        L.orc_libor.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, ctypes.c_uint64,
+                                ctypes.c_uint32, ctypes.c_uint32, _D, _D]
+        L.orc_crr_bermudan_put.restype = ctypes.c_double
+        L.orc_crr_bermudan_put.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int, ctypes.c_int]
+        L.orc_lsmc_ref.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [_D, _D, _D]
+        _I = ctypes.POINTER(ctypes.c_int)
+        L.orc_lsmc.argtypes = [ctypes.c_int] + [_D] * 5 + [_I, _I, ctypes.c_uint64, ctypes.c_uint32, _D, _D]
+        L.orc_chol4_solve.restype = ctypes.c_int
+        L.orc_chol4_solve.argtypes = [_D, _D, _D]
         _lib = L
     return _lib
 
@@ -103,6 +114,10 @@ def crr_european_call(S, X, T, r, sigma, steps=2048):
 
 def crr_american_put(S, K, T, r, sigma, steps=4096):
     return lib().orc_crr_american_put(S, K, T, r, sigma, steps)
+
+
+def crr_bermudan_put(S, K, T, r, sigma, dates, steps_per_date):
+    return lib().orc_crr_bermudan_put(S, K, T, r, sigma, dates, steps_per_date)
 
 
 def geometric_asian_call(S, X, T, r, sigma, N):
@@ -168,3 +183,58 @@ def heston_philox(S, X, T, v0, kappa, theta, xi, r, rho, n_paths, n_steps, seed,
     lib().orc_heston_philox(n_opt, n_paths, n_steps, r, rho, *ptrs, seed, stream, quirks, pres, pconf,
                             pg if greeks else None)
     return (res, conf, g) if greeks else (res, conf)
+
+
+# ---- LIBOR -------------------------------------------------------------------
+Q_LIBOR_INPLACE = 4
+
+
+def libor_path(z, L0=0.05, lam=0.2, quirks=0):
+    """One path; L0 is a scalar (flat curve) or an array of 80 rates."""
+    z, pz = _d(z)
+    curve, pc = _d(np.broadcast_to(np.asarray(L0, dtype=np.float64), (80,)))
+    Lb, pLb = _d(np.zeros(80))
+    v = lib().orc_libor_path(pz, pc, lam, quirks, pLb)
+    return v, Lb
+
+
+def libor(n_paths, L0=0.05, lam=0.2, z=None, seed=0, stream=0, quirks=0):
+    """Returns (value, value_se, deltas[80], deltas_se[80])."""
+    val, pval = _d(np.zeros(2))
+    dl, pdl = _d(np.zeros(160))
+    pz = None
+    if z is not None:
+        z, pz = _d(np.asarray(z).reshape(n_paths, 40))
+    lib().orc_libor(n_paths, L0, lam, pz, seed, stream, quirks, pval, pdl)
+    return val[0], val[1], dl[:80].copy(), dl[80:].copy()
+
+
+# ---- Longstaff-Schwartz ---------------------------------------------------------
+def lsmc_ref(n_paths, n_steps, spot, strike, vol, maturity, z):
+    z, pz = _d(np.asarray(z).reshape(-1))
+    assert z.size == n_paths * n_steps
+    am, eu = ctypes.c_double(), ctypes.c_double()
+    lib().orc_lsmc_ref(n_paths, n_steps, spot, strike, vol, maturity, pz, ctypes.byref(am), ctypes.byref(eu))
+    return am.value, eu.value
+
+
+def lsmc(S0, K, sigma, r, T, steps, paths, seed, stream=0):
+    """Returns (price[n], se[n], european[n])."""
+    n = len(S0)
+    arrs = [_d(a) for a in (S0, K, sigma, r, T)]
+    st = np.ascontiguousarray(steps, dtype=np.int32)
+    pa = np.ascontiguousarray(paths, dtype=np.int32)
+    _I = ctypes.POINTER(ctypes.c_int)
+    pr, ppr = _d(np.zeros((n, 2)))
+    eu, peu = _d(np.zeros(n))
+    lib().orc_lsmc(n, *[p for _, p in arrs], st.ctypes.data_as(_I), pa.ctypes.data_as(_I), seed, stream, ppr,
+                   peu)
+    return pr[:, 0].copy(), pr[:, 1].copy(), eu
+
+
+def chol4_solve(G, rhs):
+    G, pG = _d(np.asarray(G).reshape(16))
+    rhs, pr = _d(rhs)
+    beta, pb = _d(np.zeros(4))
+    ok = lib().orc_chol4_solve(pG, pr, pb)
+    return beta if ok else None

@@ -1,0 +1,114 @@
+"""LIBOR swaption portfolio and adjoint deltas on the GPU against the CPU
+oracle (a restatement of liborSwaptionGreeks.cpp:34-211 pinned by per-rate
+finite differences).  FP64 agrees with the oracle to <= 1e-12 relative on the
+same normals; FP32 errors are stated against the FP64 GPU run in units of the
+standard error."""
+import numpy as np
+import pytest
+
+import mcengine as mc
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+
+
+@pytest.mark.parametrize("quirks", [0, mc.QUIRK_LIBOR_INPLACE_ADJ])
+def test_buffer_fp64_matches_oracle(engine, quirks):
+    n = 1000
+    z = np.random.default_rng(1).standard_normal((n, 40))
+    v, vse, d, dse = engine.libor(n, mc.mode(mc.FP64, mc.RNG_BUFFER, quirks=quirks), L0=0.051, z=z)
+    oq = orc.Q_LIBOR_INPLACE if quirks else 0
+    ov, ovse, od, odse = orc.libor(n, L0=0.051, z=z, quirks=oq)
+    assert abs(v - ov) <= 1e-12 * abs(ov)
+    assert abs(vse - ovse) <= 1e-10 * ovse
+    assert _rel(d, od) <= 1e-12
+    assert _rel(dse, odse) <= 1e-9
+
+
+def test_buffer_fp32_matches_oracle(engine):
+    n = 2048
+    z = np.random.default_rng(2).standard_normal((n, 40))
+    v, vse, d, dse = engine.libor(n, mc.mode(mc.FP32, mc.RNG_BUFFER), L0=0.051, z=z)
+    ov, ovse, od, odse = orc.libor(n, L0=0.051, z=z)
+    assert abs(v - ov) <= 1e-5 * abs(ov)
+    assert np.all(np.abs(d - od) <= 1e-5 * np.abs(od) + 1e-3 * odse)
+
+
+@pytest.mark.parametrize("n", [4096, 1001])
+def test_philox_fp64_matches_oracle(engine, n):
+    v, vse, d, dse = engine.libor(n, mc.mode(mc.FP64, seed=17, stream=5))
+    ov, ovse, od, odse = orc.libor(n, seed=17, stream=5)
+    assert abs(v - ov) <= 1e-12 * abs(ov)
+    assert _rel(d, od) <= 1e-12
+
+
+def test_normals_seam_equals_philox(engine):
+    # generateRand replacement: buffer mode on the filled z == Philox mode
+    n = 512
+    m = mc.mode(mc.FP64, seed=23)
+    z = engine.libor_normals(n, m)
+    a = engine.libor(n, m)
+    b = engine.libor(n, mc.mode(mc.FP64, mc.RNG_BUFFER), z=z)
+    assert abs(a[0] - b[0]) <= 1e-14 * abs(a[0])
+    assert np.allclose(a[2], b[2], rtol=1e-14, atol=0)
+    zs = engine.libor_normals(n, m, mean=1.0, stddev=2.0)
+    assert np.allclose(zs, 1.0 + 2.0 * z, rtol=1e-15, atol=1e-15)
+
+
+def test_config2_fp32_vs_fp64(engine):
+    # Config 2: 262144 paths, L0 = 0.05, lambda = 0.2, FP32 vs FP64 on the
+    # same Philox stream; sum of deltas vs a flat-curve central difference.
+    n = 262144
+    v32, s32, d32, ds32 = engine.libor(n, mc.mode(mc.FP32, seed=1), L0=0.05)
+    v64, s64, d64, ds64 = engine.libor(n, mc.mode(mc.FP64, seed=1), L0=0.05)
+    assert abs(v32 - v64) < 0.01 * s64
+    assert np.all(np.abs(d32 - d64) < 0.01 * ds64)
+    h = 1e-6
+    up = engine.libor(n, mc.mode(mc.FP64, seed=1), L0=0.05 + h)[0]
+    dn = engine.libor(n, mc.mode(mc.FP64, seed=1), L0=0.05 - h)[0]
+    assert np.sum(d64) == pytest.approx((up - dn) / (2 * h), rel=1e-5)
+
+
+def test_deterministic_and_async(engine):
+    import torch
+    n = 3000
+    m = mc.mode(mc.FP32, seed=3)
+    a = engine.libor(n, m)
+    b = engine.libor(n, m)
+    assert a[0] == b[0] and np.array_equal(a[2], b[2])
+    dev = torch.device("cuda:0")
+    ov = torch.zeros(2, dtype=torch.float64, device=dev)
+    od = torch.zeros(160, dtype=torch.float64, device=dev)
+    ws = torch.empty(mc.libor_workspace(n, m), dtype=torch.uint8, device=dev)
+    mc.libor_async(n, m, ov, od, ws)
+    torch.cuda.synchronize()
+    assert ov[0].item() == a[0] and np.array_equal(od[:80].cpu().numpy(), a[2])
+
+
+def test_errors(engine):
+    with pytest.raises(mc.FsmcError):
+        engine.libor(1, mc.mode())
+    with pytest.raises(mc.FsmcError):
+        engine.libor(100, mc.mode(rng=mc.RNG_BUFFER))
+
+
+@pytest.mark.parametrize("build,rel", [("dp", 1e-5), ("sp", 1e-4)])
+def test_sample_command_line(build, rel):
+    # bin/libor_{dp,sp}: the sample's main over the drop-in liborSwaptionGreeks;
+    # its printed value/lambda are 10x the per-path means (NLOOPS) on the
+    # generateRand (seed 12345) normals, L0 = 0.051, lambda = 0.2.
+    import os
+    import re
+    import subprocess
+    exe = os.path.join(mc.HERE, "bin", f"libor_{build}")
+    r = subprocess.run([exe, "2048"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"Result: value=([-0-9.eE+]+),\s*lambda=([-0-9.eE+]+)", r.stdout)
+    assert m and re.search(r"Total time: [0-9.eE+]+ ms", r.stdout), r.stdout
+    ov, _, od, _ = orc.libor(2048, L0=0.051, seed=12345)
+    assert float(m.group(1)) == pytest.approx(10 * ov, rel=rel)
+    assert float(m.group(2)) == pytest.approx(10 * od[79], rel=rel)

@@ -240,3 +240,142 @@ def test_heston_oracle_greeks_vs_fd():
     assert g[0, 0] == pytest.approx(fd_delta, rel=2e-3)
     assert g[0, 1] == pytest.approx(fd_vega, rel=2e-3)
     assert g[0, 2] == pytest.approx(fd_rho, rel=2e-3)
+
+
+# ---- LIBOR oracle ----------------------------------------------------------------------
+def _libor_fd(z, L0, h=1e-6):
+    fd = np.zeros(80)
+    for i in range(80):
+        a, b = L0.copy(), L0.copy()
+        a[i] += h
+        b[i] -= h
+        fd[i] = (orc.libor_path(z, a)[0] - orc.libor_path(z, b)[0]) / (2 * h)
+    return fd
+
+
+def test_libor_oracle_adjoint_matches_finite_differences():
+    # path_calc_b2 + portfolio_b with a separate L_b is the exact derivative
+    # of the per-path value: central differences on each initial rate.
+    rng = np.random.default_rng(3)
+    checked = 0
+    for _ in range(6):
+        z = rng.standard_normal(40)
+        L0 = rng.uniform(0.04, 0.06, 80)
+        v, Lb = orc.libor_path(z, L0)
+        if v == 0.0:
+            continue
+        fd = _libor_fd(z, L0)
+        assert np.max(np.abs(Lb - fd)) < 1e-7 * np.max(np.abs(fd))
+        checked += 1
+    assert checked >= 3
+
+
+def test_libor_oracle_inplace_quirk_reproduces_reference_defect():
+    # liborSwaptionGreeks.cpp:178 overwrites L[n] with L_b[n] and :181 reads
+    # it back: deltas 1..78 go wrong, 0 and 79 (the one the sample prints) stay
+    # exact, and the value is unaffected.
+    rng = np.random.default_rng(3)
+    for _ in range(6):
+        z = rng.standard_normal(40)
+        L0 = np.full(80, 0.051)
+        v, Lb = orc.libor_path(z, L0)
+        vq, Lq = orc.libor_path(z, L0, quirks=orc.Q_LIBOR_INPLACE)
+        assert vq == v
+        if v == 0.0:
+            continue
+        fd = _libor_fd(z, L0)
+        scale = np.max(np.abs(fd))
+        assert abs(Lq[0] - fd[0]) < 1e-7 * scale and abs(Lq[79] - fd[79]) < 1e-7 * scale
+        assert np.max(np.abs(Lq[1:79] - fd[1:79])) > 0.1 * scale
+        return
+    raise AssertionError("no in-the-money path drawn")
+
+
+def test_libor_oracle_zero_vol_closed_form():
+    # lambda = 0: rates stay on the flat curve, the payer swap value is
+    # B_m + K S_m - 1 = S_m (K - L0) with B_m = (1 + d L0)^-(m+1) and
+    # S_m = d sum_{k<=m} B_k; swaptions pay 100 max(-(.), 0) discounted by
+    # (1 + d L0)^-40.
+    for L0 in (0.05, 0.051, 0.047):
+        d = 0.25
+        B = (1 + d * L0) ** -np.arange(1, 41)
+        S = d * np.cumsum(B)
+        mats = [4, 4, 4, 8, 8, 8, 20, 20, 20, 28, 28, 28, 40, 40, 40]
+        rates = np.array([.045, .05, .055] * 5, dtype=np.float32).astype(np.float64)
+        val = sum(100 * max(-(B[m - 1] + K * S[m - 1] - 1), 0.0) for m, K in zip(mats, rates))
+        val *= (1 + d * L0) ** -40
+        v, _ = orc.libor_path(np.random.default_rng(0).standard_normal(40), L0, lam=0.0)
+        assert v == pytest.approx(val, rel=1e-13)
+
+
+def test_libor_oracle_statistics_stable():
+    # Philox mode: the value is within 4 SE across independent streams
+    a = orc.libor(4096, L0=0.05, seed=1)
+    b = orc.libor(4096, L0=0.05, seed=2)
+    assert abs(a[0] - b[0]) < 4 * math.hypot(a[1], b[1])
+
+
+# ---- Longstaff-Schwartz oracle ---------------------------------------------------------
+def test_chol4_solve():
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((6, 4))
+    G = A.T @ A
+    b = rng.standard_normal(4)
+    assert np.allclose(orc.chol4_solve(G, b), np.linalg.solve(G, b), rtol=1e-12)
+    assert orc.chol4_solve(np.diag([1.0, 1.0, 0.0, 1.0]), b) is None
+
+
+def test_lsmc_oracle_longstaff_schwartz_table():
+    # Longstaff & Schwartz (2001) Table 1 (50 exercise dates per year, 100k
+    # paths): LSM 4.472 / 2.313 / 1.110 for S0 = 36 / 40 / 44, K = 40, r = .06,
+    # sigma = .2, T = 1.  The oracle's prices sit within 3 SE + the regression
+    # bias of the lattice value (LSMC is biased low).
+    S0 = [36.0, 40.0, 44.0]
+    n = len(S0)
+    p, se, eu = orc.lsmc(S0, [40.0] * n, [.2] * n, [.06] * n, [1.0] * n, [50] * n, [100000] * n, seed=1)
+    for i in range(n):
+        crr = orc.crr_american_put(S0[i], 40, 1, .06, .2, 4096)
+        assert crr - 3 * se[i] - 0.02 < p[i] < crr + 3 * se[i]
+        # European put on the same paths vs Black-Scholes parity
+        bs_put = orc.bs_call(S0[i], 40, 1, .06, .2) - S0[i] + 40 * math.exp(-.06)
+        assert abs(eu[i] - bs_put) < 0.05
+    assert p == pytest.approx([4.472, 2.313, 1.110], abs=0.03)
+
+
+def test_lsmc_oracle_zero_rate_no_early_exercise():
+    # r = 0, no dividends: early exercise of a put is never optimal; LSMC
+    # can only lose by exercising, so American <= European + noise, and it
+    # stays close to the European value.
+    p, se, eu = orc.lsmc([100.0], [100.0], [.3], [0.0], [1.0], [50], [50000], seed=3)
+    assert p[0] <= eu[0] + 3 * se[0]
+    assert p[0] > eu[0] - 0.05
+
+
+def test_lsmc_ref_oracle_reference_quirks():
+    # The sample's algorithm: no rate, so its "American" and "European" prices
+    # estimate the same quantity; the European leg is a plain GBM put average
+    # equal to Black-Scholes with r = 0 within MC error.
+    rng = np.random.default_rng(5)
+    n_paths, n_steps = 16384, 64
+    z = rng.standard_normal(n_paths * n_steps)
+    am, eu = orc.lsmc_ref(n_paths, n_steps, 100.0, 110.0, .25, 2.0, z)
+    bs_put = orc.bs_call(100, 110, 2, 0.0, .25) - 100 + 110
+    assert abs(eu - bs_put) < 0.5
+    assert abs(am - eu) < 1.5
+    # a path that never enters the money has zero cash flow; with strike 0 the
+    # put is worthless in both legs
+    am0, eu0 = orc.lsmc_ref(256, 8, 100.0, 1e-9, .25, 2.0, z[:256 * 8])
+    assert am0 == 0.0 and eu0 == 0.0
+
+
+def test_crr_bermudan_put():
+    # every lattice step an exercise date -> the American lattice
+    am = orc.crr_american_put(36, 40, 1, .06, .2, 1024)
+    assert orc.crr_bermudan_put(36, 40, 1, .06, .2, 1024, 1) == pytest.approx(am, rel=1e-14)
+    # one date (t = 0 and T) -> max(K - S0, European)
+    eu = orc.crr_european_call(36, 40, 1, .06, .2, 2048) - 36 + 40 * math.exp(-.06)
+    assert orc.crr_bermudan_put(36, 40, 1, .06, .2, 1, 2048) == pytest.approx(max(4.0, eu), abs=2e-3)
+    # more exercise dates are worth more, bounded by the American
+    b10 = orc.crr_bermudan_put(36, 40, 1, .06, .2, 10, 400)
+    b50 = orc.crr_bermudan_put(36, 40, 1, .06, .2, 50, 80)
+    assert eu < b10 < b50 < orc.crr_american_put(36, 40, 1, .06, .2, 4000) + 1e-3
