@@ -1,0 +1,469 @@
+// Mixed serving batch: GBM European and Heston-Asian requests of uneven
+// sizes, each returning price, delta, vega and rho (with standard errors).
+//
+// Requests are served in micro-batches of `requests_per_launch` in arrival
+// order.  Each micro-batch is three launches over a table of (request, chunk)
+// work items: one grid of GBM items (pathwise Greeks), one grid of Heston
+// items (the reverse sweep of heston_adjoint_path, checkpoints in a slot per
+// launched thread), and a finalize grid that combines each request's item
+// partials in item order (bitwise reproducible) into mean and standard error.
+// A HIP event after each micro-batch gives the per-request completion time;
+// an autotuner times chunk size x micro-batch size and keeps the fastest.
+// A request's normals and per-path arithmetic are those of the standalone
+// entry point with the request index as the option index.
+#include <algorithm>
+#include <map>
+#include <vector>
+
+#include "fsmc_device.hpp"
+#include "fsmc_host.hpp"
+#include "path_models.hpp"
+
+namespace fsmc {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kNV = 8;  // sums of price, delta, vega, rho and their squares
+
+struct ReqDev {
+  int kind, n_paths, n_steps, pad;
+  double S0, K, T, r, sigma, v0, kappa, theta, xi, rho;
+};
+
+struct BatchArgs {
+  const ReqDev* req;
+  const int2* items;    // (request, chunk), GBM items then Heston items per micro-batch
+  double* partials;     // [item][kNV]
+  void* ckpt;           // Heston checkpoints [segment][S|v][thread of the launch]
+  int chunk;
+  uint32_t k0, k1, w_gbm, w_heston;
+};
+
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void batch_gbm_kernel(BatchArgs a, int first_item) {
+  __shared__ double lds[kNV * (kBlock / 64)];
+  const int item = first_item + blockIdx.x;
+  const int2 e = a.items[item];
+  const ReqDev q = a.req[e.x];
+  const int p0 = e.y * a.chunk;
+  const int p1 = min(q.n_paths, p0 + a.chunk);
+  const double disc = exp(-q.r * q.T), fd = disc / q.S0, kt = disc * q.K * q.T;
+  const GbmConsts<Real> g(q.S0, q.K, q.T, q.r, q.sigma);
+  double s[kNV] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // Philox layout of the GBM entry point: counter (path/4, 0, request)
+  for (int base = p0 + 4 * (int)threadIdx.x; base < p1; base += 4 * kBlock) {
+    Real z[4];
+    normals4<Real>(U4{(uint32_t)(base >> 2), 0u, (uint32_t)e.x, a.w_gbm}, a.k0, a.k1, z[0], z[1], z[2], z[3]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (base + j < p1) {
+        Real st, dlog;
+        const Real c = gbm_path(g, z[j], st, dlog);
+        if (c > Real(0)) {
+          const double pr = disc * (double)c, de = fd * (double)st, ve = disc * (double)(st * dlog);
+          s[0] += pr;
+          s[1] += pr * pr;
+          s[2] += de;
+          s[3] += de * de;
+          s[4] += ve;
+          s[5] += ve * ve;
+          s[6] += kt;
+          s[7] += kt * kt;
+        }
+      }
+    }
+  }
+  block_sum<kNV>(s, lds);
+  if (threadIdx.x == 0) {
+    double* dst = a.partials + (size_t)item * kNV;
+#pragma unroll
+    for (int k = 0; k < kNV; ++k) dst[k] = s[k];
+  }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void batch_heston_kernel(BatchArgs a, int first_item) {
+  __shared__ double lds[kNV * (kBlock / 64)];
+  const int item = first_item + blockIdx.x;
+  const int2 e = a.items[item];
+  const ReqDev q = a.req[e.x];
+  const int p0 = e.y * a.chunk;
+  const int p1 = min(q.n_paths, p0 + a.chunk);
+  const double disc = exp(-q.r * q.T);
+  const HestonConsts<Real> c(q.T, q.n_steps, q.r, q.kappa, q.xi, q.theta, q.rho);
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  Real* ck = (Real*)a.ckpt + (size_t)blockIdx.x * kBlock + threadIdx.x;
+  double s[kNV] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int p = p0 + (int)threadIdx.x; p < p1; p += kBlock) {
+    double g[3];
+    const double x = heston_adjoint_path<Real>(c, q.S0, q.K, q.T, q.v0, q.r, q.n_steps, (uint32_t)p, (uint32_t)e.x,
+                                               a.w_heston, a.k0, a.k1, ck, stride, g);
+    const double pr = disc * x;
+    s[0] += pr;
+    s[1] += pr * pr;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      s[2 + 2 * k] += g[k];
+      s[3 + 2 * k] += g[k] * g[k];
+    }
+  }
+  block_sum<kNV>(s, lds);
+  if (threadIdx.x == 0) {
+    double* dst = a.partials + (size_t)item * kNV;
+#pragma unroll
+    for (int k = 0; k < kNV; ++k) dst[k] = s[k];
+  }
+}
+
+// One thread per request of the micro-batch [r0, r0 + n): its items are
+// first_item[r] .. first_item[r] + n_items[r] - 1 (contiguous per kind).
+__global__ void batch_finalize_kernel(const ReqDev* req, const int* first_item, const int* n_items,
+                                      const double* partials, int r0, int n, double* results) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = r0 + i;
+  double t[kNV] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int it = 0; it < n_items[r]; ++it)
+    for (int k = 0; k < kNV; ++k) t[k] += partials[(size_t)(first_item[r] + it) * kNV + k];
+  const double N = (double)req[r].n_paths;
+  double* res = results + (size_t)r * 8;
+  for (int k = 0; k < 4; ++k) {
+    res[k] = t[2 * k] / N;
+    const double var = (N * t[2 * k + 1] - t[2 * k] * t[2 * k]) / (N * (N - 1.0));
+    res[4 + k] = sqrt(var > 0.0 ? var : 0.0) / sqrt(N);
+  }
+}
+
+}  // namespace
+
+struct MicroBatch {
+  int r0, n;                 // requests [r0, r0 + n)
+  int gbm_first, gbm_count;  // items
+  int hes_first, hes_count;
+};
+
+struct BatchImpl {
+  int n = 0;
+  fsmc_mode mode;
+  std::vector<fsmc_request> req;
+  int max_seg = 1;
+  fsmc_batch_tuning tuning = {4096, 0};
+  std::vector<MicroBatch> waves;
+  int max_hes_grid = 0;
+  DeviceBuffer ws;
+  size_t off_req = 0, off_items = 0, off_first = 0, off_nitems = 0, off_part = 0, off_res = 0, off_ckpt = 0;
+  std::vector<hipEvent_t> done;  // one per micro-batch
+  hipEvent_t start = nullptr;
+};
+
+static std::map<std::vector<long long>, fsmc_batch_tuning>& tuning_cache() {
+  static std::map<std::vector<long long>, fsmc_batch_tuning> c;
+  return c;
+}
+static std::mutex g_cache_mu;
+
+static std::vector<long long> batch_key(const BatchImpl& b) {
+  // batches of the same size and mix share a tuning
+  std::vector<long long> k = {b.mode.precision, b.n};
+  long long gp = 0, hp = 0, hs = 0;
+  for (const auto& r : b.req) {
+    if (r.kind == FSMC_REQ_GBM_EUROPEAN) {
+      gp += r.n_paths;
+    } else {
+      hp += r.n_paths;
+      hs += (long long)r.n_paths * r.n_steps;
+    }
+  }
+  k.push_back(gp >> 16);
+  k.push_back(hp >> 16);
+  k.push_back(hs >> 20);
+  return k;
+}
+
+static void batch_release_events(BatchImpl& b) {
+  for (hipEvent_t e : b.done) (void)hipEventDestroy(e);
+  b.done.clear();
+}
+
+// Builds the item tables and workspace for a tuning (caller holds ctx->mu).
+static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
+  FSMC_REQUIRE(t.chunk_paths >= 4 * kBlock && t.chunk_paths % (4 * kBlock) == 0,
+               "chunk_paths %d must be a positive multiple of %d", t.chunk_paths, 4 * kBlock);
+  FSMC_REQUIRE(t.requests_per_launch >= 0, "requests_per_launch %d < 0", t.requests_per_launch);
+  FSMC_HIP(hipSetDevice(ctx->device));
+  FSMC_HIP(hipStreamSynchronize(ctx->stream));
+  batch_release_events(b);
+  b.tuning = t;
+  const int per = t.requests_per_launch > 0 ? std::min(t.requests_per_launch, b.n) : b.n;
+  std::vector<ReqDev> rd(b.n);
+  std::vector<int2> items;
+  std::vector<int> first(b.n), nitems(b.n);
+  b.waves.clear();
+  b.max_hes_grid = 0;
+  for (int r0 = 0; r0 < b.n; r0 += per) {
+    MicroBatch mb;
+    mb.r0 = r0;
+    mb.n = std::min(per, b.n - r0);
+    for (int pass = 0; pass < 2; ++pass) {  // GBM items, then Heston items
+      const int kind = pass == 0 ? FSMC_REQ_GBM_EUROPEAN : FSMC_REQ_HESTON_ASIAN;
+      const int begin = (int)items.size();
+      for (int i = r0; i < r0 + mb.n; ++i) {
+        if (b.req[i].kind != kind) continue;
+        first[i] = (int)items.size();
+        nitems[i] = (b.req[i].n_paths + t.chunk_paths - 1) / t.chunk_paths;
+        for (int c = 0; c < nitems[i]; ++c) items.push_back(make_int2(i, c));
+      }
+      if (pass == 0) {
+        mb.gbm_first = begin;
+        mb.gbm_count = (int)items.size() - begin;
+      } else {
+        mb.hes_first = begin;
+        mb.hes_count = (int)items.size() - begin;
+        b.max_hes_grid = std::max(b.max_hes_grid, mb.hes_count);
+      }
+    }
+    b.waves.push_back(mb);
+  }
+  for (int i = 0; i < b.n; ++i) {
+    const fsmc_request& r = b.req[i];
+    ReqDev& d = rd[i];
+    d.kind = r.kind;
+    d.n_paths = r.n_paths;
+    d.n_steps = r.n_steps;
+    d.pad = 0;
+    d.S0 = r.S0;
+    d.K = r.K;
+    d.T = r.T;
+    d.r = r.r;
+    d.sigma = r.sigma;
+    d.v0 = r.v0;
+    d.kappa = r.kappa;
+    d.theta = r.theta;
+    d.xi = r.xi;
+    d.rho = r.rho;
+  }
+  const size_t real = b.mode.precision == FSMC_FP64 ? 8 : 4;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = align_up(off + bytes, 256);
+    return at;
+  };
+  b.off_req = take(sizeof(ReqDev) * b.n);
+  b.off_items = take(sizeof(int2) * items.size());
+  b.off_first = take(sizeof(int) * b.n);
+  b.off_nitems = take(sizeof(int) * b.n);
+  b.off_part = take(sizeof(double) * kNV * items.size());
+  b.off_res = take(sizeof(double) * 8 * b.n);
+  b.off_ckpt = take(real * 2 * (size_t)b.max_seg * (size_t)std::max(1, b.max_hes_grid) * kBlock);
+  FSMC_TRY(b.ws.reserve(off));
+  char* ws = (char*)b.ws.ptr;
+  hipStream_t s = ctx->stream;
+  FSMC_HIP(hipMemcpyAsync(ws + b.off_req, rd.data(), sizeof(ReqDev) * b.n, hipMemcpyHostToDevice, s));
+  FSMC_HIP(hipMemcpyAsync(ws + b.off_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice, s));
+  FSMC_HIP(hipMemcpyAsync(ws + b.off_first, first.data(), sizeof(int) * b.n, hipMemcpyHostToDevice, s));
+  FSMC_HIP(hipMemcpyAsync(ws + b.off_nitems, nitems.data(), sizeof(int) * b.n, hipMemcpyHostToDevice, s));
+  FSMC_HIP(hipStreamSynchronize(s));
+  b.done.resize(b.waves.size(), nullptr);
+  for (auto& e : b.done) FSMC_HIP(hipEventCreate(&e));
+  return FSMC_OK;
+}
+
+template <typename Real>
+static int batch_enqueue(BatchImpl& b, hipStream_t s) {
+  char* ws = (char*)b.ws.ptr;
+  BatchArgs a;
+  a.req = (const ReqDev*)(ws + b.off_req);
+  a.items = (const int2*)(ws + b.off_items);
+  a.partials = (double*)(ws + b.off_part);
+  a.ckpt = ws + b.off_ckpt;
+  a.chunk = b.tuning.chunk_paths;
+  a.k0 = (uint32_t)b.mode.seed;
+  a.k1 = (uint32_t)(b.mode.seed >> 32);
+  a.w_gbm = counter_w(kTagGbm, b.mode.stream);
+  a.w_heston = counter_w(kTagHeston, b.mode.stream);
+  FSMC_HIP(hipEventRecord(b.start, s));
+  for (size_t w = 0; w < b.waves.size(); ++w) {
+    const MicroBatch& mb = b.waves[w];
+    if (mb.gbm_count > 0)
+      hipLaunchKernelGGL(batch_gbm_kernel<Real>, dim3(mb.gbm_count), dim3(kBlock), 0, s, a, mb.gbm_first);
+    if (mb.hes_count > 0)
+      hipLaunchKernelGGL(batch_heston_kernel<Real>, dim3(mb.hes_count), dim3(kBlock), 0, s, a, mb.hes_first);
+    hipLaunchKernelGGL(batch_finalize_kernel, dim3((mb.n + 63) / 64), dim3(64), 0, s, a.req,
+                       (const int*)(ws + b.off_first), (const int*)(ws + b.off_nitems), a.partials, mb.r0, mb.n,
+                       (double*)(ws + b.off_res));
+    FSMC_HIP(hipGetLastError());
+    FSMC_HIP(hipEventRecord(b.done[w], s));
+  }
+  return FSMC_OK;
+}
+
+}  // namespace fsmc
+
+using namespace fsmc;
+
+struct fsmc_batch {
+  fsmc_ctx* ctx;
+  BatchImpl impl;
+};
+
+extern "C" {
+
+int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_mode* m,
+                      const fsmc_batch_tuning* tuning, fsmc_batch** out) {
+  FSMC_REQUIRE(ctx && req && out, "NULL argument");
+  *out = nullptr;
+  FSMC_REQUIRE(n >= 1, "empty batch");
+  FSMC_TRY(validate_mode(m, false));
+  FSMC_REQUIRE(m->quirks == 0, "quirks are not supported in batches");
+  int max_seg = 1;
+  for (int i = 0; i < n; ++i) {
+    const fsmc_request& r = req[i];
+    FSMC_REQUIRE(r.kind == FSMC_REQ_GBM_EUROPEAN || r.kind == FSMC_REQ_HESTON_ASIAN, "request %d: bad kind %d", i,
+                 r.kind);
+    FSMC_REQUIRE(r.n_paths >= 2, "request %d: n_paths %d < 2", i, r.n_paths);
+    FSMC_REQUIRE(r.S0 > 0 && r.K >= 0 && r.T > 0, "request %d: S0, T must be > 0 and K >= 0", i);
+    if (r.kind == FSMC_REQ_GBM_EUROPEAN) {
+      FSMC_REQUIRE(r.sigma > 0, "request %d: sigma must be > 0", i);
+    } else {
+      FSMC_REQUIRE(r.n_steps >= 1 && r.v0 > 0 && r.rho >= -1 && r.rho <= 1,
+                   "request %d: Heston needs n_steps >= 1, v0 > 0, |rho| <= 1", i);
+      max_seg = std::max(max_seg, (r.n_steps + kHestonSeg - 1) / kHestonSeg);
+    }
+  }
+  fsmc_batch* b = new fsmc_batch();
+  b->ctx = ctx;
+  BatchImpl& bi = b->impl;
+  bi.n = n;
+  bi.mode = *m;
+  bi.req.assign(req, req + n);
+  bi.max_seg = max_seg;
+  fsmc_batch_tuning t = {4096, 0};
+  if (tuning) {
+    t = *tuning;
+  } else {
+    std::lock_guard<std::mutex> cl(g_cache_mu);
+    auto it = tuning_cache().find(batch_key(bi));
+    if (it != tuning_cache().end()) t = it->second;
+  }
+  int st;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    st = hipSetDevice(ctx->device) == hipSuccess && hipEventCreate(&bi.start) == hipSuccess ? FSMC_OK
+                                                                                            : FSMC_ERR_DEVICE;
+    if (st == FSMC_OK) st = batch_configure(ctx, bi, t);
+  }
+  if (st != FSMC_OK) {
+    fsmc_batch_destroy(b);
+    return st;
+  }
+  *out = b;
+  return FSMC_OK;
+}
+
+int fsmc_batch_run(fsmc_batch* b, void* stream) {
+  FSMC_REQUIRE(b != nullptr, "batch is NULL");
+  FSMC_HIP(hipSetDevice(b->ctx->device));
+  hipStream_t s = stream ? (hipStream_t)stream : b->ctx->stream;
+  return b->impl.mode.precision == FSMC_FP64 ? batch_enqueue<double>(b->impl, s) : batch_enqueue<float>(b->impl, s);
+}
+
+int fsmc_batch_results(fsmc_batch* b, void* stream, fsmc_result* out) {
+  FSMC_REQUIRE(b && out, "NULL argument");
+  BatchImpl& bi = b->impl;
+  hipStream_t s = stream ? (hipStream_t)stream : b->ctx->stream;
+  std::vector<double> res((size_t)bi.n * 8);
+  FSMC_HIP(hipSetDevice(b->ctx->device));
+  FSMC_HIP(hipMemcpyAsync(res.data(), (char*)bi.ws.ptr + bi.off_res, res.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  FSMC_HIP(hipStreamSynchronize(s));
+  for (size_t w = 0; w < bi.waves.size(); ++w) {
+    float ms = 0.0f;
+    FSMC_HIP(hipEventElapsedTime(&ms, bi.start, bi.done[w]));
+    const MicroBatch& mb = bi.waves[w];
+    for (int i = mb.r0; i < mb.r0 + mb.n; ++i) out[i].latency_us = 1e3 * ms;
+  }
+  for (int i = 0; i < bi.n; ++i) {
+    fsmc_result& r = out[i];
+    const double* v = res.data() + (size_t)i * 8;
+    r.price = v[0];
+    r.delta = v[1];
+    r.vega = v[2];
+    r.rho = v[3];
+    r.price_se = v[4];
+    r.delta_se = v[5];
+    r.vega_se = v[6];
+    r.rho_se = v[7];
+  }
+  return FSMC_OK;
+}
+
+int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen) {
+  FSMC_REQUIRE(b != nullptr, "batch is NULL");
+  BatchImpl& bi = b->impl;
+  fsmc_ctx* ctx = b->ctx;
+  const int chunks[] = {1024, 2048, 4096, 8192, 16384};
+  const int pers[] = {0, 1024, 256};
+  fsmc_batch_tuning best = bi.tuning;
+  float best_ms = 1e30f;
+  int st = FSMC_OK;
+  for (int c : chunks) {
+    for (int per : pers) {
+      if (per >= bi.n) continue;
+      {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        st = batch_configure(ctx, bi, fsmc_batch_tuning{c, per});
+      }
+      if (st != FSMC_OK) break;
+      float ms_best = 1e30f;
+      for (int rep = 0; rep < 3 && st == FSMC_OK; ++rep) {
+        st = fsmc_batch_run(b, nullptr);
+        if (st != FSMC_OK) break;
+        if (hipEventSynchronize(bi.done.back()) != hipSuccess) {
+          set_error("autotune run failed");
+          st = FSMC_ERR_DEVICE;
+          break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, bi.start, bi.done.back());
+        if (rep > 0) ms_best = std::min(ms_best, ms);  // first run untimed
+      }
+      if (st != FSMC_OK) break;
+      if (ms_best < best_ms) {
+        best_ms = ms_best;
+        best = fsmc_batch_tuning{c, per};
+      }
+    }
+    if (st != FSMC_OK) break;
+  }
+  if (st != FSMC_OK) return st;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    FSMC_TRY(batch_configure(ctx, bi, best));
+  }
+  {
+    std::lock_guard<std::mutex> cl(g_cache_mu);
+    tuning_cache()[batch_key(bi)] = best;
+  }
+  if (chosen) *chosen = best;
+  return FSMC_OK;
+}
+
+int fsmc_batch_tuning_get(const fsmc_batch* b, fsmc_batch_tuning* out) {
+  FSMC_REQUIRE(b && out, "NULL argument");
+  *out = b->impl.tuning;
+  return FSMC_OK;
+}
+
+int fsmc_batch_destroy(fsmc_batch* b) {
+  if (!b) return FSMC_OK;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  batch_release_events(b->impl);
+  if (b->impl.start) (void)hipEventDestroy(b->impl.start);
+  b->impl.ws.release();
+  delete b;
+  return FSMC_OK;
+}
+
+}  // extern "C"

@@ -42,9 +42,10 @@ __device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
     const uint64_t p1 = (uint64_t)kPhiloxM1 * c.z;
     const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
     const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-    c.x = hi1 ^ c.y ^ k0;
+    // three-input XOR in one v_bitop3_b32 (truth table 0x96)
+    c.x = __builtin_amdgcn_bitop3_b32(hi1, c.y, k0, 0x96);
     c.y = lo1;
-    c.z = hi0 ^ c.w ^ k1;
+    c.z = __builtin_amdgcn_bitop3_b32(hi0, c.w, k1, 0x96);
     c.w = lo0;
     k0 += kPhiloxW0;
     k1 += kPhiloxW1;
@@ -92,23 +93,23 @@ __device__ __forceinline__ void normals4(U4 ctr, uint32_t k0, uint32_t k1, Real&
 }
 
 // ---------------------------------------------------------------------------
-// Compensated accumulation.  In FP32 mode a lane accumulates with a
-// branch-free two-sum (Knuth); the compensation is folded in before the
-// FP64 cross-lane combine.  Contraction into FMA is switched off locally so
-// that the error term survives -O3.
+// Compensated accumulation.  In FP32 mode a lane accumulates with Kahan's
+// compensated summation (error bounded by 2 ulp of the sum, independent of
+// the number of terms, four operations per term); the compensation is folded
+// in before the FP64 cross-lane combine.  Contraction into FMA is switched
+// off locally so that the error term survives -O3.
 // ---------------------------------------------------------------------------
 template <typename Real>
 struct CompSum {
-  Real s = 0, c = 0;
+  Real s = 0, c = 0;  // c = (what was added) - (what should have been added)
   __device__ __forceinline__ void add(Real x) {
 #pragma clang fp contract(off)
-    const Real t = s + x;
-    const Real bp = t - s;
-    const Real e = (s - (t - bp)) + (x - bp);
+    const Real y = x - c;
+    const Real t = s + y;
+    c = (t - s) - y;
     s = t;
-    c += e;
   }
-  __device__ __forceinline__ double value() const { return (double)s + (double)c; }
+  __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
 template <>

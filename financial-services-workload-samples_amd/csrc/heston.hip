@@ -10,13 +10,13 @@
 // sample's chunk-of-64 running val2 (:184-201) from per-wavefront chunk sums.
 #include "fsmc_device.hpp"
 #include "fsmc_host.hpp"
+#include "path_models.hpp"
 
 namespace fsmc {
 namespace {
 
 constexpr int kBlock = 256;
 constexpr int kChunk = 64;  // VECLEN of MCAsianOptions.cpp:29 == one wavefront
-constexpr double kLog2e = 1.4426950408889634074;
 
 struct HestonArgs {
   int n_paths, n_steps;
@@ -28,43 +28,12 @@ struct HestonArgs {
   double* chunks;    // [n_options][n_paths / 64]  (running-val2 estimator)
 };
 
-template <typename Real>
-struct HestonConsts {
-  Real c_rdt, c_hdt, c_sdt, kdt, xsdt, theta, rho, srho;
-  __device__ HestonConsts(const HestonArgs& a, int o) {
-    const double dt = a.T[o] / a.n_steps;
-    const double sdt = sqrt(dt);
-    c_rdt = (Real)(a.r * dt * kLog2e);
-    c_hdt = (Real)(0.5 * dt * kLog2e);
-    c_sdt = (Real)(sdt * kLog2e);
-    kdt = (Real)(a.kappa[o] * dt);
-    xsdt = (Real)(a.xi[o] * sdt);
-    theta = (Real)a.theta[o];
-    rho = (Real)a.rho;
-    srho = (Real)sqrt(1.0 - a.rho * a.rho);
-  }
-};
-
-// One Euler step; returns nothing, updates S, v and the running average sum.
-template <typename Real, bool kVarAsVol>
-__device__ __forceinline__ void heston_step(const HestonConsts<Real>& c, Real& S, Real& v, CompSum<Real>& avg,
-                                            Real z1, Real z2) {
-  const Real vp = v > Real(0) ? v : Real(0);
-  const Real sq = fsqrt(vp);
-  const Real sv = kVarAsVol ? vp : sq;
-  const Real e = c.c_rdt - c.c_hdt * sv * sv + c.c_sdt * sv * z1;
-  S = S * fexp2(e);
-  avg.add(S);
-  const Real zc = c.rho * z1 + c.srho * z2;
-  v = v + c.kdt * (c.theta - vp) + c.xsdt * sq * zc;
-}
-
 template <typename Real, bool kBuffer, bool kVarAsVol, bool kChunks>
 __global__ __launch_bounds__(kBlock) void heston_paths_kernel(HestonArgs a) {
   __shared__ double lds[2 * (kBlock / 64)];
   const int o = blockIdx.y;
   const int p = blockIdx.x * kBlock + threadIdx.x;
-  const HestonConsts<Real> c(a, o);
+  const HestonConsts<Real> c(a.T[o], a.n_steps, a.r, a.kappa[o], a.xi[o], a.theta[o], a.rho);
   double pay = 0.0;
   if (p < a.n_paths) {
     Real S = (Real)a.S[o];
@@ -75,20 +44,21 @@ __global__ __launch_bounds__(kBlock) void heston_paths_kernel(HestonArgs a) {
       const size_t base = ((size_t)o * nchunks + (p / kChunk)) * (size_t)a.n_steps * kChunk + (p % kChunk);
       const double* z1 = a.z1 + base;
       const double* z2 = a.z2 + base;
-      for (int k = 0; k < a.n_steps; ++k)
-        heston_step<Real, kVarAsVol>(c, S, v, avg, (Real)z1[(size_t)k * kChunk], (Real)z2[(size_t)k * kChunk]);
+      for (int k = 0; k < a.n_steps; ++k) {
+        heston_step<Real, kVarAsVol>(c, S, v, (Real)z1[(size_t)k * kChunk], (Real)z2[(size_t)k * kChunk]);
+        avg.add(S);
+      }
     } else {
       const int nfull = a.n_steps & ~1;
       for (int k = 0; k < nfull; k += 2) {
         Real n0, n1, n2, n3;
         normals4<Real>(U4{(uint32_t)p, (uint32_t)(k >> 1), (uint32_t)o, a.w}, a.k0, a.k1, n0, n1, n2, n3);
-        heston_step<Real, kVarAsVol>(c, S, v, avg, n0, n1);
-        heston_step<Real, kVarAsVol>(c, S, v, avg, n2, n3);
+        heston_pair<Real, kVarAsVol>(c, S, v, avg, n0, n1, n2, n3, true);
       }
       if (a.n_steps & 1) {
         Real n0, n1, n2, n3;
         normals4<Real>(U4{(uint32_t)p, (uint32_t)(nfull >> 1), (uint32_t)o, a.w}, a.k0, a.k1, n0, n1, n2, n3);
-        heston_step<Real, kVarAsVol>(c, S, v, avg, n0, n1);
+        heston_pair<Real, kVarAsVol>(c, S, v, avg, n0, n1, n2, n3, false);
       }
     }
     const double mean = avg.value() / (double)a.n_steps;
@@ -151,112 +121,28 @@ __global__ void heston_finalize_kernel(int n_options, int nbx, int n_paths, doub
   }
 }
 
-// ---------------------------------------------------------------------------
-// Adjoint (reverse sweep).  Pathwise derivatives of
-//   P = e^{-rT} max(A - K, 0),  A = (1/N) sum_{k=1..N} S_k,
-//   S_{k+1} = S_k exp(e_k),  e_k = r dt - vp_k dt / 2 + sqrt(vp_k dt) z1_k,
-//   v_{k+1} = v_k + kappa dt (theta - vp_k) + xi sqrt(vp_k dt) zc_k,
-// with vp = max(v, 0).  Going backwards with U_{k+1} = dP/de_k =
-// (e^{-rT}/N) sum_{j>k} S_j:
-//   vbar_k = vbar_{k+1} dv_{k+1}/dv_k + U_{k+1} de_k/dv_k,
-//   rbar   = -T P + dt sum_k U_{k+1},   dP/dS0 = U_1 / S0,
-// and vega = dP/dsqrt(v0) = 2 sqrt(v0) vbar_0.  The forward sweep stores
-// (S, v) every kSeg steps in the workspace; the reverse sweep recomputes one
-// segment at a time into registers, regenerating its normals from the Philox
-// counters, then walks it backwards.
-// ---------------------------------------------------------------------------
-constexpr int kSeg = 16;  // checkpoint interval; even, so segments start on a Philox pair
-
+// Adjoint kernel: one path per lane through heston_adjoint_path (reverse
+// sweep over kHestonSeg-step checkpoints kept in the workspace as
+// [option][segment][S|v][path]); the forward sweep also yields the price.
 template <typename Real>
 __global__ __launch_bounds__(kBlock) void heston_adjoint_kernel(HestonArgs a, Real* ckpt) {
   __shared__ double lds[8 * (kBlock / 64)];
   const int o = blockIdx.y;
   const int p = blockIdx.x * kBlock + threadIdx.x;
-  const HestonConsts<Real> c(a, o);
   double out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (p < a.n_paths) {
-    const int N = a.n_steps;
-    const int nseg = (N + kSeg - 1) / kSeg;
-    const double Td = a.T[o];
-    const double dtd = Td / N;
-    const Real hdt = (Real)(0.5 * dtd), sdt = (Real)sqrt(dtd);
-    const Real S0 = (Real)a.S[o];
-    const Real v00 = (Real)a.v0[o];
-    Real* ck = ckpt + (size_t)o * nseg * 2 * a.n_paths + p;  // [o][seg][S|v][path]
-    Real S = S0, v = v00;
-    CompSum<Real> avg;
-    for (int seg = 0; seg < nseg; ++seg) {
-      ck[(size_t)(2 * seg) * a.n_paths] = S;
-      ck[(size_t)(2 * seg + 1) * a.n_paths] = v;
+    const HestonConsts<Real> c(a.T[o], a.n_steps, a.r, a.kappa[o], a.xi[o], a.theta[o], a.rho);
+    const int nseg = (a.n_steps + kHestonSeg - 1) / kHestonSeg;
+    Real* ck = ckpt + (size_t)o * nseg * 2 * a.n_paths + p;
+    double g[3];
+    const double x = heston_adjoint_path<Real>(c, a.S[o], a.X[o], a.T[o], a.v0[o], a.r, a.n_steps, (uint32_t)p,
+                                               (uint32_t)o, a.w, a.k0, a.k1, ck, (size_t)a.n_paths, g);
+    out[0] = x;
+    out[1] = x * x;
 #pragma unroll
-      for (int j = 0; j < kSeg; j += 2) {
-        const int k = seg * kSeg + j;
-        if (k < N) {
-          Real n0, n1, n2, n3;
-          normals4<Real>(U4{(uint32_t)p, (uint32_t)(k >> 1), (uint32_t)o, a.w}, a.k0, a.k1, n0, n1, n2, n3);
-          heston_step<Real, false>(c, S, v, avg, n0, n1);
-          if (k + 1 < N) heston_step<Real, false>(c, S, v, avg, n2, n3);
-        }
-      }
-    }
-    const double x = avg.value() / (double)N - a.X[o];
-    if (x > 0.0) {
-      const double disc = exp(-a.r * Td);
-      const Real abar = (Real)(disc / N);
-      Real U = 0, vbar = 0, ebar_sum = 0;
-      Real Snext = S;  // S_N
-      for (int seg = nseg - 1; seg >= 0; --seg) {
-        Real Ss[kSeg], vs[kSeg], z1s[kSeg], zcs[kSeg];
-        Real Sr = ck[(size_t)(2 * seg) * a.n_paths];
-        Real vr = ck[(size_t)(2 * seg + 1) * a.n_paths];
-        CompSum<Real> dummy;
-#pragma unroll
-        for (int j = 0; j < kSeg; j += 2) {
-          const int k = seg * kSeg + j;
-          if (k < N) {
-            Real n0, n1, n2, n3;
-            normals4<Real>(U4{(uint32_t)p, (uint32_t)(k >> 1), (uint32_t)o, a.w}, a.k0, a.k1, n0, n1, n2, n3);
-            Ss[j] = Sr;
-            vs[j] = vr;
-            z1s[j] = n0;
-            zcs[j] = c.rho * n0 + c.srho * n1;
-            heston_step<Real, false>(c, Sr, vr, dummy, n0, n1);
-            Ss[j + 1] = Sr;
-            vs[j + 1] = vr;
-            z1s[j + 1] = n2;
-            zcs[j + 1] = c.rho * n2 + c.srho * n3;
-            if (k + 1 < N) heston_step<Real, false>(c, Sr, vr, dummy, n2, n3);
-          }
-        }
-#pragma unroll
-        for (int j = kSeg - 1; j >= 0; --j) {
-          const int k = seg * kSeg + j;
-          if (k < N) {
-            const Real Skp1 = (j + 1 < kSeg && k + 1 < N) ? Ss[j + 1] : Snext;
-            U += abar * Skp1;
-            ebar_sum += U;
-            const Real vk = vs[j];
-            if (vk > Real(0)) {
-              const Real h = Real(0.5) / fsqrt(vk);
-              const Real dedv = sdt * z1s[j] * h - hdt;
-              const Real dvdv = Real(1) - c.kdt + c.xsdt * zcs[j] * h;
-              vbar = vbar * dvdv + U * dedv;
-            }
-          }
-        }
-        Snext = Ss[0];
-      }
-      const double delta = (double)U / a.S[o];
-      const double vega = (double)vbar * 2.0 * sqrt(a.v0[o]);
-      const double rho = -Td * disc * x + dtd * (double)ebar_sum;
-      out[0] = x;
-      out[1] = x * x;
-      out[2] = delta;
-      out[3] = delta * delta;
-      out[4] = vega;
-      out[5] = vega * vega;
-      out[6] = rho;
-      out[7] = rho * rho;
+    for (int k = 0; k < 3; ++k) {
+      out[2 + 2 * k] = g[k];
+      out[3 + 2 * k] = g[k] * g[k];
     }
   }
   block_sum<8>(out, lds);
@@ -298,7 +184,7 @@ HestonPlan heston_plan(const fsmc_heston_params* p, const fsmc_mode* m) {
   h.chunk_bytes = align_up((size_t)p->n_options * (p->n_paths / kChunk + 1) * sizeof(double), 256);
   h.ws = h.partial_bytes + h.chunk_bytes;
   const size_t real = m->precision == FSMC_FP64 ? sizeof(double) : sizeof(float);
-  const size_t nseg = (size_t)(p->n_steps + kSeg - 1) / kSeg;
+  const size_t nseg = (size_t)(p->n_steps + kHestonSeg - 1) / kHestonSeg;
   h.gpartial_bytes = align_up((size_t)p->n_options * h.nbx * 8 * sizeof(double), 256);
   h.ckpt_bytes = align_up((size_t)p->n_options * nseg * 2 * (size_t)p->n_paths * real, 256);
   h.gws = h.gpartial_bytes + h.ckpt_bytes;

@@ -28,7 +28,6 @@ constexpr int NN = 80;    // forward rates           (params.h:27)
 constexpr int NMAT = 40;  // simulation steps        (params.h:28)
 constexpr int NOPT = 15;  // swaptions               (params.h:30)
 constexpr int kBlock = 128;
-constexpr double kLog2e = 1.4426950408889634074;
 
 // Portfolio of params.h:38-40.  The sample initialises its REAL swap rates
 // from float literals, so the FP64 build sees the float-rounded values.

@@ -1,0 +1,183 @@
+// Per-path models shared by the per-workload kernels and the mixed-batch
+// kernels, so that a request priced inside a batch follows exactly the
+// arithmetic (and Philox counters) of the standalone entry point.
+#pragma once
+
+#include "fsmc_device.hpp"
+
+namespace fsmc {
+
+constexpr double kPathLog2e = 1.4426950408889634074;
+
+// ---------------------------------------------------------------------------
+// Heston (MCAsianOptions.cpp:51-74 with sqrt(v+) as the volatility)
+// ---------------------------------------------------------------------------
+template <typename Real>
+struct HestonConsts {
+  Real c_rdt, c_hdt, c_sdt, kdt, xsdt, theta, rho, srho, xr, xs;
+  __device__ HestonConsts(double T, int n_steps, double r, double kappa, double xi, double th, double rh) {
+    const double dt = T / n_steps;
+    const double sdt = sqrt(dt);
+    c_rdt = (Real)(r * dt * kPathLog2e);
+    c_hdt = (Real)(0.5 * dt * kPathLog2e);
+    c_sdt = (Real)(sdt * kPathLog2e);
+    kdt = (Real)(kappa * dt);
+    xsdt = (Real)(xi * sdt);
+    theta = (Real)th;
+    rho = (Real)rh;
+    srho = (Real)sqrt(1.0 - rh * rh);
+    xr = (Real)(xi * sdt * rh);                   // xi sqrt(dt) rho
+    xs = (Real)(xi * sdt * sqrt(1.0 - rh * rh));  // xi sqrt(dt) sqrt(1 - rho^2)
+  }
+};
+
+// One full-truncation Euler step: the spot moves with the variance at the
+// start of the step, then the variance is advanced:
+//   S *= 2^(log2e ((r - sv^2/2) dt + sv sqrt(dt) z1)),  sv = sqrt(v+) (v+ with
+//   the sample's variance-as-volatility convention),
+//   v += kappa dt (theta - v+) + xi sqrt(v+ dt) (rho z1 + sqrt(1-rho^2) z2).
+// With sv = sqrt(v+), sv^2 is taken as v+ itself.
+template <typename Real, bool kVarAsVol>
+__device__ __forceinline__ void heston_step(const HestonConsts<Real>& c, Real& S, Real& v, Real z1, Real z2) {
+  const Real vp = v > Real(0) ? v : Real(0);
+  const Real sq = fsqrt(vp);
+  const Real sv = kVarAsVol ? vp : sq;
+  const Real sv2 = kVarAsVol ? vp * vp : vp;
+  const Real e = (c.c_rdt - c.c_hdt * sv2) + (c.c_sdt * sv) * z1;
+  S = S * fexp2(e);
+  v = v + c.kdt * (c.theta - vp) + sq * (c.xr * z1 + c.xs * z2);
+}
+
+// Two consecutive steps sharing one Philox call (normals n0..n3); the running
+// sum of the monitored spots takes the pair's sum in one compensated add.
+template <typename Real, bool kVarAsVol>
+__device__ __forceinline__ void heston_pair(const HestonConsts<Real>& c, Real& S, Real& v, CompSum<Real>& avg,
+                                            Real n0, Real n1, Real n2, Real n3, bool second) {
+  heston_step<Real, kVarAsVol>(c, S, v, n0, n1);
+  if (second) {
+    const Real s1 = S;
+    heston_step<Real, kVarAsVol>(c, S, v, n2, n3);
+    avg.add(s1 + S);
+  } else {
+    avg.add(S);
+  }
+}
+
+// Adjoint (reverse sweep) of one Heston-Asian path.  Pathwise derivatives of
+//   P = e^{-rT} max(A - K, 0),  A = (1/N) sum_{k=1..N} S_k,
+//   S_{k+1} = S_k exp(e_k),  e_k = r dt - vp_k dt / 2 + sqrt(vp_k dt) z1_k,
+//   v_{k+1} = v_k + kappa dt (theta - vp_k) + xi sqrt(vp_k dt) zc_k,
+// with vp = max(v, 0).  Going backwards with U_{k+1} = dP/de_k =
+// (e^{-rT}/N) sum_{j>k} S_j:
+//   vbar_k = vbar_{k+1} dv_{k+1}/dv_k + U_{k+1} de_k/dv_k,
+//   rbar   = -T P + dt sum_k U_{k+1},   dP/dS0 = U_1 / S0,
+// and vega = dP/dsqrt(v0) = 2 sqrt(v0) vbar_0.  The forward sweep stores
+// (S, v) every kHestonSeg steps at ck[(2 seg) * stride] / ck[(2 seg + 1) *
+// stride]; the reverse sweep recomputes one segment at a time into registers,
+// regenerating its normals from the Philox counters (path, step/2, c2).
+// Returns the undiscounted payoff; g = {dP/dS0, dP/dsqrt(v0), dP/dr}.
+constexpr int kHestonSeg = 16;  // even: segments start on a Philox pair
+
+template <typename Real>
+__device__ __forceinline__ double heston_adjoint_path(const HestonConsts<Real>& c, double S0d, double Kd, double Td,
+                                                      double v0d, double r, int N, uint32_t p, uint32_t c2,
+                                                      uint32_t w, uint32_t k0, uint32_t k1, Real* ck, size_t stride,
+                                                      double (&g)[3]) {
+  const int nseg = (N + kHestonSeg - 1) / kHestonSeg;
+  const double dtd = Td / N;
+  const Real hdt = (Real)(0.5 * dtd), sdt = (Real)sqrt(dtd);
+  Real S = (Real)S0d, v = (Real)v0d;
+  CompSum<Real> avg;
+  for (int seg = 0; seg < nseg; ++seg) {
+    ck[(size_t)(2 * seg) * stride] = S;
+    ck[(size_t)(2 * seg + 1) * stride] = v;
+#pragma unroll
+    for (int j = 0; j < kHestonSeg; j += 2) {
+      const int k = seg * kHestonSeg + j;
+      if (k < N) {
+        Real n0, n1, n2, n3;
+        normals4<Real>(U4{p, (uint32_t)(k >> 1), c2, w}, k0, k1, n0, n1, n2, n3);
+        heston_pair<Real, false>(c, S, v, avg, n0, n1, n2, n3, k + 1 < N);
+      }
+    }
+  }
+  g[0] = g[1] = g[2] = 0.0;
+  const double x = avg.value() / (double)N - Kd;
+  if (!(x > 0.0)) return 0.0;
+  const double disc = exp(-r * Td);
+  const Real abar = (Real)(disc / N);
+  Real U = 0, vbar = 0, ebar_sum = 0;
+  Real Snext = S;  // S_N
+  for (int seg = nseg - 1; seg >= 0; --seg) {
+    Real Ss[kHestonSeg], vs[kHestonSeg], z1s[kHestonSeg], zcs[kHestonSeg];
+    Real Sr = ck[(size_t)(2 * seg) * stride];
+    Real vr = ck[(size_t)(2 * seg + 1) * stride];
+#pragma unroll
+    for (int j = 0; j < kHestonSeg; j += 2) {
+      const int k = seg * kHestonSeg + j;
+      if (k < N) {
+        Real n0, n1, n2, n3;
+        normals4<Real>(U4{p, (uint32_t)(k >> 1), c2, w}, k0, k1, n0, n1, n2, n3);
+        Ss[j] = Sr;
+        vs[j] = vr;
+        z1s[j] = n0;
+        zcs[j] = c.rho * n0 + c.srho * n1;
+        heston_step<Real, false>(c, Sr, vr, n0, n1);
+        Ss[j + 1] = Sr;
+        vs[j + 1] = vr;
+        z1s[j + 1] = n2;
+        zcs[j + 1] = c.rho * n2 + c.srho * n3;
+        if (k + 1 < N) heston_step<Real, false>(c, Sr, vr, n2, n3);
+      }
+    }
+#pragma unroll
+    for (int j = kHestonSeg - 1; j >= 0; --j) {
+      const int k = seg * kHestonSeg + j;
+      if (k < N) {
+        const Real Skp1 = (j + 1 < kHestonSeg && k + 1 < N) ? Ss[j + 1] : Snext;
+        U += abar * Skp1;
+        ebar_sum += U;
+        const Real vk = vs[j];
+        if (vk > Real(0)) {
+          const Real h = Real(0.5) / fsqrt(vk);
+          const Real dedv = sdt * z1s[j] * h - hdt;
+          const Real dvdv = Real(1) - c.kdt + c.xsdt * zcs[j] * h;
+          vbar = vbar * dvdv + U * dedv;
+        }
+      }
+    }
+    Snext = Ss[0];
+  }
+  g[0] = (double)U / S0d;
+  g[1] = (double)vbar * 2.0 * sqrt(v0d);
+  g[2] = -Td * disc * x + dtd * (double)ebar_sum;
+  return x;
+}
+
+// ---------------------------------------------------------------------------
+// GBM European call (MonteCarloInsideBlockingDP.cpp:239-257, log2 domain)
+// ---------------------------------------------------------------------------
+template <typename Real>
+struct GbmConsts {
+  Real mu, vol, Y, Z, sqrtT, sigT;
+  __device__ GbmConsts(double S, double X, double T, double r, double sigma) {
+    const double sqrtTd = sqrt(T);
+    mu = (Real)(kPathLog2e * (r - 0.5 * sigma * sigma) * T);
+    vol = (Real)(kPathLog2e * sigma * sqrtTd);
+    Y = (Real)S;
+    Z = (Real)X;
+    sqrtT = (Real)sqrtTd;
+    sigT = (Real)(sigma * T);
+  }
+};
+
+// Terminal value of one path and, when in the money, the pathwise
+// derivative kernels: returns c = S_T - K; st = S_T; dlog = dlnS_T/dsigma.
+template <typename Real>
+__device__ __forceinline__ Real gbm_path(const GbmConsts<Real>& g, Real z, Real& st, Real& dlog) {
+  st = g.Y * fexp2(g.mu + g.vol * z);
+  dlog = g.sqrtT * z - g.sigT;
+  return st - g.Z;
+}
+
+}  // namespace fsmc

@@ -57,6 +57,25 @@ class LsmcRefParams(ctypes.Structure):
                 ("strike", ctypes.c_double), ("vol", ctypes.c_double), ("maturity", ctypes.c_double)]
 
 
+REQ_GBM_EUROPEAN, REQ_HESTON_ASIAN = 0, 1
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("n_paths", ctypes.c_int), ("n_steps", ctypes.c_int),
+                ("S0", ctypes.c_double), ("K", ctypes.c_double), ("T", ctypes.c_double), ("r", ctypes.c_double),
+                ("sigma", ctypes.c_double), ("v0", ctypes.c_double), ("kappa", ctypes.c_double),
+                ("theta", ctypes.c_double), ("xi", ctypes.c_double), ("rho", ctypes.c_double)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("price", "delta", "vega", "rho", "price_se", "delta_se", "vega_se",
+                                               "rho_se", "latency_us")]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("chunk_paths", ctypes.c_int), ("requests_per_launch", ctypes.c_int)]
+
+
 class HestonParams(ctypes.Structure):
     _fields_ = [("n_options", ctypes.c_int), ("n_paths", ctypes.c_int), ("n_steps", ctypes.c_int),
                 ("riskfree", ctypes.c_double), ("rho", ctypes.c_double)]
@@ -111,6 +130,13 @@ def lib():
         L.fsmc_lsmc_plan_destroy.argtypes = [_VP]
         L.fsmc_american_lsmc.argtypes = [_VP, ctypes.c_int, PM] + [_D] * 5 + [_I, _I, _D, _D]
         L.fsmc_american_lsmc_ref.argtypes = [_VP, ctypes.POINTER(LsmcRefParams), PM, _D, _D, _D]
+        L.fsmc_batch_create.argtypes = [_VP, ctypes.POINTER(Request), ctypes.c_int, PM, ctypes.POINTER(BatchTuning),
+                                        ctypes.POINTER(_VP)]
+        L.fsmc_batch_run.argtypes = [_VP, _VP]
+        L.fsmc_batch_results.argtypes = [_VP, _VP, ctypes.POINTER(Result)]
+        L.fsmc_batch_autotune.argtypes = [_VP, ctypes.POINTER(BatchTuning)]
+        L.fsmc_batch_tuning_get.argtypes = [_VP, ctypes.POINTER(BatchTuning)]
+        L.fsmc_batch_destroy.argtypes = [_VP]
         _lib = L
     return _lib
 
@@ -277,6 +303,76 @@ class LsmcPlan:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """A mixed GBM/Heston request batch served in micro-batches."""
+
+    FIELDS = ("price", "delta", "vega", "rho", "price_se", "delta_se", "vega_se", "rho_se", "latency_us")
+
+    def __init__(self, engine, requests, m, tuning=None):
+        self.n = len(requests)
+        self._reqs = (Request * self.n)(*requests)
+        self._b = _VP()
+        t = BatchTuning(*tuning) if tuning is not None else None
+        _check(lib().fsmc_batch_create(engine._ctx, self._reqs, self.n, ctypes.byref(m),
+                                       ctypes.byref(t) if t is not None else None, ctypes.byref(self._b)),
+               "fsmc_batch_create")
+
+    def run(self, stream=None):
+        _check(lib().fsmc_batch_run(self._b, stream), "fsmc_batch_run")
+
+    def results(self, stream=None):
+        out = (Result * self.n)()
+        _check(lib().fsmc_batch_results(self._b, stream, out), "fsmc_batch_results")
+        return {f: np.array([getattr(r, f) for r in out]) for f in self.FIELDS}
+
+    def autotune(self):
+        t = BatchTuning()
+        _check(lib().fsmc_batch_autotune(self._b, ctypes.byref(t)), "fsmc_batch_autotune")
+        return t.chunk_paths, t.requests_per_launch
+
+    def tuning(self):
+        t = BatchTuning()
+        _check(lib().fsmc_batch_tuning_get(self._b, ctypes.byref(t)), "fsmc_batch_tuning_get")
+        return t.chunk_paths, t.requests_per_launch
+
+    def close(self):
+        if self._b:
+            lib().fsmc_batch_destroy(self._b)
+            self._b = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gbm_request(n_paths, S0, K, T, r, sigma):
+    return Request(REQ_GBM_EUROPEAN, n_paths, 1, S0, K, T, r, sigma, 0, 0, 0, 0, 0)
+
+
+def heston_request(n_paths, n_steps, S0, K, T, r, v0, kappa, theta, xi, rho):
+    return Request(REQ_HESTON_ASIAN, n_paths, n_steps, S0, K, T, r, 0, v0, kappa, theta, xi, rho)
+
+
+def mixed_requests(n, seed=4096):
+    """The mixed serving batch: 70 % GBM European (paths ~U{8192..262144},
+    S0~U[5,30], K~U[1,100], T~U[0.25,10], r = .02, sigma = .3) and 30 %
+    Heston Asian (steps ~U{52..365}, paths ~U{16384..131072}, S0~U[80,120],
+    K = 100, T = 1, r = .03, kappa = 2, theta = .04, xi = .3, rho = -.7,
+    v0 = .04), drawn from one seed."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(n):
+        if rng.random() < 0.7:
+            reqs.append(gbm_request(int(rng.integers(8192, 262145)), rng.uniform(5, 30), rng.uniform(1, 100),
+                                    rng.uniform(.25, 10), .02, .3))
+        else:
+            reqs.append(heston_request(int(rng.integers(16384, 131073)), int(rng.integers(52, 366)),
+                                       rng.uniform(80, 120), 100.0, 1.0, .03, .04, 2.0, .04, .3, -.7))
+    return reqs
 
 
 # ---- asynchronous entry points on torch device tensors ------------------------

@@ -197,6 +197,55 @@ typedef struct fsmc_lsmc_ref_params {
 int fsmc_american_lsmc_ref(fsmc_ctx* ctx, const fsmc_lsmc_ref_params* p, const fsmc_mode* m, const double* z,
                            double* american, double* european);
 
+/* ---- Mixed request batches (serving) ----------------------------------------
+ * No counterpart in the samples, which price one workload per process: many
+ * GBM European and Heston-Asian requests of uneven sizes, each returning the
+ * price and delta, vega, rho (GBM vega = dV/dsigma; Heston vega =
+ * dV/dsqrt(v0)) with standard errors.  Requests are served in micro-batches
+ * of requests_per_launch (0 = all) in arrival order, each as one GBM grid, one
+ * Heston grid and one finalize grid over (request, chunk) work items.  A
+ * request's normals are those of the standalone entry point with the request
+ * index as the option index, so a batch reproduces fsmc_gbm_european /
+ * fsmc_heston_asian (with Greeks) up to summation order.  latency_us is the
+ * time from the start of the run to the end of the request's micro-batch
+ * (HIP events).  Runs of one batch share its workspace: do not overlap them. */
+#define FSMC_REQ_GBM_EUROPEAN 0
+#define FSMC_REQ_HESTON_ASIAN 1
+
+typedef struct fsmc_request {
+  int kind;    /* FSMC_REQ_*                 */
+  int n_paths;
+  int n_steps; /* Heston monitoring steps    */
+  double S0, K, T, r;
+  double sigma;                      /* GBM     */
+  double v0, kappa, theta, xi, rho;  /* Heston  */
+} fsmc_request;
+
+typedef struct fsmc_result {
+  double price, delta, vega, rho;
+  double price_se, delta_se, vega_se, rho_se;
+  double latency_us;
+} fsmc_result;
+
+typedef struct fsmc_batch_tuning {
+  int chunk_paths;         /* paths per work item, a multiple of 1024 */
+  int requests_per_launch; /* micro-batch size, 0 = the whole batch   */
+} fsmc_batch_tuning;
+
+typedef struct fsmc_batch fsmc_batch;
+
+/* tuning NULL: the autotuner's cached choice for a batch of this shape, or
+ * {4096, 0}. */
+int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_mode* m,
+                      const fsmc_batch_tuning* tuning, fsmc_batch** out);
+int fsmc_batch_run(fsmc_batch* b, void* stream);
+int fsmc_batch_results(fsmc_batch* b, void* stream, fsmc_result* out);
+/* Times the batch over chunk sizes x micro-batch sizes, keeps the fastest
+ * and caches it for batches of the same shape in this process. */
+int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen);
+int fsmc_batch_tuning_get(const fsmc_batch* b, fsmc_batch_tuning* out);
+int fsmc_batch_destroy(fsmc_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

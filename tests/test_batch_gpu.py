@@ -1,0 +1,116 @@
+"""Mixed request batches (micro-batches of grouped GBM / Heston grids).
+
+A request priced inside a batch uses the normals and per-path arithmetic of
+the standalone entry point with the request index as option index, so FP64
+batch results equal the standalone GPU results (and therefore the oracle) to
+summation order; reruns, with eager work in between, reproduce the first run
+bitwise."""
+import numpy as np
+import pytest
+
+import mcengine as mc
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(r, k):
+    return np.array([getattr(x, k) for x in r])
+
+
+def test_batch_matches_standalone_heston(engine):
+    n, paths, steps = 6, 3000, 37
+    rng = np.random.default_rng(1)
+    S = rng.uniform(80, 120, n)
+    reqs = [mc.heston_request(paths, steps, S[i], 100.0, 1.0, .03, .04, 2.0, .04, .3, -.7) for i in range(n)]
+    m = mc.mode(mc.FP64, seed=11)
+    b = mc.Batch(engine, reqs, m, tuning=(1024, 0))
+    b.run()
+    out = b.results()
+    b.close()
+    one = np.ones(n)
+    res, conf, g = engine.heston_asian(S, 100 * one, one, .04 * one, 2 * one, .04 * one, .3 * one, .03, -.7, paths,
+                                       steps, m, greeks=True)
+    assert np.allclose(out["price"], res, rtol=1e-12, atol=0)
+    for k, name in enumerate(("delta", "vega", "rho")):
+        assert np.allclose(out[name], g[:, k], rtol=1e-10, atol=1e-14)
+    # and the oracle's tangent-mode Greeks
+    ores, oconf, og = orc.heston_philox(S, 100 * one, one, .04 * one, 2 * one, .04 * one, .3 * one, .03, -.7, paths,
+                                        steps, seed=11, greeks=True)
+    assert np.allclose(out["price"], ores, rtol=1e-12, atol=0)
+    assert np.allclose(out["delta"], og[:, 0], rtol=1e-9, atol=0)
+
+
+def test_batch_matches_standalone_gbm(engine):
+    n, paths = 5, 5000
+    rng = np.random.default_rng(2)
+    S, X, T = rng.uniform(20, 30, n), rng.uniform(5, 20, n), rng.uniform(.5, 3, n)
+    reqs = [mc.gbm_request(paths, S[i], X[i], T[i], .02, .3) for i in range(n)]
+    m = mc.mode(mc.FP64, seed=12)
+    b = mc.Batch(engine, reqs, m, tuning=(1024, 2))
+    b.run()
+    out = b.results()
+    b.close()
+    res, conf, g = engine.gbm_european(S, X, T, .02, .3, paths, m, greeks=True)
+    assert np.allclose(out["price"], res, rtol=1e-12, atol=0)
+    assert np.allclose(out["price_se"] * 1.96, conf, rtol=1e-9, atol=0)
+    for k, name in enumerate(("delta", "vega", "rho")):
+        assert np.allclose(out[name], g[:, k], rtol=1e-12, atol=1e-14)
+
+
+def test_mixed_batch_reruns_and_tuning(engine):
+    import torch
+    reqs = mc.mixed_requests(96, seed=5)
+    m = mc.mode(mc.FP32, seed=3)
+    b = mc.Batch(engine, reqs, m, tuning=(4096, 32))
+    b.run()
+    first = b.results()
+    for _ in range(3):
+        x = torch.randn(1 << 20, device="cuda")  # eager work between runs
+        torch.cuda.synchronize()
+        del x
+        b.run()
+        again = b.results()
+        for k in ("price", "delta", "vega", "rho"):
+            assert np.array_equal(first[k], again[k])
+    assert np.all(first["latency_us"] > 0)
+    # micro-batches complete in order
+    assert np.all(np.diff(first["latency_us"][::32]) >= 0)
+    # a different tuning changes only the summation order
+    b2 = mc.Batch(engine, reqs, m, tuning=(2048, 0))
+    b2.run()
+    other = b2.results()
+    b2.close()
+    assert np.allclose(other["price"], first["price"], rtol=1e-9, atol=1e-12)
+    chosen = b.autotune()
+    assert chosen[0] % 1024 == 0 and chosen[1] >= 0
+    assert b.tuning() == chosen
+    b.run()
+    tuned = b.results()
+    b.close()
+    assert np.allclose(tuned["price"], first["price"], rtol=1e-9, atol=1e-12)
+
+
+def test_mixed_batch_statistics(engine):
+    # GBM requests within 3 SE of Black-Scholes (Greeks too)
+    reqs = mc.mixed_requests(200, seed=6)
+    b = mc.Batch(engine, reqs, mc.mode(mc.FP32, seed=4))
+    b.run()
+    out = b.results()
+    b.close()
+    zs = []
+    for i, q in enumerate(reqs):
+        if q.kind != mc.REQ_GBM_EUROPEAN or out["price_se"][i] <= 0:
+            continue
+        if (out["price"][i] / out["price_se"][i]) ** 2 < 50:
+            continue
+        zs.append((out["price"][i] - orc.bs_call(q.S0, q.K, q.T, q.r, q.sigma)) / out["price_se"][i])
+    zs = np.array(zs)
+    assert len(zs) > 60 and np.max(np.abs(zs)) < 5 and abs(np.mean(zs * zs) - 1) < 0.5
+
+
+def test_batch_errors(engine):
+    with pytest.raises(mc.FsmcError):
+        mc.Batch(engine, [mc.gbm_request(1, 10, 10, 1, .02, .3)], mc.mode())
+    with pytest.raises(mc.FsmcError):
+        mc.Batch(engine, [mc.gbm_request(100, 10, 10, 1, .02, .3)], mc.mode(), tuning=(1000, 0))
