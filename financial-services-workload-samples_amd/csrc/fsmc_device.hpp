@@ -177,6 +177,9 @@ __device__ __forceinline__ double fexp2(double x) { return exp2(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// Explicit fused multiply-add: a * b + c with one rounding.
+__device__ __forceinline__ float ffma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double ffma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
 
 }  // namespace fsmc

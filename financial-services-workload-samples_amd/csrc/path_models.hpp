@@ -36,16 +36,20 @@ struct HestonConsts {
 //   S *= 2^(log2e ((r - sv^2/2) dt + sv sqrt(dt) z1)),  sv = sqrt(v+) (v+ with
 //   the sample's variance-as-volatility convention),
 //   v += kappa dt (theta - v+) + xi sqrt(v+ dt) (rho z1 + sqrt(1-rho^2) z2).
-// With sv = sqrt(v+), sv^2 is taken as v+ itself.
+// With sv = sqrt(v+), sv^2 is taken as v+ itself.  Contraction is off and the
+// fused multiply-adds are explicit, so every kernel that inlines the step
+// (forward, adjoint forward sweep, reverse-sweep recompute, batch) rounds
+// exactly alike whatever the optimiser does around it.
 template <typename Real, bool kVarAsVol>
 __device__ __forceinline__ void heston_step(const HestonConsts<Real>& c, Real& S, Real& v, Real z1, Real z2) {
+#pragma clang fp contract(off)
   const Real vp = v > Real(0) ? v : Real(0);
   const Real sq = fsqrt(vp);
   const Real sv = kVarAsVol ? vp : sq;
   const Real sv2 = kVarAsVol ? vp * vp : vp;
-  const Real e = (c.c_rdt - c.c_hdt * sv2) + (c.c_sdt * sv) * z1;
+  const Real e = ffma(c.c_sdt * sv, z1, ffma(-c.c_hdt, sv2, c.c_rdt));
   S = S * fexp2(e);
-  v = v + c.kdt * (c.theta - vp) + sq * (c.xr * z1 + c.xs * z2);
+  v = ffma(sq, ffma(c.xr, z1, c.xs * z2), ffma(c.kdt, c.theta - vp, v));
 }
 
 // Two consecutive steps sharing one Philox call (normals n0..n3); the running
@@ -53,6 +57,7 @@ __device__ __forceinline__ void heston_step(const HestonConsts<Real>& c, Real& S
 template <typename Real, bool kVarAsVol>
 __device__ __forceinline__ void heston_pair(const HestonConsts<Real>& c, Real& S, Real& v, CompSum<Real>& avg,
                                             Real n0, Real n1, Real n2, Real n3, bool second) {
+#pragma clang fp contract(off)
   heston_step<Real, kVarAsVol>(c, S, v, n0, n1);
   if (second) {
     const Real s1 = S;
