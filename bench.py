@@ -8,7 +8,17 @@ K=100, T=1, r=0.03, kappa=2, theta=0.04, xi=0.3, rho=-0.7, v0=0.04).
 
 Prints one JSON line: path-steps/s, the roofline of the path kernel against
 the FP32 vector peak, the same batch in plain FP64 and with adjoint Greeks,
-and the CPU oracle timed on a bounded sample on this host.
+and the CPU oracle timed on a bounded sample on this host.  Under "configs"
+the same line carries the other four workloads, each timed with HIP events
+on the launching stream, in FP32 and in plain FP64:
+  gbm_european   1024 calls x 262144 paths (S0~U[5,30], K~U[1,100],
+                 T~U[.25,10], r=.02, sigma=.3), error against Black-Scholes;
+  libor_greeks   262144 paths of the 80-rate / 40-step LMM, 15 swaptions,
+                 value + 80 adjoint deltas;
+  american_lsmc  256 puts, S0~U[80,120], K=100, sigma~U[.1,.5], r=.03,
+                 T~U[.25,2], steps~U{50..250}, paths~U{16384..262144};
+  serving_batch  4096 mixed GBM / Heston requests with delta, vega, rho;
+                 autotuned, p50 / p99 request completion latency.
 """
 import argparse
 import json
@@ -30,6 +40,12 @@ R, RHO = 0.03, -0.7
 # is not counted.
 FLOP_PER_PATH_STEP = 34
 FP32_VECTOR_PEAK_TFLOPS = 157.3
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+# LIBOR per path (same convention, division = reciprocal + multiply): each of
+# the 2380 forward (step, rate) updates 10 FLOP (2 T: rcp, exp2), each of the
+# 2380 adjoint updates 14 FLOP (2 T: two rcp), portfolio and its reverse
+# sweep 600.
+LIBOR_FLOP_PER_PATH = 2380 * (10 + 14) + 600
 
 
 def _params(np):
@@ -76,6 +92,180 @@ def _cpu_baseline(np, P, target_s):
                       f"of the same workload, {el:.1f} s"}
 
 
+def _timed(torch, np, fn, steps, warmup):
+    """Mean ms per call of fn (which launches on torch's current stream),
+    from HIP events recorded on that stream."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for a, b in evs:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    return float(np.mean([a.elapsed_time(b) for a, b in evs]))
+
+
+def _bs_call(S, K, T, r, sigma, np):
+    from scipy.special import ndtr
+    sq = sigma * np.sqrt(T)
+    d1 = (np.log(S / K) + (r + 0.5 * sigma * sigma) * T) / sq
+    return S * ndtr(d1) - K * np.exp(-r * T) * ndtr(d1 - sq)
+
+
+def _gbm_config(np, torch, mc, eng, precisions, steps):
+    """GBM European, 1024 calls x 262144 paths."""
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(1024)
+    n, paths, r, sig = 1024, 262144, 0.02, 0.30
+    S, X, T = rng.uniform(5, 30, n), rng.uniform(1, 100, n), rng.uniform(0.25, 10, n)
+    tS, tX, tT = (torch.tensor(a, dtype=torch.float64, device=dev) for a in (S, X, T))
+    res = {}
+    for name, m in precisions:
+        ws = torch.empty(mc.gbm_european_workspace(n, paths, r, sig, m), dtype=torch.uint8, device=dev)
+        o = torch.zeros(n, dtype=torch.float64, device=dev)
+        c = torch.zeros_like(o)
+        ms = _timed(torch, np, lambda: mc.gbm_european_async(tS, tX, tT, r, sig, paths, m, o, c, ws), steps, 2)
+        res[name] = (ms, o.cpu().numpy(), c.cpu().numpy() / 1.96)
+    out = {"options": n, "paths": paths}
+    for name in res:
+        out["ms_" + name] = res[name][0]
+        out["path_steps_per_s_" + name] = n * paths / (res[name][0] * 1e-3)
+    if "fp64" in res and "fp32" in res:
+        from scipy.special import ndtr
+        # standard errors are trustworthy where enough paths finish in the money
+        sq = sig * np.sqrt(T)
+        itm = paths * ndtr((np.log(S / X) + (r - 0.5 * sig * sig) * T) / sq)
+        ok = itm >= 1000
+        se = res["fp64"][2]
+        z = (res["fp64"][1] - _bs_call(S, X, T, r, sig, np))[ok] / se[ok]
+        out.update({"fp32_speedup_over_fp64": res["fp64"][0] / res["fp32"][0],
+                    "fp32_vs_fp64_max_err_in_se": float(np.max(np.abs(res["fp32"][1] - res["fp64"][1])[ok] / se[ok])),
+                    "fp64_vs_black_scholes": {"options_with_1000_itm_paths": int(ok.sum()),
+                                              "max_abs_z": float(np.max(np.abs(z))),
+                                              "mean_z2": float(np.mean(z * z))}})
+    return out
+
+
+def _libor_config(np, torch, mc, eng, precisions, steps):
+    """LIBOR swaption portfolio, 262144 paths, value + 80 adjoint deltas."""
+    dev = torch.device("cuda:0")
+    lp = 262144
+    res = {}
+    for name, m in precisions:
+        ws = torch.empty(mc.libor_workspace(lp, m), dtype=torch.uint8, device=dev)
+        v = torch.zeros(2, dtype=torch.float64, device=dev)
+        d = torch.zeros(160, dtype=torch.float64, device=dev)
+        ms = _timed(torch, np, lambda: mc.libor_async(lp, m, v, d, ws), steps, 2)
+        res[name] = (ms, v.cpu().numpy(), d.cpu().numpy())
+        del ws
+    out = {"paths": lp, "flop_per_path": LIBOR_FLOP_PER_PATH}
+    for name in res:
+        ms = res[name][0]
+        out["ms_" + name] = ms
+        out["greeks_per_s_" + name] = 80 * lp / (ms * 1e-3)
+        out["path_steps_per_s_" + name] = 40 * lp / (ms * 1e-3)
+    if "fp32" in res:
+        achieved = LIBOR_FLOP_PER_PATH * lp / (res["fp32"][0] * 1e-3) / 1e12
+        out.update({"achieved_tflops_fp32": achieved, "frac_fp32_peak": achieved / FP32_VECTOR_PEAK_TFLOPS})
+    if "fp64" in res and "fp32" in res:
+        se_d = res["fp64"][2][80:]
+        err_d = np.abs(res["fp32"][2][:80] - res["fp64"][2][:80])
+        out.update({"fp32_speedup_over_fp64": res["fp64"][0] / res["fp32"][0],
+                    "fp32_vs_fp64_value_err_in_se": float(abs(res["fp32"][1][0] - res["fp64"][1][0])
+                                                          / res["fp64"][1][1]),
+                    "fp32_vs_fp64_max_delta_err_in_se": float(np.max(err_d[se_d > 0] / se_d[se_d > 0]))})
+    return out
+
+
+def _lsmc_config(np, torch, mc, eng, precisions, steps):
+    """Longstaff-Schwartz, 256 puts of uneven sizes in one plan."""
+    rng = np.random.default_rng(256)
+    n = 256
+    S0, sg, T = rng.uniform(80, 120, n), rng.uniform(0.1, 0.5, n), rng.uniform(0.25, 2, n)
+    K, rr = np.full(n, 100.0), np.full(n, 0.03)
+    st, pa = rng.integers(50, 251, n), rng.integers(16384, 262145, n)
+    ps = int(np.sum(st.astype(np.int64) * pa))
+    res = {}
+    for name, m in precisions:
+        plan = mc.LsmcPlan(eng, S0, K, sg, rr, T, st, pa, m)
+        ms = _timed(torch, np, lambda: plan.run(mc._stream()), steps, 1)
+        pr, se, _ = plan.results(mc._stream())
+        plan.close()
+        res[name] = (ms, pr, se)
+    out = {"options": n, "path_steps": ps}
+    for name in res:
+        out["ms_" + name] = res[name][0]
+        out["path_steps_per_s_" + name] = ps / (res[name][0] * 1e-3)
+    if "fp64" in res and "fp32" in res:
+        out.update({"fp32_speedup_over_fp64": res["fp64"][0] / res["fp32"][0],
+                    "fp32_vs_fp64_max_err_in_se": float(np.max(np.abs(res["fp32"][1] - res["fp64"][1])
+                                                               / res["fp64"][2]))})
+    return out
+
+
+def _batch_config(np, torch, mc, eng, precisions, steps):
+    """Mixed serving batch, 4096 requests with adjoint Greeks: tuned for
+    throughput (whole batch) and served in micro-batches of 256 requests for
+    latency."""
+    reqs = mc.mixed_requests(4096)
+    ps = sum(q.n_paths * q.n_steps for q in reqs)
+    npaths = sum(q.n_paths for q in reqs)
+    out = {"requests": len(reqs), "path_steps": ps}
+    for name, m in precisions:
+        b = mc.Batch(eng, reqs, m)
+        t0 = time.perf_counter()
+        tuning = b.autotune()
+        tune_s = time.perf_counter() - t0
+        ms = _timed(torch, np, lambda: b.run(mc._stream()), steps, 1)
+        b.close()
+        lb = mc.Batch(eng, reqs, m, tuning=(tuning[0], 256))
+        lms = _timed(torch, np, lambda: lb.run(mc._stream()), steps, 1)
+        lat = lb.results(mc._stream())["latency_us"] / 1e3
+        lb.close()
+        out.update({"ms_" + name: ms, "requests_per_s_" + name: len(reqs) / (ms * 1e-3),
+                    "path_steps_per_s_" + name: ps / (ms * 1e-3),
+                    "path_greeks_per_s_" + name: 3 * npaths / (ms * 1e-3),
+                    "tuning_" + name: {"chunk_paths": tuning[0], "requests_per_launch": tuning[1]},
+                    "autotune_s_" + name: tune_s,
+                    "microbatch256_ms_" + name: lms,
+                    "latency_p50_ms_" + name: float(np.percentile(lat, 50)),
+                    "latency_p99_ms_" + name: float(np.percentile(lat, 99))})
+    return out
+
+
+CONFIGS = {"gbm_european": _gbm_config, "libor_greeks": _libor_config, "american_lsmc": _lsmc_config,
+           "serving_batch": _batch_config}
+
+
+def _other_configs(np, torch, mc, only=None, steps=10, fp64=True):
+    """Runs on a side stream of default priority: the plan and batch objects
+    read a NULL stream as their context's own stream, so the events must sit
+    on a real stream for the timing to bracket their launches."""
+    f32, f64 = mc.mode(mc.FP32, seed=7), mc.mode(mc.FP64, seed=7)
+    precisions = [("fp32", f32)] + ([("fp64", f64)] if fp64 else [])
+    eng = mc.Engine()
+    out = {}
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        for name, fn in CONFIGS.items():
+            if only is None or name == only:
+                out[name] = fn(np, torch, mc, eng, precisions, steps)
+    torch.cuda.synchronize()
+    eng.close()
+    return out
+
+
+def _copy_probe(torch, np, steps):
+    """A plain device-to-device copy of 1 GiB (the scale for PMC byte counters)."""
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    ms = _timed(torch, np, lambda: b.copy_(a), steps, 1)
+    return {"bytes_read": a.numel() * 4, "bytes_written": b.numel() * 4, "ms": ms,
+            "gb_per_s": 2 * a.numel() * 4 / (ms * 1e-3) / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +274,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--workload", choices=["heston", "heston_adjoint", "copy"] + list(CONFIGS),
+                    help="profiling aid: run only this workload in FP32 for --steps launches and print its timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every path shards by "
@@ -95,6 +287,13 @@ def main():
     import torch
     import mcengine as mc
 
+    if args.workload in CONFIGS:
+        print(json.dumps(_other_configs(np, torch, mc, only=args.workload, steps=args.steps, fp64=False)),
+              flush=True)
+        return
+    if args.workload == "copy":
+        print(json.dumps(_copy_probe(torch, np, args.steps)), flush=True)
+        return
     dev = torch.device("cuda:0")
     P = _params(np)
     prm = [torch.tensor(P[k], dtype=torch.float64, device=dev) for k in ("S", "X", "T", "v0", "kappa", "theta",
@@ -122,7 +321,14 @@ def main():
         return wall / steps * 1e3, kern_ms, out.cpu().numpy().copy(), conf.cpu().numpy().copy()
 
     m32 = mc.mode(mc.FP32, seed=1)
+    if args.workload == "heston_adjoint":
+        msg, kg, _, _ = run(m32, args.steps, args.warmup, with_greeks=True)
+        print(json.dumps({"workload": "heston_adjoint", "ms": msg, "kernel_ms": kg}), flush=True)
+        return
     ms, kern_ms, p32, c32 = run(m32, args.steps, args.warmup)
+    if args.workload == "heston":
+        print(json.dumps({"workload": "heston", "ms": ms, "kernel_ms": kern_ms}), flush=True)
+        return
     path_steps = N_OPT * N_PATHS * N_STEPS
     value = path_steps / (ms * 1e-3)
     achieved = FLOP_PER_PATH_STEP * path_steps / (kern_ms * 1e-3) / 1e12
@@ -150,6 +356,8 @@ def main():
             "adjoint_greeks_per_s": N_OPT * N_PATHS * 3 / (msg * 1e-3),
             "adjoint_path_steps_per_s": path_steps / (msg * 1e-3),
         }
+    if not args.no_secondary:
+        rec["configs"] = _other_configs(np, torch, mc)
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = _cpu_baseline(np, P, args.cpu_seconds)
     print(json.dumps(rec), flush=True)
