@@ -58,6 +58,57 @@ __device__ __forceinline__ Real normal_at(const LiborArgs& a, int p, int n, Real
   return j == 0 ? zq[0] : j == 1 ? zq[1] : j == 2 ? zq[2] : zq[3];
 }
 
+// portfolio_b (liborSwaptionGreeks.cpp:120-211) on the final rates L (row
+// 40): returns the discounted portfolio value and sets Lb = dV/dL(40, .).
+template <typename Real, bool kInplace>
+__device__ __forceinline__ Real portfolio_adjoint(const Real (&L)[NN], Real (&Lb)[NN]) {
+  using O = MathOps<Real>;
+  const Real delta = Real(0.25);
+  Real B[NMAT], Bb[NMAT], Sb[NMAT];
+  Real b = 1, s = 0, val = 0;
+#pragma unroll
+  for (int m = 0; m < NMAT; ++m) {
+    b = O::div(b, Real(1) + delta * L[m + NMAT]);
+    s = s + delta * b;
+    B[m] = b;
+    Bb[m] = 0;
+    Sb[m] = 0;
+#pragma unroll
+    for (int j = 0; j < NOPT; ++j) {
+      if (mat_of(j) - 1 == m) {
+        const Real K = (Real)rate_of(j);
+        const Real sw = b + K * s - Real(1);
+        if (sw < Real(0)) {
+          val += Real(-100) * sw;
+          Sb[m] += Real(-100) * K;
+          Bb[m] += Real(-100);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int m = NMAT - 1; m >= 0; --m) {
+    const int n = m + NMAT;
+    Bb[m] += delta * Sb[m];
+    const Real dn = Real(1) + delta * L[n];
+    Lb[n] = O::div(-Bb[m] * B[m] * delta, dn);
+    if (m > 0) {
+      Sb[m - 1] += Sb[m];
+      Bb[m - 1] += O::div(Bb[m], kInplace ? Real(1) + delta * Lb[n] : dn);
+    }
+  }
+  b = 1;
+#pragma unroll
+  for (int n = 0; n < NMAT; ++n) b = O::div(b, Real(1) + delta * L[n]);
+  val = b * val;
+#pragma unroll
+  for (int n = 0; n < NMAT; ++n) Lb[n] = O::div(-val * delta, Real(1) + delta * L[n]);
+#pragma unroll
+  for (int n = NMAT; n < NN; ++n) Lb[n] = b * Lb[n];
+
+  return val;
+}
+
 template <typename Real, bool kBuffer, bool kInplace>
 __global__ __launch_bounds__(kBlock) void libor_path_kernel(LiborArgs a) {
   using O = MathOps<Real>;
@@ -94,48 +145,8 @@ __global__ __launch_bounds__(kBlock) void libor_path_kernel(LiborArgs a) {
   }
 
   // ---- portfolio_b ---------------------------------------------------------------
-  Real B[NMAT], Bb[NMAT], Sb[NMAT];
-  Real b = 1, s = 0, val = 0;
-#pragma unroll
-  for (int m = 0; m < NMAT; ++m) {
-    b = O::div(b, Real(1) + delta * L[m + NMAT]);
-    s = s + delta * b;
-    B[m] = b;
-    Bb[m] = 0;
-    Sb[m] = 0;
-#pragma unroll
-    for (int j = 0; j < NOPT; ++j) {
-      if (mat_of(j) - 1 == m) {
-        const Real K = (Real)rate_of(j);
-        const Real sw = b + K * s - Real(1);
-        if (sw < Real(0)) {
-          val += Real(-100) * sw;
-          Sb[m] += Real(-100) * K;
-          Bb[m] += Real(-100);
-        }
-      }
-    }
-  }
   Real Lb[NN];
-#pragma unroll
-  for (int m = NMAT - 1; m >= 0; --m) {
-    const int n = m + NMAT;
-    Bb[m] += delta * Sb[m];
-    const Real dn = Real(1) + delta * L[n];
-    Lb[n] = O::div(-Bb[m] * B[m] * delta, dn);
-    if (m > 0) {
-      Sb[m - 1] += Sb[m];
-      Bb[m - 1] += O::div(Bb[m], kInplace ? Real(1) + delta * Lb[n] : dn);
-    }
-  }
-  b = 1;
-#pragma unroll
-  for (int n = 0; n < NMAT; ++n) b = O::div(b, Real(1) + delta * L[n]);
-  val = b * val;
-#pragma unroll
-  for (int n = 0; n < NMAT; ++n) Lb[n] = O::div(-val * delta, Real(1) + delta * L[n]);
-#pragma unroll
-  for (int n = NMAT; n < NN; ++n) Lb[n] = b * Lb[n];
+  const Real val = portfolio_adjoint<Real, kInplace>(L, Lb);
 
   // ---- adjoint: path_calc_b2 (L holds row n+1, Lb the adjoints) ---------------------
   for (int n = NMAT - 1; n >= 0; --n) {
@@ -164,6 +175,102 @@ __global__ __launch_bounds__(kBlock) void libor_path_kernel(LiborArgs a) {
   }
 
   Real* out = (Real*)a.out + p;
+  out[0] = val;
+#pragma unroll
+  for (int i = 0; i < NN; ++i) out[(size_t)(i + 1) * P] = Lb[i];
+}
+
+// FP32 variant without the trajectory buffer.  The reverse sweep rebuilds row
+// n from row n+1 instead of reading it back: for i = n+1, n+2, ... the forward
+// map L(n+1,i) = x exp(con1 (v(i-1) + con1 x / (1 + delta x)) + c2(n)) with
+// x = L(n,i) is inverted through the fixed point
+//   x = w exp(-a t(x)),  t(x) = x / (1 + delta x),  a = con1^2,
+//   w = L(n+1,i) exp(-(con1 v(i-1) + c2(n))),
+// a contraction of factor a t'(x) x ~ 1e-4 (con1 = delta lambda): one step
+// from x = w, x1 = w exp(-a t(w)), plus its second-order term,
+// x = x1 (1 + a^2 t(w)^2 / (1 + delta w)), reaches FP32 rounding (checked
+// for L up to 1 against two full iterations).  The adjoint then runs in
+// the same ascending order: its running sum v1(i) = sum_{k >= i} lambda
+// L(n+1,k) Lb(k) is taken as the row total minus the prefix below i.  No
+// HBM traffic beyond the [81][path] outputs; the cost is about 1.7x the
+// VALU work of the stored-trajectory kernel.
+template <bool kBuffer, bool kInplace>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void libor_path_rc_kernel(
+    LiborArgs a) {  // two waves per SIMD: at most 256 registers per lane, no spills (80 rates + 80 adjoints)
+  using O = MathOps<float>;
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= a.n_paths) return;  // no workgroup synchronisation below
+  const size_t P = (size_t)a.n_paths;
+  const float delta = 0.25f, sqrt_delta = 0.5f;
+  const float lam = (float)a.lambda;
+  const float con1 = delta * lam;
+  const float ce = con1 * O::kE;   // exponent scale of v (log2 domain)
+  const float kk = ce * con1;      // exponent scale of a rate's own term
+  const float a2 = (float)((double)kk * 0.69314718055994531 * (double)kk * 0.69314718055994531);
+  const float L0 = (float)a.L0;
+
+  // ---- forward: path_calc_b1 (no stores) ---------------------------------------
+  float L[NN];
+#pragma unroll
+  for (int i = 0; i < NN; ++i) L[i] = L0;
+  float zq[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int n = 0; n < NMAT; ++n) {
+    const float sqez = sqrt_delta * normal_at<float, kBuffer>(a, p, n, zq);
+    const float c2 = lam * (sqez - 0.5f * con1) * O::kE;
+    float v = 0.f;
+#pragma unroll
+    for (int i = 1; i < NN; ++i) {
+      if (i > n) {
+        const float Li = L[i];
+        v += con1 * O::div(Li, 1.f + delta * Li);
+        L[i] = Li * O::ex(ce * v + c2);
+      }
+    }
+  }
+
+  float Lb[NN];
+  const float val = portfolio_adjoint<float, kInplace>(L, Lb);
+
+  // ---- adjoint: path_calc_b2, rows rebuilt in registers ------------------------
+  for (int n = NMAT - 1; n >= 0; --n) {
+    float z;
+    if (kBuffer) {
+      z = (float)a.z[(size_t)p * NMAT + n];
+    } else {
+      // regenerated per step (one Philox call per step, not per four): keeps
+      // the four normals out of the registers live across the row loops
+      float zr[4];
+      normals4<float>(U4{(uint32_t)p, (uint32_t)(n >> 2), 0u, a.w}, a.k0, a.k1, zr[0], zr[1], zr[2], zr[3]);
+      const int j = n & 3;
+      z = j == 0 ? zr[0] : j == 1 ? zr[1] : j == 2 ? zr[2] : zr[3];
+    }
+    const float c2 = lam * (sqrt_delta * z - 0.5f * con1) * O::kE;
+    float S = 0.f;
+#pragma unroll
+    for (int i = 1; i < NN; ++i)
+      if (i > n) S += lam * L[i] * Lb[i];
+    float v = 0.f, P0 = 0.f;
+#pragma unroll
+    for (int i = 1; i < NN; ++i) {
+      if (i > n) {
+        const float y = L[i];  // L(n+1, i)
+        const float w = y * O::ex(-(ce * v + c2));
+        const float r0 = __builtin_amdgcn_rcpf(1.f + delta * w);
+        const float q = w * r0;
+        const float x1 = w * O::ex(-kk * q);
+        const float x = x1 + x1 * (a2 * (q * q) * r0);
+        const float r = __builtin_amdgcn_rcpf(1.f + delta * x);
+        v += con1 * (x * r);
+        const float faci = delta * r;
+        const float v1 = S - P0;
+        P0 += lam * y * Lb[i];
+        Lb[i] = Lb[i] * O::div(y, x) + v1 * lam * faci * faci;
+        L[i] = x;
+      }
+    }
+  }
+
+  float* out = (float*)a.out + p;
   out[0] = val;
 #pragma unroll
   for (int i = 0; i < NN; ++i) out[(size_t)(i + 1) * P] = Lb[i];
@@ -228,7 +335,8 @@ LiborPlan libor_plan(const fsmc_libor_params* p, const fsmc_mode* m) {
   LiborPlan l;
   l.real = m->precision == FSMC_FP64 ? sizeof(double) : sizeof(float);
   const size_t P = (size_t)p->n_paths;
-  l.traj_bytes = align_up((size_t)kSlots * P * l.real, 256);
+  // the FP32 kernel rebuilds the trajectory in registers; FP64 stores it
+  l.traj_bytes = m->precision == FSMC_FP64 ? align_up((size_t)kSlots * P * l.real, 256) : 0;
   l.out_bytes = align_up((size_t)(NN + 1) * P * l.real, 256);
   l.part_bytes = align_up((size_t)(NN + 1) * kReduceChunks * 2 * sizeof(double), 256);
   l.res_bytes = align_up(2 * (NN + 1) * sizeof(double), 256);
@@ -247,8 +355,11 @@ int libor_validate(const fsmc_libor_params* p, const fsmc_mode* m) {
 
 template <typename Real, bool B, bool Q>
 void launch(const LiborArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((libor_path_kernel<Real, B, Q>), dim3((a.n_paths + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                     a);
+  const dim3 grid((a.n_paths + kBlock - 1) / kBlock);
+  if constexpr (sizeof(Real) == 4)
+    hipLaunchKernelGGL((libor_path_rc_kernel<B, Q>), grid, dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((libor_path_kernel<Real, B, Q>), grid, dim3(kBlock), 0, s, a);
 }
 
 template <typename Real>

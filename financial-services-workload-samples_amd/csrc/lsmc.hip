@@ -13,9 +13,14 @@
 //     that keeps the Gram matrix well conditioned.  In FP32 the Gram matrix
 //     and right-hand side come from v_mfma_f32_4x4x1f32 outer products
 //     (a = (1, x, x^2, x^3), b = (1, x^3, Y, 0) gives all seven moments and
-//     the four right-hand sides); FP64 accumulates on the VALU.  The last
-//     workgroup of each option (device-scope ticket, no waiting) combines the
-//     partial sums in FP64 and solves the 4x4 system by Cholesky in FP64.
+//     the four right-hand sides); FP64 accumulates on the VALU.  Each
+//     workgroup writes its partial sums;
+//   * lsmc_solve_kernel, after each pass: one workgroup per option still
+//     regressing combines that option's partials in FP64 (fixed order) and
+//     solves the 4x4 system by Cholesky in FP64.  A separate launch rather
+//     than a last-workgroup ticket: on this GPU every device-scope release
+//     fence writes back the XCD's L2, and 35k such fences per pass cost
+//     milliseconds, while the kernel boundary costs microseconds.
 //   * lsmc_final_kernel: price = max(K - S0, mean discounted cash flow).
 // Options are sorted by step count (descending) so the options still active
 // at backward pass j are a prefix of the workgroup table: pass j launches
@@ -23,8 +28,8 @@
 //
 // Reference algorithm ("refcompat", MonteCarloAmericanOptions/main.cpp:115-279
 // and utilities.cpp:42-123) for one option: path matrix, lagged quadratic
-// regression with ridge 0.01, cofactor 3x3 inverse, no discounting; same
-// ticket scheme for the per-step solve.
+// regression with ridge 0.01, cofactor 3x3 inverse, no discounting; the
+// per-step solve is a one-workgroup kernel after each step kernel.
 #include <algorithm>
 #include <numeric>
 #include <vector>
@@ -36,7 +41,7 @@ namespace fsmc {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kPPT = 4;  // paths per thread per workgroup
+constexpr int kPPT = 8;  // paths per thread per workgroup
 constexpr int kChunk = kBlock * kPPT;
 constexpr int kNS = 12;  // partial sums: m0..m6, c0..c3, spare
 constexpr double kLog2e = 1.4426950408889634074;
@@ -56,28 +61,13 @@ struct LsmcArgs {
   void* mat;
   void* cf;
   double* partials;    // [workgroup][kNS]
-  unsigned* tickets;   // [option]
+  const int* order;    // option indices sorted by steps (descending)
   double* beta;        // [2][option][5]: coefficients, valid flag (double-buffered by pass)
   double* eur;         // [workgroup] European payoff sums
   int n_options;
   int pass;
   uint32_t k0, k1, w;
 };
-
-// Ticket of a finishing workgroup for option o: returns true in every thread
-// of the workgroup that finishes last (its partials and everyone else's are
-// visible to it).  The last workgroup resets the ticket for the next launch.
-__device__ __forceinline__ bool last_of_option(unsigned* ticket, int nblk) {
-  __shared__ unsigned s_ticket;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0)
-    s_ticket = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  const bool last = s_ticket == (unsigned)(nblk - 1);
-  if (last) __threadfence();
-  return last;
-}
 
 template <typename Real>
 __device__ __forceinline__ Real pick4(const Real (&z)[4], int j) {
@@ -102,21 +92,36 @@ __global__ __launch_bounds__(kBlock) void lsmc_paths_kernel(LsmcArgs a) {
   Real* cf = (Real*)a.cf + m.cf_off;
   const size_t P = (size_t)m.paths;
   double eu[1] = {0.0};
-  for (int q = 0; q < kPPT; ++q) {
-    const int p = e.y * kChunk + q * kBlock + threadIdx.x;
-    if (p >= m.paths) continue;
-    Real S = (Real)m.S0;
-    Real z[4] = {0, 0, 0, 0};
-    for (int j = 0; j < m.steps; ++j) {
-      if ((j & 3) == 0)
-        normals4<Real>(U4{(uint32_t)p, (uint32_t)(j >> 2), (uint32_t)e.x, a.w}, a.k0, a.k1, z[0], z[1], z[2], z[3]);
-      S = S * O::ex(drift + vol * pick4(z, j & 3));
-      mat[(size_t)j * P + p] = S;
+  // four paths per thread advance together (independent exp chains and
+  // Philox calls in flight), each storing its row entry per date
+  constexpr int kG = 4;
+  for (int q0 = 0; q0 < kPPT; q0 += kG) {
+    int pp[kG];
+    Real S[kG], z[kG][4];
+#pragma unroll
+    for (int g = 0; g < kG; ++g) {
+      pp[g] = e.y * kChunk + (q0 + g) * kBlock + threadIdx.x;
+      S[g] = (Real)m.S0;
     }
-    const Real pay = K - S > Real(0) ? K - S : Real(0);
-    const Real c = discT * pay;
-    cf[p] = c;
-    eu[0] += (double)c;
+    for (int j = 0; j < m.steps; ++j) {
+#pragma unroll
+      for (int g = 0; g < kG; ++g) {
+        if (pp[g] >= m.paths) continue;
+        if ((j & 3) == 0)
+          normals4<Real>(U4{(uint32_t)pp[g], (uint32_t)(j >> 2), (uint32_t)e.x, a.w}, a.k0, a.k1, z[g][0], z[g][1],
+                         z[g][2], z[g][3]);
+        S[g] = S[g] * O::ex(drift + vol * pick4(z[g], j & 3));
+        mat[(size_t)j * P + pp[g]] = S[g];
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < kG; ++g) {
+      if (pp[g] >= m.paths) continue;
+      const Real pay = K - S[g] > Real(0) ? K - S[g] : Real(0);
+      const Real c = discT * pay;
+      cf[pp[g]] = c;
+      eu[0] += (double)c;
+    }
   }
   block_sum<1>(eu, lds);
   if (threadIdx.x == 0) a.eur[blockIdx.x] = eu[0];
@@ -155,15 +160,24 @@ __device__ __forceinline__ void accum_mfma(v4f& acc, bool itm, float x, float y)
   }
 }
 
-// Solves the 4x4 normal equations of option o from its workgroups' partials.
-__device__ void lsmc_solve(const LsmcArgs& a, const OptMeta& m, double* beta_out) {
-  double s[kNS];
+// Sums the NV-wide partials of workgroups [first, first + n) with every
+// thread of the calling workgroup: thread t takes workgroups t, t + 256, ...
+// in order, then the fixed block tree (so the result is reproducible).  The
+// total lands in thread 0.
+template <int NV>
+__device__ __forceinline__ void combine_partials(const double* partials, int first, int n, double (&t)[NV],
+                                                 double* lds) {
 #pragma unroll
-  for (int q = 0; q < kNS; ++q) s[q] = 0.0;
-  for (int b = 0; b < m.nblk; ++b) {
+  for (int q = 0; q < NV; ++q) t[q] = 0.0;
+  for (int b = threadIdx.x; b < n; b += kBlock) {
 #pragma unroll
-    for (int q = 0; q < kNS; ++q) s[q] += a.partials[(size_t)(m.first_block + b) * kNS + q];
+    for (int q = 0; q < NV; ++q) t[q] += partials[(size_t)(first + b) * NV + q];
   }
+  block_sum<NV>(t, lds);
+}
+
+// Solves the 4x4 normal equations from the option's summed moments s.
+__device__ void lsmc_solve(const double (&s)[kNS], double* beta_out) {
   // Cholesky G = L L^T of G_ij = m_{i+j}; a pivot that is not clearly
   // positive (fewer than 4 ITM paths, or a singular basis) disables exercise
   // at this date.  Fully unrolled so L stays in registers.
@@ -215,7 +229,6 @@ __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
   const int k = d - 1;             // regression date
   const double dt = m.T / m.steps;
   const double* bin = a.beta + (size_t)((a.pass & 1) * a.n_options + o) * 5;
-  double* bout = a.beta + (size_t)(((a.pass + 1) & 1) * a.n_options + o) * 5;
   const bool decide = a.pass >= 1 && bin[4] != 0.0;
   const Real b0 = (Real)bin[0], b1 = (Real)bin[1], b2 = (Real)bin[2], b3 = (Real)bin[3];
   const Real K = (Real)m.K, invK = (Real)(1.0 / m.K);
@@ -228,30 +241,36 @@ __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
 #pragma unroll
   for (int q = 0; q < kNS; ++q) s[q] = 0.0;
   v4f acc = {0.f, 0.f, 0.f, 0.f};
+  // every load of the workgroup's paths is issued before any use: a pass is
+  // one short stream per path, so memory-level parallelism sets its speed.
+  // A spot of K marks "not in the money" (dead lanes, no decision date).
+  Real cv[kPPT], sd[kPPT], sk[kPPT];
+#pragma unroll
   for (int q = 0; q < kPPT; ++q) {
     const int p = e.y * kChunk + q * kBlock + threadIdx.x;
     const bool live = p < m.paths;
-    Real c = live ? cf[p] : Real(0);
-    if (live && decide) {
-      const Real Sd = mat[(size_t)(d - 1) * P + p];
-      if (K - Sd > Real(0)) {
-        const Real x = Real(1) + (Sd * invK - Real(1)) * isw_d;
-        const Real cont = b0 + x * (b1 + x * (b2 + x * b3));
-        const Real ex = (K - Sd) * disc_d;
-        if (ex > cont) {
-          c = ex;
-          cf[p] = c;
-        }
+    cv[q] = live ? cf[p] : Real(0);
+    sd[q] = live && a.pass >= 1 ? mat[(size_t)(d - 1) * P + p] : K;  // decide is known later
+    sk[q] = live && k >= 1 ? mat[(size_t)(k - 1) * P + p] : K;
+  }
+#pragma unroll
+  for (int q = 0; q < kPPT; ++q) {
+    const int p = e.y * kChunk + q * kBlock + threadIdx.x;
+    Real c = cv[q];
+    const Real Sd = sd[q];
+    if (decide && K - Sd > Real(0)) {
+      const Real x = Real(1) + (Sd * invK - Real(1)) * isw_d;
+      const Real cont = b0 + x * (b1 + x * (b2 + x * b3));
+      const Real ex = (K - Sd) * disc_d;
+      if (ex > cont) {
+        c = ex;
+        cf[p] = c;
       }
     }
     if (k >= 1) {  // uniform per workgroup
-      bool itm = false;
-      Real x = 0;
-      if (live) {
-        const Real Sk = mat[(size_t)(k - 1) * P + p];
-        itm = K - Sk > Real(0);
-        x = Real(1) + (Sk * invK - Real(1)) * isw_k;
-      }
+      const Real Sk = sk[q];
+      const bool itm = K - Sk > Real(0);
+      const Real x = itm ? Real(1) + (Sk * invK - Real(1)) * isw_k : Real(0);
       if constexpr (kMfma)
         accum_mfma(acc, itm, (float)x, (float)c);
       else
@@ -288,12 +307,20 @@ __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
     double* dst = a.partials + (size_t)blockIdx.x * kNS;
     for (int q = 0; q < kNS; ++q) dst[q] = s[q];
   }
-  if (last_of_option(a.tickets + o, m.nblk)) {
-    if (threadIdx.x == 0) {
-      lsmc_solve(a, m, bout);
-      a.tickets[o] = 0u;
-    }
-  }
+}
+
+// After pass j: workgroup i takes the i-th option by step count (the options
+// still alive at pass j are a prefix), sums its workgroups' partials and
+// solves for the coefficients that pass j + 1 applies.
+__global__ __launch_bounds__(kBlock) void lsmc_solve_kernel(LsmcArgs a) {
+  __shared__ double lds[kNS * (kBlock / 64)];
+  const int o = a.order[blockIdx.x];
+  const OptMeta m = a.meta[o];
+  if (m.steps - a.pass - 1 < 1) return;  // no regression date left (uniform)
+  double* bout = a.beta + (size_t)(((a.pass + 1) & 1) * a.n_options + o) * 5;
+  double t[kNS];
+  combine_partials<kNS>(a.partials, m.first_block, m.nblk, t, lds);
+  if (threadIdx.x == 0) lsmc_solve(t, bout);
 }
 
 template <typename Real>
@@ -309,6 +336,11 @@ __global__ __launch_bounds__(kBlock) void lsmc_final_kernel(LsmcArgs a, double* 
     s[1] += c * c;
   }
   block_sum<2>(s, lds);
+  double eu[1] = {0.0};
+  if (european) {  // uniform; the barrier orders thread 0's reads of lds before its reuse
+    __syncthreads();
+    combine_partials<1>(a.eur, m.first_block, m.nblk, eu, lds);
+  }
   if (threadIdx.x == 0) {
     const double N = (double)m.paths;
     const double mean = s[0] / N;
@@ -316,17 +348,8 @@ __global__ __launch_bounds__(kBlock) void lsmc_final_kernel(LsmcArgs a, double* 
     const double now = m.K - m.S0 > 0.0 ? m.K - m.S0 : 0.0;
     price2[2 * o] = mean > now ? mean : now;
     price2[2 * o + 1] = sqrt(var > 0.0 ? var : 0.0) / sqrt(N);
-    if (european) {
-      double eu = 0.0;
-      for (int b = 0; b < m.nblk; ++b) eu += a.eur[m.first_block + b];
-      european[o] = eu / N;
-    }
+    if (european) european[o] = eu[0] / N;
   }
-}
-
-__global__ void zero_u32_kernel(unsigned* p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0u;
 }
 
 __global__ void zero_f64_kernel(double* p, int n) {
@@ -346,7 +369,7 @@ struct RefArgs {
   void* a;                   // Real [n_steps + 1][n_paths]
   void* cf;                  // Real [n_paths]
   double* partials;          // [workgroup][8]
-  unsigned* ticket;
+  int n_partials;            // workgroups of the step kernel
   double* beta;              // [2][3] double-buffered by step parity
   double* out;               // [2]: american, european
 };
@@ -386,7 +409,6 @@ __global__ __launch_bounds__(kBlock) void ref_step_kernel(RefArgs r, int s) {
   const Real* ap = (const Real*)r.a + (size_t)s * N;
   Real* cf = (Real*)r.cf;
   const double* bin = r.beta + (s & 1) * 3;
-  double* bout = r.beta + ((s + 1) & 1) * 3;
   const Real b0 = (Real)bin[0], b1 = (Real)bin[1], b2 = (Real)bin[2];
   const Real K = (Real)r.strike;
   const Real avg = (Real)(1.0 / r.n_paths);
@@ -419,10 +441,15 @@ __global__ __launch_bounds__(kBlock) void ref_step_kernel(RefArgs r, int s) {
   block_sum<8>(v, lds);
   if (threadIdx.x == 0)
     for (int q = 0; q < 8; ++q) r.partials[(size_t)blockIdx.x * 8 + q] = v[q];
-  if (last_of_option(r.ticket, gridDim.x) && threadIdx.x == 0) {
-    double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (unsigned b = 0; b < gridDim.x; ++b)
-      for (int q = 0; q < 8; ++q) t[q] += r.partials[(size_t)b * 8 + q];
+}
+
+// One workgroup after each step kernel: the regression of step s.
+__global__ __launch_bounds__(kBlock) void ref_solve_kernel(RefArgs r, int s) {
+  __shared__ double lds[8 * (kBlock / 64)];
+  double* bout = r.beta + ((s + 1) & 1) * 3;
+  double t[8];
+  combine_partials<8>(r.partials, 0, r.n_partials, t, lds);
+  if (threadIdx.x == 0) {
     const double ridge = 0.01;  // main.cpp:244-246
     const double s0 = t[0] + ridge, s1 = t[1] + ridge, s2 = t[2] + ridge, s3 = t[3] + ridge, s4 = t[4] + ridge;
     const double x[3][3] = {{s0, s1, s2}, {s1, s2, s3}, {s2, s3, s4}};
@@ -443,7 +470,6 @@ __global__ __launch_bounds__(kBlock) void ref_step_kernel(RefArgs r, int s) {
     bout[0] = t[5] * inv[0][0] + t[6] * inv[1][0] + t[7] * inv[2][0];
     bout[1] = t[5] * inv[0][1] + t[6] * inv[1][1] + t[7] * inv[2][1];
     bout[2] = t[5] * inv[0][2] + t[6] * inv[1][2] + t[7] * inv[2][2];
-    *r.ticket = 0u;
   }
 }
 
@@ -478,10 +504,12 @@ struct LsmcPlanImpl {
   fsmc_mode mode;
   std::vector<OptMeta> meta;
   std::vector<int2> table;
+  std::vector<int> order;         // option indices sorted by steps (descending)
   std::vector<int> grid_of_pass;  // workgroups launched by backward pass j
+  std::vector<int> live_of_pass;  // options with a date left at backward pass j
   int max_steps = 0;
   size_t real = 4, mat_elems = 0, cf_elems = 0;
-  size_t off_meta, off_table, off_mat, off_cf, off_part, off_tick, off_beta, off_eur, off_out, ws = 0;
+  size_t off_meta, off_table, off_order, off_mat, off_cf, off_part, off_beta, off_eur, off_out, ws = 0;
 };
 
 int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K, const double* sigma,
@@ -525,12 +553,18 @@ int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K,
   pl->mat_elems = (size_t)mat;
   pl->cf_elems = (size_t)cfo;
   pl->max_steps = steps[order[0]];
+  pl->order = order;
   pl->grid_of_pass.assign(pl->max_steps, 0);
+  pl->live_of_pass.assign(pl->max_steps, 0);
   // options sorted by steps: those with steps > j are a prefix of the table
   for (int j = 0; j < pl->max_steps; ++j) {
-    int blocks = 0;
-    for (int idx = 0; idx < n && steps[order[idx]] > j; ++idx) blocks += pl->meta[order[idx]].nblk;
+    int blocks = 0, live = 0;
+    for (int idx = 0; idx < n && steps[order[idx]] > j; ++idx) {
+      blocks += pl->meta[order[idx]].nblk;
+      ++live;
+    }
     pl->grid_of_pass[j] = blocks;
+    pl->live_of_pass[j] = live;
   }
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -540,10 +574,10 @@ int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K,
   };
   pl->off_meta = take(sizeof(OptMeta) * n);
   pl->off_table = take(sizeof(int2) * pl->table.size());
+  pl->off_order = take(sizeof(int) * n);
   pl->off_mat = take(pl->real * pl->mat_elems);
   pl->off_cf = take(pl->real * pl->cf_elems);
   pl->off_part = take(sizeof(double) * kNS * pl->table.size());
-  pl->off_tick = take(sizeof(unsigned) * n);
   pl->off_beta = take(sizeof(double) * 10 * n);
   pl->off_eur = take(sizeof(double) * pl->table.size());
   pl->off_out = take(sizeof(double) * 3 * n);
@@ -559,7 +593,7 @@ int lsmc_enqueue(const LsmcPlanImpl& pl, char* ws, hipStream_t s) {
   a.mat = ws + pl.off_mat;
   a.cf = ws + pl.off_cf;
   a.partials = (double*)(ws + pl.off_part);
-  a.tickets = (unsigned*)(ws + pl.off_tick);
+  a.order = (const int*)(ws + pl.off_order);
   a.beta = (double*)(ws + pl.off_beta);
   a.eur = (double*)(ws + pl.off_eur);
   a.n_options = pl.n;
@@ -567,7 +601,6 @@ int lsmc_enqueue(const LsmcPlanImpl& pl, char* ws, hipStream_t s) {
   a.k1 = (uint32_t)(pl.mode.seed >> 32);
   a.w = counter_w(kTagLsmc, pl.mode.stream);
   a.pass = 0;
-  hipLaunchKernelGGL(zero_u32_kernel, dim3((pl.n + 255) / 256), dim3(256), 0, s, a.tickets, pl.n);
   hipLaunchKernelGGL(zero_f64_kernel, dim3((10 * pl.n + 255) / 256), dim3(256), 0, s, a.beta, 10 * pl.n);
   const int nb = (int)pl.table.size();
   hipLaunchKernelGGL(lsmc_paths_kernel<Real>, dim3(nb), dim3(kBlock), 0, s, a);
@@ -575,6 +608,7 @@ int lsmc_enqueue(const LsmcPlanImpl& pl, char* ws, hipStream_t s) {
   for (int j = 0; j < pl.max_steps; ++j) {
     a.pass = j;
     hipLaunchKernelGGL(lsmc_pass_kernel<Real>, dim3(pl.grid_of_pass[j]), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(lsmc_solve_kernel, dim3(pl.live_of_pass[j]), dim3(kBlock), 0, s, a);
   }
   FSMC_HIP(hipGetLastError());
   double* out = (double*)(ws + pl.off_out);
@@ -612,6 +646,8 @@ int fsmc_lsmc_plan_create(fsmc_ctx* ctx, int n_options, const fsmc_mode* m, cons
       if (hipMemcpyAsync(ws + pl->impl.off_meta, pl->impl.meta.data(), sizeof(OptMeta) * n_options,
                          hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
           hipMemcpyAsync(ws + pl->impl.off_table, pl->impl.table.data(), sizeof(int2) * pl->impl.table.size(),
+                         hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+          hipMemcpyAsync(ws + pl->impl.off_order, pl->impl.order.data(), sizeof(int) * n_options,
                          hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
           hipStreamSynchronize(ctx->stream) != hipSuccess) {
         set_error("upload of the LSMC plan failed");
@@ -689,8 +725,7 @@ int fsmc_american_lsmc_ref(fsmc_ctx* ctx, const fsmc_lsmc_ref_params* p, const f
     return at;
   };
   const size_t o_a = take(real * (size_t)(p->n_steps + 1) * N), o_cf = take(real * N),
-               o_part = take(sizeof(double) * 8 * nb), o_tick = take(sizeof(unsigned)),
-               o_beta = take(sizeof(double) * 6), o_out = take(sizeof(double) * 2);
+               o_part = take(sizeof(double) * 8 * nb), o_beta = take(sizeof(double) * 6), o_out = take(sizeof(double) * 2);
   const size_t zn = buffer ? N * (size_t)p->n_steps : 0;
   FSMC_TRY(ctx_prepare(ctx, 0, zn * sizeof(double), 0, off));
   hipStream_t s = ctx->stream;
@@ -712,10 +747,9 @@ int fsmc_american_lsmc_ref(fsmc_ctx* ctx, const fsmc_lsmc_ref_params* p, const f
   r.a = ws + o_a;
   r.cf = ws + o_cf;
   r.partials = (double*)(ws + o_part);
-  r.ticket = (unsigned*)(ws + o_tick);
+  r.n_partials = nb;
   r.beta = (double*)(ws + o_beta);
   r.out = (double*)(ws + o_out);
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, r.ticket, 1);
   hipLaunchKernelGGL(zero_f64_kernel, dim3(1), dim3(64), 0, s, r.beta, 6);
   const int pb = (int)((N + kBlock - 1) / kBlock);
   const bool f64 = m->precision == FSMC_FP64;
@@ -733,6 +767,7 @@ int fsmc_american_lsmc_ref(fsmc_ctx* ctx, const fsmc_lsmc_ref_params* p, const f
       hipLaunchKernelGGL(ref_step_kernel<double>, dim3(nb), dim3(kBlock), 0, s, r, st);
     else
       hipLaunchKernelGGL(ref_step_kernel<float>, dim3(nb), dim3(kBlock), 0, s, r, st);
+    hipLaunchKernelGGL(ref_solve_kernel, dim3(1), dim3(kBlock), 0, s, r, st);
   }
   FSMC_HIP(hipGetLastError());
   if (f64)
