@@ -64,10 +64,12 @@ __host__ __device__ __forceinline__ uint32_t counter_w(uint32_t tag, uint32_t st
 // same uniforms.  z0 = r cos(2 pi u2), z1 = r sin(2 pi u2), r = sqrt(-2 ln u1).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float& z0, float& z1) {
-  const float u1 = (float)((a >> 8) + 1u) * 0x1.0p-24f;
+  // u1 = f 2^-24 with f = (a >> 8) + 1 exact; v_log_f32 is log2, so
+  // -2 ln u1 = -2 ln2 (log2 f - 24) = fma(-2 ln2, log2 f, 48 ln2).
+  const float f = (float)((a >> 8) + 1u);
   const float u2 = (float)(b >> 8) * 0x1.0p-24f;
-  // v_log_f32 is log2; -2 ln u = -2 ln2 log2 u.
-  const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+  const float r = __builtin_amdgcn_sqrtf(__builtin_fmaf(-1.3862943611198906f, __builtin_amdgcn_logf(f),
+                                                       33.271064666877376f));
   // v_sin_f32 / v_cos_f32 take their argument in revolutions.
   z0 = r * __builtin_amdgcn_cosf(u2);
   z1 = r * __builtin_amdgcn_sinf(u2);
@@ -115,6 +117,16 @@ struct CompSum {
 template <>
 struct CompSum<double> {
   double s = 0;
+  __device__ __forceinline__ void add(double x) { s += x; }
+  __device__ __forceinline__ double value() const { return s; }
+};
+
+// Running sum of one path's monitored spots (the arithmetic-average leg): an
+// FP64 accumulator fed FP32 or FP64 terms.  On this GPU an FP64 add issues at
+// the FP32 rate, so one conversion and one add per term cost less than the
+// four operations of an FP32 compensated sum and lose nothing.
+struct PathSum {
+  double s = 0.0;
   __device__ __forceinline__ void add(double x) { s += x; }
   __device__ __forceinline__ double value() const { return s; }
 };

@@ -38,7 +38,7 @@ __global__ __launch_bounds__(kBlock) void heston_paths_kernel(HestonArgs a) {
   if (p < a.n_paths) {
     Real S = (Real)a.S[o];
     Real v = (Real)a.v0[o];
-    CompSum<Real> avg;
+    PathSum avg;
     if (kBuffer) {
       const int nchunks = a.n_paths / kChunk;
       const size_t base = ((size_t)o * nchunks + (p / kChunk)) * (size_t)a.n_steps * kChunk + (p % kChunk);

@@ -10,11 +10,13 @@
 //     regression of the discounted cash flow on {1, x, x^2, x^3} over the
 //     in-the-money paths of date d-1.  x = 1 + (S/K - 1)/(sigma sqrt(t)) is
 //     an affine map of S/K (same fitted values as the plain monomials of S/K)
-//     that keeps the Gram matrix well conditioned.  In FP32 the Gram matrix
-//     and right-hand side come from v_mfma_f32_4x4x1f32 outer products
-//     (a = (1, x, x^2, x^3), b = (1, x^3, Y, 0) gives all seven moments and
-//     the four right-hand sides); FP64 accumulates on the VALU.  Each
-//     workgroup writes its partial sums;
+//     that keeps the Gram matrix well conditioned.  Each lane accumulates
+//     the seven moments and four right-hand sides of its own paths in FP64
+//     (FP64 FMA issues at the FP32 rate here, and a 4x4 Gram matrix is too
+//     small for matrix cores: a v_mfma_f32_4x4x1f32 formulation needs the
+//     path's basis values spread over four lanes and cost ~117 instructions
+//     per 64 paths against ~14 for per-lane FMAs).  Each workgroup writes its
+//     partial sums;
 //   * lsmc_solve_kernel, after each pass: one workgroup per option still
 //     regressing combines that option's partials in FP64 (fixed order) and
 //     solves the 4x4 system by Cholesky in FP64.  A separate launch rather
@@ -45,8 +47,6 @@ constexpr int kPPT = 8;  // paths per thread per workgroup
 constexpr int kChunk = kBlock * kPPT;
 constexpr int kNS = 12;  // partial sums: m0..m6, c0..c3, spare
 constexpr double kLog2e = 1.4426950408889634074;
-
-typedef float v4f __attribute__((ext_vector_type(4)));
 
 struct OptMeta {
   long long mat_off;  // first element of the option's path matrix (row k-1 = date k)
@@ -139,27 +139,6 @@ __device__ __forceinline__ void accum_valu(double (&s)[kNS], bool itm, double x,
   }
 }
 
-// FP32: 64 paths of a wavefront through four v_mfma_f32_4x4x1f32.  In round
-// t the 16 4-lane groups each take the path held by their lane t; lane i of
-// the group supplies a_i = x^i and b_i = (1, x^3, y, 0)[i] of that path, so
-// group g accumulates C_g[i][j] += a_i b_j.
-__device__ __forceinline__ void accum_mfma(v4f& acc, bool itm, float x, float y) {
-  const int lane = threadIdx.x & 63;
-  const int i = lane & 3;
-  const int qb = lane & ~3;
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const float xt = __shfl(x, qb + t, 64);
-    const float yt = __shfl(y, qb + t, 64);
-    const int it = __shfl(itm ? 1 : 0, qb + t, 64);
-    const float x2 = xt * xt, x3 = x2 * xt;
-    float av = i == 0 ? 1.0f : i == 1 ? xt : i == 2 ? x2 : x3;
-    float bv = i == 0 ? 1.0f : i == 1 ? x3 : i == 2 ? yt : 0.0f;
-    if (!it) av = 0.0f;
-    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(av, bv, acc, 0, 0, 0);
-  }
-}
-
 // Sums the NV-wide partials of workgroups [first, first + n) with every
 // thread of the calling workgroup: thread t takes workgroups t, t + 256, ...
 // in order, then the fixed block tree (so the result is reproducible).  The
@@ -220,7 +199,6 @@ __device__ void lsmc_solve(const double (&s)[kNS], double* beta_out) {
 
 template <typename Real>
 __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
-  constexpr bool kMfma = sizeof(Real) == 4;
   __shared__ double lds[kNS * (kBlock / 64)];
   const int2 e = a.table[blockIdx.x];
   const int o = e.x;
@@ -240,7 +218,6 @@ __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
   double s[kNS];
 #pragma unroll
   for (int q = 0; q < kNS; ++q) s[q] = 0.0;
-  v4f acc = {0.f, 0.f, 0.f, 0.f};
   // every load of the workgroup's paths is issued before any use: a pass is
   // one short stream per path, so memory-level parallelism sets its speed.
   // A spot of K marks "not in the money" (dead lanes, no decision date).
@@ -271,37 +248,10 @@ __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
       const Real Sk = sk[q];
       const bool itm = K - Sk > Real(0);
       const Real x = itm ? Real(1) + (Sk * invK - Real(1)) * isw_k : Real(0);
-      if constexpr (kMfma)
-        accum_mfma(acc, itm, (float)x, (float)c);
-      else
-        accum_valu(s, itm, (double)x, (double)c);
+      accum_valu(s, itm, (double)x, (double)c);
     }
   }
   if (k < 1) return;  // last pass of this option: decision only
-  if constexpr (kMfma) {
-    // lane l holds C_g[r][l & 3] (g = l / 4) in acc[r]; sum the 16 groups
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      float v = acc[r];
-      for (int off = 4; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
-      acc[r] = v;
-    }
-    const int lane = threadIdx.x & 63;
-    // column 0: m_r; column 1: m_{r+3}; column 2: c_r (lanes 0, 1, 2)
-    double col[4] = {(double)acc[0], (double)acc[1], (double)acc[2], (double)acc[3]};
-    double c0[4], c1[4], c2[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      c0[r] = __shfl(col[r], 0, 64);
-      c1[r] = __shfl(col[r], 1, 64);
-      c2[r] = __shfl(col[r], 2, 64);
-    }
-    if (lane == 0) {
-      for (int r = 0; r < 4; ++r) s[r] = c0[r];
-      for (int r = 1; r < 4; ++r) s[3 + r] = c1[r];
-      for (int r = 0; r < 4; ++r) s[7 + r] = c2[r];
-    }
-  }
   block_sum<kNS>(s, lds);
   if (threadIdx.x == 0) {
     double* dst = a.partials + (size_t)blockIdx.x * kNS;

@@ -53,9 +53,10 @@ __device__ __forceinline__ void heston_step(const HestonConsts<Real>& c, Real& S
 }
 
 // Two consecutive steps sharing one Philox call (normals n0..n3); the running
-// sum of the monitored spots takes the pair's sum in one compensated add.
+// sum of the monitored spots takes the pair's sum (one FP32 rounding) in one
+// FP64 add.
 template <typename Real, bool kVarAsVol>
-__device__ __forceinline__ void heston_pair(const HestonConsts<Real>& c, Real& S, Real& v, CompSum<Real>& avg,
+__device__ __forceinline__ void heston_pair(const HestonConsts<Real>& c, Real& S, Real& v, PathSum& avg,
                                             Real n0, Real n1, Real n2, Real n3, bool second) {
 #pragma clang fp contract(off)
   heston_step<Real, kVarAsVol>(c, S, v, n0, n1);
@@ -92,7 +93,7 @@ __device__ __forceinline__ double heston_adjoint_path(const HestonConsts<Real>& 
   const double dtd = Td / N;
   const Real hdt = (Real)(0.5 * dtd), sdt = (Real)sqrt(dtd);
   Real S = (Real)S0d, v = (Real)v0d;
-  CompSum<Real> avg;
+  PathSum avg;
   for (int seg = 0; seg < nseg; ++seg) {
     ck[(size_t)(2 * seg) * stride] = S;
     ck[(size_t)(2 * seg + 1) * stride] = v;
