@@ -41,6 +41,8 @@ R, RHO = 0.03, -0.7
 FLOP_PER_PATH_STEP = 34
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 FP64_VECTOR_PEAK_TFLOPS = 78.6
+# 256 CUs x 4 SIMDs, each issuing one wave64 VALU instruction per 4 cycles at 2.4 GHz
+VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4
 # LIBOR per path (same convention, division = reciprocal + multiply): each of
 # the 2380 forward (step, rate) updates 10 FLOP (2 T: rcp, exp2), each of the
 # 2380 adjoint updates 14 FLOP (2 T: two rcp), portfolio and its reverse
@@ -55,21 +57,24 @@ def _params(np):
                 xi=np.full(N_OPT, 0.3))
 
 
-def _traffic():
-    """HBM bytes per launch of the path kernel from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
-    path = os.path.join(ROOT, "profiles", "heston_asian_fp32_config1_traffic.txt")
-    if not os.path.exists(path):
-        return None
+def _profile_figures():
+    """HBM bytes and VALU wavefront instructions per launch of the path
+    kernel, from the committed rocprofv3 summary of this workload
+    (FETCH_SIZE + WRITE_SIZE scaled by a plain copy; SQ_INSTS_VALU)."""
+    path = os.path.join(ROOT, "profiles", "heston_asian_fp32_64x131072x365_traffic.txt")
     vals = {}
-    for line in open(path):
-        if "=" in line and not line.startswith("#"):
-            k, v = line.split("=", 1)
-            vals[k.strip()] = v.strip()
-    try:
-        return float(vals["bytes_per_launch"])
-    except (KeyError, ValueError):
-        return None
+    if os.path.exists(path):
+        for line in open(path):
+            if "=" in line and not line.startswith("#"):
+                k, v = line.split("=", 1)
+                vals[k.strip()] = v.strip()
+
+    def num(k):
+        try:
+            return float(vals[k])
+        except (KeyError, ValueError):
+            return None
+    return num("bytes_per_launch"), num("valu_insts_per_launch")
 
 
 def _cpu_baseline(np, P, target_s):
@@ -326,6 +331,7 @@ def main():
         print(json.dumps({"workload": "heston_adjoint", "ms": msg, "kernel_ms": kg}), flush=True)
         return
     ms, kern_ms, p32, c32 = run(m32, args.steps, args.warmup)
+    traffic, valu = _profile_figures()
     if args.workload == "heston":
         print(json.dumps({"workload": "heston", "ms": ms, "kernel_ms": kern_ms}), flush=True)
         return
@@ -339,9 +345,11 @@ def main():
         "config": {"workload": "heston_asian_arithmetic_call", "options": N_OPT, "paths": N_PATHS,
                    "steps": N_STEPS, "rng": "philox4x32-10+box-muller", "parallelism": "single-gpu"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "traffic": _traffic(),
+                     "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "heston_paths_kernel<float> (+finalize)", "kernel_ms": kern_ms,
-                     "flop_per_path_step": FLOP_PER_PATH_STEP},
+                     "flop_per_path_step": FLOP_PER_PATH_STEP,
+                     # issue ceiling: one wave64 VALU instruction per SIMD per 4 cycles
+                     "valu_issue_frac": (valu / (kern_ms * 1e-3) / VALU_ISSUE_PEAK) if valu else None},
     }
     if not args.no_secondary:
         ms64, k64, p64, c64 = run(mc.mode(mc.FP64, seed=1), 3, 1)

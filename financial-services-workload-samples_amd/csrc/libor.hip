@@ -204,12 +204,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   const float delta = 0.25f, sqrt_delta = 0.5f;
   const float lam = (float)a.lambda;
   const float con1 = delta * lam;
-  const float ce = con1 * O::kE;   // exponent scale of v (log2 domain)
-  const float kk = ce * con1;      // exponent scale of a rate's own term
+  const float kk = con1 * con1 * O::kE;  // log2 exponent per unit of sum L / (1 + delta L)
   const float a2 = (float)((double)kk * 0.69314718055994531 * (double)kk * 0.69314718055994531);
   const float L0 = (float)a.L0;
 
   // ---- forward: path_calc_b1 (no stores) ---------------------------------------
+  // vs = v / con1 = sum_k L_k / (1 + delta L_k), so the log2 exponent of the
+  // rate ratio is kk vs + c2 with kk = con1^2 log2(e).
   float L[NN];
 #pragma unroll
   for (int i = 0; i < NN; ++i) L[i] = L0;
@@ -217,13 +218,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   for (int n = 0; n < NMAT; ++n) {
     const float sqez = sqrt_delta * normal_at<float, kBuffer>(a, p, n, zq);
     const float c2 = lam * (sqez - 0.5f * con1) * O::kE;
-    float v = 0.f;
+    float vs = 0.f;
 #pragma unroll
     for (int i = 1; i < NN; ++i) {
       if (i > n) {
         const float Li = L[i];
-        v += con1 * O::div(Li, 1.f + delta * Li);
-        L[i] = Li * O::ex(ce * v + c2);
+        vs = __builtin_fmaf(Li, __builtin_amdgcn_rcpf(__builtin_fmaf(delta, Li, 1.f)), vs);
+        L[i] = Li * O::ex(__builtin_fmaf(kk, vs, c2));
       }
     }
   }
@@ -232,6 +233,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   const float val = portfolio_adjoint<float, kInplace>(L, Lb);
 
   // ---- adjoint: path_calc_b2, rows rebuilt in registers ------------------------
+  // With S = sum_{k>n} L(n+1,k) Lb(k) and P its prefix below i, the
+  // reference's v1 lambda faci^2 is con1^2 (S - P) / (1 + delta L(n,i))^2.
+  const float con1sq = con1 * con1;
   for (int n = NMAT - 1; n >= 0; --n) {
     float z;
     if (kBuffer) {
@@ -248,23 +252,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     float S = 0.f;
 #pragma unroll
     for (int i = 1; i < NN; ++i)
-      if (i > n) S += lam * L[i] * Lb[i];
-    float v = 0.f, P0 = 0.f;
+      if (i > n) S = __builtin_fmaf(L[i], Lb[i], S);
+    float vs = 0.f, P0 = 0.f;
 #pragma unroll
     for (int i = 1; i < NN; ++i) {
       if (i > n) {
         const float y = L[i];  // L(n+1, i)
-        const float w = y * O::ex(-(ce * v + c2));
-        const float r0 = __builtin_amdgcn_rcpf(1.f + delta * w);
+        const float w = y * O::ex(-__builtin_fmaf(kk, vs, c2));
+        const float r0 = __builtin_amdgcn_rcpf(__builtin_fmaf(delta, w, 1.f));
         const float q = w * r0;
         const float x1 = w * O::ex(-kk * q);
-        const float x = x1 + x1 * (a2 * (q * q) * r0);
-        const float r = __builtin_amdgcn_rcpf(1.f + delta * x);
-        v += con1 * (x * r);
-        const float faci = delta * r;
-        const float v1 = S - P0;
-        P0 += lam * y * Lb[i];
-        Lb[i] = Lb[i] * O::div(y, x) + v1 * lam * faci * faci;
+        const float x = __builtin_fmaf(x1, a2 * (q * q) * r0, x1);
+        const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(delta, x, 1.f));
+        vs = __builtin_fmaf(x, r, vs);
+        const float d = S - P0;
+        P0 = __builtin_fmaf(y, Lb[i], P0);
+        Lb[i] = __builtin_fmaf(d * (r * r), con1sq, Lb[i] * (y * __builtin_amdgcn_rcpf(x)));
         L[i] = x;
       }
     }
