@@ -92,7 +92,17 @@ def _cpu_baseline(np, P, target_s):
                       seed=5)
     el = time.perf_counter() - t0
     steps = N_OPT * paths * N_STEPS
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": steps / el, "unit": "path-steps/s", "cores": threads, "kind": "port",
+            "host": {"cpu": model, "logical_cpus": os.cpu_count(),
+                     "usable_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None},
             "sample": f"FP64 oracle (C, OpenMP over options), {N_OPT} options x {paths} paths x {N_STEPS} steps "
                       f"of the same workload, {el:.1f} s"}
 

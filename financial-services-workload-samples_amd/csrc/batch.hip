@@ -207,8 +207,15 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
     for (int pass = 0; pass < 2; ++pass) {  // GBM items, then Heston items
       const int kind = pass == 0 ? FSMC_REQ_GBM_EUROPEAN : FSMC_REQ_HESTON_ASIAN;
       const int begin = (int)items.size();
-      for (int i = r0; i < r0 + mb.n; ++i) {
-        if (b.req[i].kind != kind) continue;
+      // longest requests first (the dispatcher hands out workgroups in
+      // index order, so the longest items start first and the tail is short);
+      // a request's own items stay contiguous and in chunk order
+      std::vector<int> order;
+      for (int i = r0; i < r0 + mb.n; ++i)
+        if (b.req[i].kind == kind) order.push_back(i);
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int x, int y) { return b.req[x].n_steps > b.req[y].n_steps; });
+      for (int i : order) {
         first[i] = (int)items.size();
         nitems[i] = (b.req[i].n_paths + t.chunk_paths - 1) / t.chunk_paths;
         for (int c = 0; c < nitems[i]; ++c) items.push_back(make_int2(i, c));
