@@ -1,7 +1,9 @@
-// AsianOptionsPricing command line on the GPU engine, output-compatible with
-// the sample's MonteCarloTimeStep report (AsianOptionsPricing/MCAsianOptions.cpp:
-// :204-208, and the rate on stderr) so that its MPTest scripts (which grep
-// "Computation rate MC by time Step -") parse GPU runs unchanged.  No
+// AsianOptionsPricing command line on the GPU engine, printing the sample's
+// MonteCarloTimeStep report in its formats (AsianOptionsPricing/MCAsianOptions.cpp:
+// :204-208, and the rate on stderr).  Its MPTest getresults.sh greps the rate
+// line with three spaces after "-", which "%8.4f" produces only for rates
+// below 10 options/s; GPU rates need the pattern without them (awk's $8 is
+// still the rate).  No
 // arguments, as the sample: 100 options, 131072 paths, 128 steps, r = 0.03,
 // rho = -0.65 (:27-43), parameters drawn with the process's unseeded
 // std::rand() in the sample's order S, X, T, v0, kappa, theta, xi per option
