@@ -14,16 +14,15 @@ namespace {
 constexpr int kNmat = 40;   // normals per path, params.h:28
 constexpr int kLoops = 10;  // NLOOPS, liborSwaptionGreeks.cpp:27
 
-// Fills num doubles; paths are generated whole and the tail is dropped.
+// Fills num doubles with generateRand's stream: MRG32k3a seeded with seed,
+// inverse-CDF normals, positions 0 .. num-1 (mklRng.cpp:44-139).
 bool fill_normals(std::vector<double>& out, int num, double mean, double stddev, int seed) {
   fsmc_ctx* ctx = fsmc_dropin::context();
   if (!ctx || num <= 0) return false;
-  const int paths = (num + kNmat - 1) / kNmat;
-  out.resize((size_t)paths * kNmat);
-  fsmc_mode m = {FSMC_FP64, FSMC_RNG_PHILOX, (uint64_t)(uint32_t)seed, 0u, 0u};
-  const int st = fsmc_libor_normals(ctx, paths, &m, mean, stddev, out.data());
+  out.resize((size_t)num);
+  const int st = fsmc_mrg32k3a_gaussian(ctx, (uint32_t)seed, 0, num, mean, stddev, out.data());
   if (st != FSMC_OK) {
-    fsmc_dropin::report("fsmc_libor_normals", st);
+    fsmc_dropin::report("fsmc_mrg32k3a_gaussian", st);
     return false;
   }
   return true;

@@ -121,6 +121,11 @@ def lib():
         L.fsmc_libor_workspace.argtypes = [PL, PM]
         L.fsmc_libor_swaption_greeks_async.argtypes = [PL, PM, _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]
         L.fsmc_libor_normals.argtypes = [_VP, ctypes.c_int, PM, ctypes.c_double, ctypes.c_double, _D]
+        L.fsmc_mrg32k3a_uniform.argtypes = [_VP, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_longlong, _D]
+        L.fsmc_mrg32k3a_gaussian.argtypes = [_VP, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_longlong,
+                                             ctypes.c_double, ctypes.c_double, _D]
+        L.fsmc_mrg32k3a_gaussian_async.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_longlong,
+                                                   ctypes.c_double, ctypes.c_double, _VP, _VP]
         _I = ctypes.POINTER(ctypes.c_int)
         L.fsmc_lsmc_plan_create.argtypes = [_VP, ctypes.c_int, PM] + [_D] * 5 + [_I, _I, ctypes.POINTER(_VP)]
         L.fsmc_lsmc_plan_workspace.restype = ctypes.c_size_t
@@ -259,6 +264,20 @@ class Engine:
         _check(lib().fsmc_american_lsmc_ref(self._ctx, ctypes.byref(p), ctypes.byref(m), pz, ctypes.byref(am),
                                             ctypes.byref(eu)), "fsmc_american_lsmc_ref")
         return am.value, eu.value
+
+    def mrg32k3a_uniform(self, seed, offset, n):
+        """Positions offset .. offset+n-1 of the MKL-compatible MRG32k3a stream."""
+        u = np.zeros(n)
+        _check(lib().fsmc_mrg32k3a_uniform(self._ctx, seed, offset, n, u.ctypes.data_as(_D)),
+               "fsmc_mrg32k3a_uniform")
+        return u
+
+    def mrg32k3a_gaussian(self, seed, offset, n, mean=0.0, stddev=1.0):
+        """generateRand's normals (MRG32k3a + inverse CDF), stream order."""
+        z = np.zeros(n)
+        _check(lib().fsmc_mrg32k3a_gaussian(self._ctx, seed, offset, n, mean, stddev, z.ctypes.data_as(_D)),
+               "fsmc_mrg32k3a_gaussian")
+        return z
 
     def libor_normals(self, n_paths, m, mean=0.0, stddev=1.0):
         z, pz = _d(np.zeros(n_paths * 40))

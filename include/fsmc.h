@@ -149,19 +149,35 @@ typedef struct fsmc_libor_params {
 int fsmc_libor_swaption_greeks(fsmc_ctx* ctx, const fsmc_libor_params* p, const fsmc_mode* m, const double* z,
                                double* value, double* deltas);
 size_t fsmc_libor_workspace(const fsmc_libor_params* p, const fsmc_mode* m);
-/* Fills z[n_paths][40] with mean + stddev * (the normals Philox mode uses),
- * the device-side replacement of generateRand, mklRng.h:9 / mklRng.cpp:44. */
+/* Fills z[n_paths][40] with mean + stddev * (the normals Philox mode uses):
+ * a buffer-mode run on this z equals the Philox-mode run. */
 int fsmc_libor_normals(fsmc_ctx* ctx, int n_paths, const fsmc_mode* m, double mean, double stddev, double* z);
 int fsmc_libor_swaption_greeks_async(const fsmc_libor_params* p, const fsmc_mode* m, const double* d_z,
                                      double* d_value, double* d_deltas, void* d_workspace,
                                      size_t workspace_bytes, void* stream);
 
+/* ---- MKL-compatible MRG32k3a stream -----------------------------------------
+ * The stream generateRand draws (LiborMarketModel/mklRng.h:9,
+ * mklRng.cpp:44-139: vslNewStream(VSL_BRNG_MRG32K3A, seed), per-thread
+ * vslSkipAheadStream, vdRngGaussian/vsRngGaussian with
+ * VSL_RNG_METHOD_GAUSSIAN_ICDF), generated on the GPU.  Position k of the
+ * stream is written to out[k - offset] for offset <= k < offset + n.
+ * Uniforms equal MKL's bit for bit: ((x_k - y_k) mod m1) * (1 / m1).  Normals
+ * are mean + stddev * icdf(u_k) with an FP64 inverse normal CDF accurate to
+ * ~1e-16 relative; MKL's own ICDF differs from the exact inverse by up to
+ * ~2.4e-9 relative, so the normals match MKL's to that. */
+int fsmc_mrg32k3a_uniform(fsmc_ctx* ctx, uint32_t seed, uint64_t offset, long long n, double* u);
+int fsmc_mrg32k3a_gaussian(fsmc_ctx* ctx, uint32_t seed, uint64_t offset, long long n, double mean,
+                           double stddev, double* z);
+int fsmc_mrg32k3a_gaussian_async(uint32_t seed, uint64_t offset, long long n, double mean, double stddev,
+                                 double* d_z, void* stream);
+
 /* ---- Longstaff-Schwartz American puts -------------------------------------
  * Batched LSMC for many options with uneven step and path counts: one
  * exercise date per time step, cash flows discounted at r, regression of the
  * discounted cash flow on {1, x, x^2, x^3} of the in-the-money paths with x an
- * affine rescaling of S/K (Gram matrix on FP32 MFMA in FP32 mode, solved per
- * date by FP64 Cholesky), Philox normals.  It replaces the sweep inlined in
+ * affine rescaling of S/K (moment sums per lane in FP64, solved per date by
+ * FP64 Cholesky), Philox normals.  It replaces the sweep inlined in
  * MonteCarloAmericanOptions/main.cpp:131-279 (with perform_sims,
  * utilities.cpp:42-88) for the configurations that sample cannot express
  * (rates, many options).  price[n][2] = {price, standard error}; european

@@ -704,3 +704,125 @@ void orc_lsmc(int n_opt, const double* S0, const double* K, const double* sigma,
     lsmc_one(o, S0[o], K[o], sigma[o], r[o], T[o], steps[o], paths[o], seed, stream, price2 + 2 * o,
              european ? european + o : NULL);
 }
+
+/* ==========================================================================
+ * MKL-compatible MRG32k3a + inverse-CDF normals (mklRng.cpp:22-32, 44-139;
+ * mklRng.h:9).  Recurrences of L'Ecuyer, "Good parameters and implementations
+ * for combined multiple recursive random number generators", Oper. Res. 47
+ * (1999):  x_n = (1403580 x_{n-2} - 810728 x_{n-3}) mod m1,
+ *          y_n = (527612 y_{n-1} - 1370589 y_{n-3}) mod m2.
+ * Skip-ahead by matrix powers of the one-step transitions.
+ * ========================================================================== */
+#define ORC_M1 4294967087ULL
+#define ORC_M2 4294944443ULL
+
+static uint64_t mulmod(uint64_t a, uint64_t b, uint64_t m) {
+  return (uint64_t)(((unsigned __int128)a * b) % m);
+}
+
+/* C = A B (3x3, mod m) */
+static void mat3_mul(const uint64_t A[9], const uint64_t B[9], uint64_t m, uint64_t C[9]) {
+  uint64_t T[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      uint64_t acc = 0;
+      for (int k = 0; k < 3; ++k) acc = (acc + mulmod(A[3 * i + k], B[3 * k + j], m)) % m;
+      T[3 * i + j] = acc;
+    }
+  memcpy(C, T, sizeof(T));
+}
+
+static void mat3_vec(const uint64_t A[9], const uint64_t v[3], uint64_t m, uint64_t out[3]) {
+  uint64_t t[3];
+  for (int i = 0; i < 3; ++i) {
+    uint64_t acc = 0;
+    for (int k = 0; k < 3; ++k) acc = (acc + mulmod(A[3 * i + k], v[k], m)) % m;
+    t[i] = acc;
+  }
+  memcpy(out, t, sizeof(t));
+}
+
+/* state (s[0], s[1], s[2]) = (x_{n-3}, x_{n-2}, x_{n-1}) advanced by k steps */
+static void mrg_skip(uint64_t x[3], uint64_t y[3], uint64_t k) {
+  uint64_t A1[9] = {0, 1, 0, 0, 0, 1, ORC_M1 - 810728ULL, 1403580ULL, 0};
+  uint64_t A2[9] = {0, 1, 0, 0, 0, 1, ORC_M2 - 1370589ULL, 0, 527612ULL};
+  while (k) {
+    if (k & 1) {
+      mat3_vec(A1, x, ORC_M1, x);
+      mat3_vec(A2, y, ORC_M2, y);
+    }
+    mat3_mul(A1, A1, ORC_M1, A1);
+    mat3_mul(A2, A2, ORC_M2, A2);
+    k >>= 1;
+  }
+}
+
+void orc_mrg32k3a_uniform(uint32_t seed, uint64_t offset, int64_t n, double* u) {
+  uint64_t x[3] = {(uint64_t)seed % ORC_M1, 1, 1}, y[3] = {1, 1, 1};
+  mrg_skip(x, y, offset);
+  const double inv_m1 = 1.0 / 4294967087.0;
+  for (int64_t i = 0; i < n; ++i) {
+    /* -a x mod m = a (m - x): every product stays below 2^53 */
+    const uint64_t xn = (1403580ULL * x[1] + 810728ULL * (ORC_M1 - x[0])) % ORC_M1;
+    const uint64_t yn = (527612ULL * y[2] + 1370589ULL * (ORC_M2 - y[0])) % ORC_M2;
+    x[0] = x[1];
+    x[1] = x[2];
+    x[2] = xn;
+    y[0] = y[1];
+    y[1] = y[2];
+    y[2] = yn;
+    const uint64_t z = (xn + ORC_M1 - yn % ORC_M1) % ORC_M1;
+    /* z = 0 (probability 2^-32) would give u = 0; it is taken half a step up */
+    u[i] = z ? (double)z * inv_m1 : 0.5 * inv_m1;
+  }
+}
+
+/* Wichura, "Algorithm AS 241: The percentage points of the normal
+ * distribution", Appl. Statist. 37 (1988) 477-484, PPND16. */
+double orc_icdf(double p) {
+  const double q = p - 0.5;
+  if (fabs(q) <= 0.425) {
+    const double r = 0.180625 - q * q;
+    const double num = ((((((2.5090809287301226727e+3 * r + 3.3430575583588128105e+4) * r +
+                            6.7265770927008700853e+4) * r + 4.5921953931549871457e+4) * r +
+                          1.3731693765509461125e+4) * r + 1.9715909503065514427e+3) * r +
+                        1.3314166789178437745e+2) * r + 3.3871328727963666080e+0;
+    const double den = ((((((5.2264952788528545610e+3 * r + 2.8729085735721942674e+4) * r +
+                            3.9307895800092710610e+4) * r + 2.1213794301586595867e+4) * r +
+                          5.3941960214247511077e+3) * r + 6.8718700749205790830e+2) * r +
+                        4.2313330701600911252e+1) * r + 1.0;
+    return q * num / den;
+  }
+  double r = q < 0.0 ? p : 1.0 - p;
+  r = sqrt(-log(r));
+  double x;
+  if (r <= 5.0) {
+    r -= 1.6;
+    const double num = ((((((7.74545014278341407640e-4 * r + 2.27238449892691845833e-2) * r +
+                            2.41780725177450611770e-1) * r + 1.27045825245236838258e+0) * r +
+                          3.64784832476320460504e+0) * r + 5.76949722146069140550e+0) * r +
+                        4.63033784615654529590e+0) * r + 1.42343711074968357734e+0;
+    const double den = ((((((1.05075007164441684324e-9 * r + 5.47593808499534494600e-4) * r +
+                            1.51986665636164571966e-2) * r + 1.48103976427480074590e-1) * r +
+                          6.89767334985100004550e-1) * r + 1.67638483018380384940e+0) * r +
+                        2.05319162663775882187e+0) * r + 1.0;
+    x = num / den;
+  } else {
+    r -= 5.0;
+    const double num = ((((((2.01033439929228813265e-7 * r + 2.71155556874348757815e-5) * r +
+                            1.24266094738807843860e-3) * r + 2.65321895265761230930e-2) * r +
+                          2.96560571828504891230e-1) * r + 1.78482653991729133580e+0) * r +
+                        5.46378491116411436990e+0) * r + 6.65790464350110377720e+0;
+    const double den = ((((((2.04426310338993978564e-15 * r + 1.42151175831644588870e-7) * r +
+                            1.84631831751005468180e-5) * r + 7.86869131145613259100e-4) * r +
+                          1.48753612908506148525e-2) * r + 1.36929880922735805310e-1) * r +
+                        5.99832206555887937690e-1) * r + 1.0;
+    x = num / den;
+  }
+  return q < 0.0 ? -x : x;
+}
+
+void orc_mrg32k3a_gaussian(uint32_t seed, uint64_t offset, int64_t n, double mean, double stddev, double* z) {
+  orc_mrg32k3a_uniform(seed, offset, n, z);
+  for (int64_t i = 0; i < n; ++i) z[i] = mean + stddev * orc_icdf(z[i]);
+}

@@ -81,6 +81,20 @@ void orc_lsmc(int n_opt, const double* S0, const double* K, const double* sigma,
  * non-positive pivot. */
 int orc_chol4_solve(const double* G, const double* rhs, double* beta);
 
+/* ---- MKL-compatible MRG32k3a stream (the generateRand seam) ---------------
+ * L'Ecuyer's MRG32k3a as VSL_BRNG_MRG32K3A seeds it from one 32-bit seed
+ * (x = (seed mod m1, 1, 1), y = (1, 1, 1)); uniform i is
+ * ((x_i - y_i) mod m1) * (1/m1).  u[k] = uniform offset + k, k < n: the
+ * stream order in which generateRand (LiborMarketModel/mklRng.cpp:44-139)
+ * fills z, whatever the thread count (each thread skips ahead to its
+ * start).  z[k] = mean + stddev * icdf(u[k]) with icdf = Wichura's AS241
+ * (PPND16), the inverse normal CDF to ~1e-16 relative (MKL's own ICDF is not
+ * reproducible bit for bit and differs from the exact inverse by up to a few
+ * 1e-9). */
+void orc_mrg32k3a_uniform(uint32_t seed, uint64_t offset, int64_t n, double* u);
+double orc_icdf(double u);
+void orc_mrg32k3a_gaussian(uint32_t seed, uint64_t offset, int64_t n, double mean, double stddev, double* z);
+
 #ifdef __cplusplus
 }
 #endif

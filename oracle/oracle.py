@@ -62,6 +62,11 @@ def lib():
         L.orc_lsmc.argtypes = [ctypes.c_int] + [_D] * 5 + [_I, _I, ctypes.c_uint64, ctypes.c_uint32, _D, _D]
         L.orc_chol4_solve.restype = ctypes.c_int
         L.orc_chol4_solve.argtypes = [_D, _D, _D]
+        L.orc_mrg32k3a_uniform.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64, _D]
+        L.orc_icdf.restype = ctypes.c_double
+        L.orc_icdf.argtypes = [ctypes.c_double]
+        L.orc_mrg32k3a_gaussian.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64, ctypes.c_double,
+                                            ctypes.c_double, _D]
         _lib = L
     return _lib
 
@@ -238,3 +243,21 @@ def chol4_solve(G, rhs):
     beta, pb = _d(np.zeros(4))
     ok = lib().orc_chol4_solve(pG, pr, pb)
     return beta if ok else None
+
+
+def mrg32k3a_uniform(seed, offset, n):
+    """Uniforms offset .. offset+n-1 of the MKL-compatible MRG32k3a stream."""
+    u, pu = _d(np.zeros(n))
+    lib().orc_mrg32k3a_uniform(seed, offset, n, pu)
+    return u
+
+
+def icdf(u):
+    return lib().orc_icdf(float(u))
+
+
+def mrg32k3a_gaussian(seed, offset, n, mean=0.0, stddev=1.0):
+    """generateRand's normals (MRG32k3a + inverse CDF), stream order."""
+    z, pz = _d(np.zeros(n))
+    lib().orc_mrg32k3a_gaussian(seed, offset, n, mean, stddev, pz)
+    return z

@@ -46,6 +46,18 @@ def test_philox_fp64_matches_oracle(engine, n):
     assert _rel(d, od) <= 1e-12
 
 
+def test_mrg32k3a_stream_matches_oracle(engine):
+    # generateRand's stream on the GPU: uniforms bit for bit, normals to the
+    # last bits of the FP64 log/sqrt in the inverse CDF
+    for seed, off, n in ((12345, 0, 5000), (7, 123457, 3001), (4000000000, 2**40 + 5, 2048)):
+        u = engine.mrg32k3a_uniform(seed, off, n)
+        assert np.array_equal(u, orc.mrg32k3a_uniform(seed, off, n))
+        z = engine.mrg32k3a_gaussian(seed, off, n, 0.5, 2.0)
+        oz = orc.mrg32k3a_gaussian(seed, off, n, 0.5, 2.0)
+        assert np.max(np.abs(z - oz) / np.maximum(np.abs(oz), 1.0)) < 1e-14
+    assert engine.mrg32k3a_gaussian(12345, 0, 0).size == 0
+
+
 def test_normals_seam_equals_philox(engine):
     # generateRand replacement: buffer mode on the filled z == Philox mode
     n = 512
@@ -100,7 +112,8 @@ def test_errors(engine):
 def test_sample_command_line(build, rel):
     # bin/libor_{dp,sp}: the sample's main over the drop-in liborSwaptionGreeks;
     # its printed value/lambda are 10x the per-path means (NLOOPS) on the
-    # generateRand (seed 12345) normals, L0 = 0.051, lambda = 0.2.
+    # generateRand normals (MRG32k3a seed 12345, inverse CDF, stream order),
+    # L0 = 0.051, lambda = 0.2.
     import os
     import re
     import subprocess
@@ -109,6 +122,7 @@ def test_sample_command_line(build, rel):
     assert r.returncode == 0, r.stderr
     m = re.search(r"Result: value=([-0-9.eE+]+),\s*lambda=([-0-9.eE+]+)", r.stdout)
     assert m and re.search(r"Total time: [0-9.eE+]+ ms", r.stdout), r.stdout
-    ov, _, od, _ = orc.libor(2048, L0=0.051, seed=12345)
+    z = orc.mrg32k3a_gaussian(12345, 0, 2048 * 40).reshape(2048, 40)
+    ov, _, od, _ = orc.libor(2048, L0=0.051, z=z)
     assert float(m.group(1)) == pytest.approx(10 * ov, rel=rel)
     assert float(m.group(2)) == pytest.approx(10 * od[79], rel=rel)

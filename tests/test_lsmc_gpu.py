@@ -2,7 +2,7 @@
 
 Reference algorithm (MonteCarloAmericanOptions) in buffer mode matches the
 oracle's restatement to <= 1e-12 relative in FP64.  The batched LSMC matches
-the oracle's FP64 LSMC to <= 1e-12 on the same Philox stream; its FP32/MFMA
+the oracle's FP64 LSMC to <= 1e-12 on the same Philox stream; its FP32
 build is checked against the FP64 GPU run and against a Bermudan CRR lattice
 with the same exercise dates (LSMC is biased low by its regression)."""
 import math
@@ -85,12 +85,12 @@ def test_batched_longstaff_schwartz_table(engine):
 
 
 def test_config3_against_bermudan_lattice(engine):
-    # Config 3: 256 puts, steps ~U{50..250}, paths ~U{16384..262144}, FP32 MFMA.
+    # Config 3: 256 puts, steps ~U{50..250}, paths ~U{16384..262144}, FP32.
     P = _opts(256, 3)
     p, se, eu = _call(engine, P, mc.mode(mc.FP32, seed=3))
     p64, se64, eu64 = _call(engine, P, mc.mode(mc.FP64, seed=3))
     assert np.all(np.isfinite(p)) and np.all(se > 0)
-    # FP32 (MFMA Gram, FP32 paths) against FP64 on the same stream: rounding
+    # FP32 paths (FP64 moment sums) against FP64 on the same stream: rounding
     # moves a few exercise decisions at the regression boundary; measured
     # median 0.05 SE, worst of 256 options 0.71 SE
     d = np.abs(p - p64) / se64

@@ -379,3 +379,52 @@ def test_crr_bermudan_put():
     b10 = orc.crr_bermudan_put(36, 40, 1, .06, .2, 10, 400)
     b50 = orc.crr_bermudan_put(36, 40, 1, .06, .2, 50, 80)
     assert eu < b10 < b50 < orc.crr_american_put(36, 40, 1, .06, .2, 4000) + 1e-3
+
+
+# ---- MKL-compatible MRG32k3a + inverse CDF (the generateRand seam) -----------
+# Printed by MKL 2021.4 for vslNewStream(VSL_BRNG_MRG32K3A, 12345): the first
+# uniforms (vdRngUniform) and the first normals of vdRngGaussian with
+# VSL_RNG_METHOD_GAUSSIAN_ICDF, i.e. z[0..5] of LiborMarketModel's buffer.
+MKL_MRG_U = [0.67025736022828808, 0.55588071611222578, 0.21607400713476063]
+MKL_MRG_Z = [0.44062392580367954, 0.14053339627998879, -0.78552125308716181,
+             2.1380353354757777, -0.77725596281159304, 0.090591905930727717]
+
+
+def test_mrg32k3a_reproduces_mkl_uniforms_and_normals():
+    u = orc.mrg32k3a_uniform(12345, 0, 3)
+    assert list(u) == MKL_MRG_U  # bit for bit
+    z = orc.mrg32k3a_gaussian(12345, 0, 6)
+    # MKL's ICDF itself is off the exact inverse by up to ~2.4e-9 relative
+    assert np.max(np.abs(z - MKL_MRG_Z) / np.abs(MKL_MRG_Z)) < 3e-9
+    # vslSkipAheadStream(4) continues with normals 4 and 5
+    assert np.array_equal(orc.mrg32k3a_gaussian(12345, 4, 2), z[4:6])
+
+
+def _mrg_bigint(seed, n):
+    m1, m2 = 4294967087, 4294944443
+    x, y, out = [seed % m1, 1, 1], [1, 1, 1], []
+    for _ in range(n):
+        xn = (1403580 * x[1] - 810728 * x[0]) % m1
+        yn = (527612 * y[2] - 1370589 * y[0]) % m2
+        x, y = [x[1], x[2], xn], [y[1], y[2], yn]
+        z = (xn - yn) % m1
+        out.append(z * (1.0 / m1) if z else 0.5 / m1)
+    return np.array(out)
+
+
+def test_mrg32k3a_recurrence_and_skip_ahead():
+    ref = _mrg_bigint(987654321, 60000)  # exact integer arithmetic
+    assert np.array_equal(orc.mrg32k3a_uniform(987654321, 0, 60000), ref)
+    for off in (1, 3, 1000, 41237, 59000):  # matrix-power skip == stepping
+        assert np.array_equal(orc.mrg32k3a_uniform(987654321, off, 1000), ref[off:off + 1000])
+
+
+def test_icdf_matches_exact_inverse():
+    from scipy.special import ndtri
+    u = np.concatenate([np.logspace(-300, -1, 300), np.linspace(0.01, 0.99, 999), 1 - np.logspace(-16, -2, 100)])
+    x = np.array([orc.icdf(v) for v in u])
+    assert np.max(np.abs(x - ndtri(u)) / np.maximum(np.abs(ndtri(u)), 1e-300)) < 2e-15
+    # the three branches meet continuously; odd symmetry about 1/2
+    for v in (0.075, 0.5 - 0.425, np.exp(-25.0)):
+        assert orc.icdf(v) == pytest.approx(ndtri(v), rel=1e-15)
+    assert orc.icdf(0.3) == -orc.icdf(0.7) or abs(orc.icdf(0.3) + orc.icdf(0.7)) < 1e-15
