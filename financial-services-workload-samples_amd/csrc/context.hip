@@ -38,11 +38,12 @@ void DeviceBuffer::release() {
   bytes = 0;
 }
 
-int validate_mode(const fsmc_mode* m, bool buffer_allowed) {
+int validate_mode(const fsmc_mode* m, bool buffer_allowed, bool mrg_allowed) {
   FSMC_REQUIRE(m != nullptr, "mode is NULL");
   FSMC_REQUIRE(m->precision == FSMC_FP32 || m->precision == FSMC_FP64, "precision %d is neither FP32 nor FP64",
                m->precision);
-  FSMC_REQUIRE(m->rng == FSMC_RNG_PHILOX || (buffer_allowed && m->rng == FSMC_RNG_BUFFER),
+  FSMC_REQUIRE(m->rng == FSMC_RNG_PHILOX || (buffer_allowed && m->rng == FSMC_RNG_BUFFER) ||
+                   (mrg_allowed && m->rng == FSMC_RNG_MRG32K3A),
                "rng mode %d not supported here", m->rng);
   return FSMC_OK;
 }

@@ -52,7 +52,11 @@ struct DeviceBuffer {
   void release();
 };
 
-int validate_mode(const fsmc_mode* m, bool buffer_allowed);
+int validate_mode(const fsmc_mode* m, bool buffer_allowed, bool mrg_allowed = false);
+// Enqueues positions offset .. offset+n-1 of the MKL-compatible MRG32k3a
+// normal stream (mean + stddev * icdf(u)) into d_out on stream s.
+int mrg_gaussian_enqueue(uint32_t seed, uint64_t offset, long long n, double mean, double stddev, double* d_out,
+                         hipStream_t s);
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 

@@ -347,9 +347,9 @@ LiborPlan libor_plan(const fsmc_libor_params* p, const fsmc_mode* m) {
   return l;
 }
 
-int libor_validate(const fsmc_libor_params* p, const fsmc_mode* m) {
+int libor_validate(const fsmc_libor_params* p, const fsmc_mode* m, bool mrg_allowed = false) {
   FSMC_REQUIRE(p != nullptr, "params is NULL");
-  FSMC_TRY(validate_mode(m, true));
+  FSMC_TRY(validate_mode(m, true, mrg_allowed));
   FSMC_REQUIRE(p->n_paths >= 2, "n_paths %d < 2", p->n_paths);
   FSMC_REQUIRE(p->L0 > 0.0, "L0 must be > 0");
   FSMC_REQUIRE(p->lambda >= 0.0, "lambda must be >= 0");
@@ -453,18 +453,27 @@ int fsmc_libor_normals(fsmc_ctx* ctx, int n_paths, const fsmc_mode* m, double me
 int fsmc_libor_swaption_greeks(fsmc_ctx* ctx, const fsmc_libor_params* p, const fsmc_mode* m, const double* z,
                                double* value, double* deltas) {
   FSMC_REQUIRE(ctx != nullptr, "ctx is NULL");
-  FSMC_TRY(libor_validate(p, m));
+  FSMC_TRY(libor_validate(p, m, true));
   FSMC_REQUIRE(value && deltas, "NULL host pointer");
-  const bool buffer = m->rng == FSMC_RNG_BUFFER;
-  FSMC_REQUIRE(!buffer || z, "buffer mode needs z");
+  const bool mrg = m->rng == FSMC_RNG_MRG32K3A;
+  const bool buffer = m->rng == FSMC_RNG_BUFFER || mrg;
+  FSMC_REQUIRE(m->rng != FSMC_RNG_BUFFER || z, "buffer mode needs z");
   std::lock_guard<std::mutex> lock(ctx->mu);
   const size_t zn = buffer ? (size_t)p->n_paths * NMAT : 0;
   FSMC_TRY(ctx_prepare(ctx, 0, zn * sizeof(double), (2 + 2 * NN) * sizeof(double), libor_plan(p, m).ws));
   hipStream_t s = ctx->stream;
-  if (buffer) FSMC_HIP(hipMemcpyAsync(ctx->normals.ptr, z, zn * sizeof(double), hipMemcpyHostToDevice, s));
+  fsmc_mode mb = *m;  // the kernels read the generated stream as a buffer
+  if (mrg) {
+    // generateRand's layout z[40 path + step] is the stream order
+    FSMC_TRY(mrg_gaussian_enqueue((uint32_t)m->seed, (uint64_t)m->stream * zn, (long long)zn, 0.0, 1.0,
+                                  (double*)ctx->normals.ptr, s));
+    mb.rng = FSMC_RNG_BUFFER;
+  } else if (buffer) {
+    FSMC_HIP(hipMemcpyAsync(ctx->normals.ptr, z, zn * sizeof(double), hipMemcpyHostToDevice, s));
+  }
   double* dv = (double*)ctx->outputs.ptr;
   double* dd = dv + 2;
-  int st = fsmc_libor_swaption_greeks_async(p, m, (const double*)ctx->normals.ptr, dv, dd, ctx->workspace.ptr,
+  int st = fsmc_libor_swaption_greeks_async(p, &mb, (const double*)ctx->normals.ptr, dv, dd, ctx->workspace.ptr,
                                             ctx->workspace.bytes, s);
   if (st != FSMC_OK) {
     (void)hipStreamSynchronize(s);

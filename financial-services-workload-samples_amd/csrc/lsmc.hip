@@ -314,7 +314,8 @@ struct RefArgs {
   int n_paths, n_steps;
   double spot, strike;
   double fwd_l2e, vsdt_l2e;  // main.cpp:55-59, log2 domain as perform_sims
-  const double* z;           // buffer mode: [n_steps][n_paths]
+  const double* z;           // buffer / MRG32k3a modes: step s-1 of path p at z[(s-1) * zrow + p]
+  long long zrow;            // n_paths (caller buffer) or n_paths / 1024 (the sample's overlapping streams)
   uint32_t k0, k1, w;
   void* a;                   // Real [n_steps + 1][n_paths]
   void* cf;                  // Real [n_paths]
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void ref_paths_kernel(RefArgs r) {
   for (int s = 1; s <= r.n_steps; ++s) {
     Real z;
     if (kBuffer) {
-      z = (Real)r.z[(size_t)(s - 1) * N + p];
+      z = (Real)r.z[(size_t)(s - 1) * (size_t)r.zrow + p];
     } else {
       const int j = s - 1;
       if ((j & 3) == 0)
@@ -660,10 +661,11 @@ int fsmc_american_lsmc(fsmc_ctx* ctx, int n_options, const fsmc_mode* m, const d
 int fsmc_american_lsmc_ref(fsmc_ctx* ctx, const fsmc_lsmc_ref_params* p, const fsmc_mode* m, const double* z,
                            double* american, double* european) {
   FSMC_REQUIRE(ctx && p && american && european, "NULL argument");
-  FSMC_TRY(validate_mode(m, true));
+  FSMC_TRY(validate_mode(m, true, true));
   FSMC_REQUIRE(p->n_paths >= 2 && p->n_steps >= 2, "n_paths %d / n_steps %d too small", p->n_paths, p->n_steps);
-  const bool buffer = m->rng == FSMC_RNG_BUFFER;
-  FSMC_REQUIRE(!buffer || z, "buffer mode needs z");
+  const bool mrg = m->rng == FSMC_RNG_MRG32K3A;
+  const bool buffer = m->rng == FSMC_RNG_BUFFER || mrg;  // both read z from the normals slot
+  FSMC_REQUIRE(m->rng != FSMC_RNG_BUFFER || z, "buffer mode needs z");
   std::lock_guard<std::mutex> lock(ctx->mu);
   const size_t N = (size_t)p->n_paths;
   const size_t real = m->precision == FSMC_FP64 ? 8 : 4;
@@ -676,10 +678,18 @@ int fsmc_american_lsmc_ref(fsmc_ctx* ctx, const fsmc_lsmc_ref_params* p, const f
   };
   const size_t o_a = take(real * (size_t)(p->n_steps + 1) * N), o_cf = take(real * N),
                o_part = take(sizeof(double) * 8 * nb), o_beta = take(sizeof(double) * 6), o_out = take(sizeof(double) * 2);
-  const size_t zn = buffer ? N * (size_t)p->n_steps : 0;
+  // MRG32k3a: stream s-1 is the seed stream skipped ahead by nb (s-1)
+  // (main.cpp:93-106) and draws n_paths per pricing (utilities.cpp:52-72), so
+  // step s of path p reads position nb (s-1) + p of one contiguous run
+  const long long nb_skip = (long long)(N / 1024);
+  const size_t zn = mrg ? N + (size_t)nb_skip * (size_t)(p->n_steps - 1) : buffer ? N * (size_t)p->n_steps : 0;
   FSMC_TRY(ctx_prepare(ctx, 0, zn * sizeof(double), 0, off));
   hipStream_t s = ctx->stream;
-  if (buffer) FSMC_HIP(hipMemcpyAsync(ctx->normals.ptr, z, zn * sizeof(double), hipMemcpyHostToDevice, s));
+  if (mrg)
+    FSMC_TRY(mrg_gaussian_enqueue((uint32_t)m->seed, (uint64_t)m->stream * N, (long long)zn, 0.0, 1.0,
+                                  (double*)ctx->normals.ptr, s));
+  else if (buffer)
+    FSMC_HIP(hipMemcpyAsync(ctx->normals.ptr, z, zn * sizeof(double), hipMemcpyHostToDevice, s));
   char* ws = (char*)ctx->workspace.ptr;
   RefArgs r;
   r.n_paths = p->n_paths;
@@ -691,6 +701,7 @@ int fsmc_american_lsmc_ref(fsmc_ctx* ctx, const fsmc_lsmc_ref_params* p, const f
   r.vsdt_l2e = p->vol * sqrt(dt) * kLog2e;
   r.fwd_l2e = p->vol * p->vol * dt * -0.5 * kLog2e;
   r.z = (const double*)ctx->normals.ptr;
+  r.zrow = mrg ? nb_skip : (long long)N;
   r.k0 = (uint32_t)m->seed;
   r.k1 = (uint32_t)(m->seed >> 32);
   r.w = counter_w(kTagLsmc, m->stream);

@@ -219,6 +219,13 @@ int mrg_sync(fsmc_ctx* ctx, bool normal, uint32_t seed, uint64_t offset, long lo
 }
 
 }  // namespace
+
+int mrg_gaussian_enqueue(uint32_t seed, uint64_t offset, long long n, double mean, double stddev, double* d_out,
+                         hipStream_t s) {
+  FSMC_TRY(mrg_validate(n, d_out));
+  return mrg_enqueue(true, mrg_args(seed, offset, n, mean, stddev, d_out), s);
+}
+
 }  // namespace fsmc
 
 using namespace fsmc;

@@ -5,9 +5,12 @@
 // (utilities.h:24-25), spot 100, strike 110, vol 0.25, maturity 2
 // (main.cpp:46-49).  Each loop runs the sample's algorithm
 // (fsmc_american_lsmc_ref: path matrix, lagged quadratic regression with ridge
-// 0.01, cofactor inverse) with the engine's Philox normals, on the GPU as one
-// stream-ordered sequence, so the three phase timers are not separated and
-// print 0.  50 timed loops after one untimed, as the sample's progLoops.
+// 0.01, cofactor inverse) on the sample's own normals: MRG32k3a seed 12345,
+// stream s-1 skipped ahead by 512 (s-1) and advancing by numPaths per loop, as
+// the single-thread run draws them (main.cpp:93-106, utilities.cpp:52-72),
+// generated on the GPU.  Everything runs as one stream-ordered sequence, so
+// the three phase timers are not separated and print 0.  50 timed loops after
+// one untimed, as the sample's progLoops.
 // -DUSE_DOUBLE selects the FP64 engine path, otherwise FP32.
 #include <chrono>
 #include <cstdio>
@@ -33,10 +36,10 @@ int main() {
     return EXIT_FAILURE;
   }
   fsmc_lsmc_ref_params p = {num_paths, num_steps, 100.0, 110.0, 0.25, 2.0};
-  fsmc_mode m = {kPrecision, FSMC_RNG_PHILOX, 12345, 0, 0};
+  fsmc_mode m = {kPrecision, FSMC_RNG_MRG32K3A, 12345, 0, 0};
   double american = 0.0, european = 0.0, compute_ms = 0.0;
   for (int l = 0; l <= loops; ++l) {
-    m.stream = (uint32_t)l;  // fresh paths per loop, as the sample's streams advance
+    m.stream = (uint32_t)l;  // loop l continues every stream by numPaths draws
     const auto t0 = std::chrono::steady_clock::now();
     if (fsmc_american_lsmc_ref(ctx, &p, &m, nullptr, &american, &european) != FSMC_OK) {
       std::cerr << "fsmc_american_lsmc_ref: " << fsmc_last_error() << std::endl;

@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libfsmc.so")
 
 FSMC_OK = 0
 FP32, FP64 = 0, 1
-RNG_PHILOX, RNG_BUFFER = 0, 1
+RNG_PHILOX, RNG_BUFFER, RNG_MRG32K3A = 0, 1, 2
 QUIRK_ASIAN_VAR_AS_VOL = 1 << 0
 QUIRK_ASIAN_RUNNING_VAL2 = 1 << 1
 QUIRK_LIBOR_INPLACE_ADJ = 1 << 2

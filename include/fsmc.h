@@ -50,6 +50,8 @@ extern "C" {
 
 #define FSMC_RNG_PHILOX 0 /* in-kernel Philox4x32-10 + Box-Muller          */
 #define FSMC_RNG_BUFFER 1 /* caller-supplied normals (reference layout)     */
+#define FSMC_RNG_MRG32K3A 2 /* the sample's own MKL stream, generated on the GPU
+                             * (LIBOR and the LSMC reference algorithm only) */
 
 /* Reference-compatibility switches ("refcompat"): each reproduces one
  * documented behaviour of the reference sample bit for bit. */
@@ -59,7 +61,7 @@ extern "C" {
 
 typedef struct fsmc_mode {
   int precision;   /* FSMC_FP32 | FSMC_FP64                               */
-  int rng;         /* FSMC_RNG_PHILOX | FSMC_RNG_BUFFER                   */
+  int rng;         /* FSMC_RNG_PHILOX | FSMC_RNG_BUFFER | FSMC_RNG_MRG32K3A */
   uint64_t seed;   /* Philox key                                          */
   uint32_t stream; /* Philox sub-stream id (28 bits), for independent runs */
   uint32_t quirks; /* FSMC_QUIRK_* bit mask                               */
@@ -146,6 +148,9 @@ typedef struct fsmc_libor_params {
   double lambda;
 } fsmc_libor_params;
 
+/* m->rng = FSMC_RNG_MRG32K3A (synchronous entry point only): z is not read;
+ * the normals are generateRand's (mklRng.cpp:44-139): MRG32k3a seeded with
+ * m->seed, stream positions k = m->stream * 40 n_paths + 40 path + step. */
 int fsmc_libor_swaption_greeks(fsmc_ctx* ctx, const fsmc_libor_params* p, const fsmc_mode* m, const double* z,
                                double* value, double* deltas);
 size_t fsmc_libor_workspace(const fsmc_libor_params* p, const fsmc_mode* m);
@@ -210,6 +215,12 @@ typedef struct fsmc_lsmc_ref_params {
   double spot, strike, vol, maturity;
 } fsmc_lsmc_ref_params;
 
+/* m->rng = FSMC_RNG_MRG32K3A: z is not read; step s of path p takes the
+ * sample's single-thread draw (main.cpp:93-106, utilities.cpp:52-72: stream
+ * s-1 of MRG32k3a seed m->seed skipped ahead by nb (s-1), nb = n_paths / 1024,
+ * and advanced by n_paths per pricing): stream position
+ * nb (s-1) + p + m->stream * n_paths, where m->stream counts previous
+ * pricings (the sample's loop index). */
 int fsmc_american_lsmc_ref(fsmc_ctx* ctx, const fsmc_lsmc_ref_params* p, const fsmc_mode* m, const double* z,
                            double* american, double* european);
 

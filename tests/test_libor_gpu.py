@@ -58,6 +58,23 @@ def test_mrg32k3a_stream_matches_oracle(engine):
     assert engine.mrg32k3a_gaussian(12345, 0, 0).size == 0
 
 
+def test_sample_stream_mode(engine):
+    # FSMC_RNG_MRG32K3A: generateRand's normals generated in place, equal to
+    # buffer mode on the same stream (stream id = block of 40 n normals)
+    n = 1500
+    for stream in (0, 2):
+        a = engine.libor(n, mc.mode(mc.FP64, mc.RNG_MRG32K3A, seed=12345, stream=stream), L0=0.051)
+        z = engine.mrg32k3a_gaussian(12345, stream * 40 * n, 40 * n).reshape(n, 40)
+        b = engine.libor(n, mc.mode(mc.FP64, mc.RNG_BUFFER), L0=0.051, z=z)
+        assert a[0] == b[0] and np.array_equal(a[2], b[2])
+    oz = orc.mrg32k3a_gaussian(12345, 0, 40 * n).reshape(n, 40)
+    ov, _, od, _ = orc.libor(n, L0=0.051, z=oz)
+    a = engine.libor(n, mc.mode(mc.FP64, mc.RNG_MRG32K3A, seed=12345), L0=0.051)
+    assert abs(a[0] - ov) <= 1e-12 * abs(ov) and _rel(a[2], od) <= 1e-12
+    with pytest.raises(mc.FsmcError):  # not for the Philox-only workloads
+        engine.gbm_european([10.0], [10.0], [1.0], .02, .3, 100, mc.mode(rng=mc.RNG_MRG32K3A))
+
+
 def test_normals_seam_equals_philox(engine):
     # generateRand replacement: buffer mode on the filled z == Philox mode
     n = 512

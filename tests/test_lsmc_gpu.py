@@ -136,3 +136,27 @@ def test_edge_cases(engine):
         _call(engine, dict(P, steps=[0, 30, 30]), mc.mode())
     with pytest.raises(mc.FsmcError):
         engine.american_lsmc_ref(100, 10, 100.0, 110.0, .25, 2.0, mc.mode(rng=mc.RNG_BUFFER))
+
+
+def _sample_stream_buffer(draw, n_paths, n_steps, loop):
+    # the sample's single-thread draw: step s of path p is stream position
+    # (n_paths // 1024) (s - 1) + p + loop n_paths
+    nb = n_paths // 1024
+    G = draw(12345, loop * n_paths, n_paths + nb * (n_steps - 1))
+    return np.stack([G[nb * j:nb * j + n_paths] for j in range(n_steps)])
+
+
+@pytest.mark.parametrize("loop", [0, 3])
+def test_reference_algorithm_on_sample_stream(engine, loop):
+    n_paths, n_steps = 8192, 64
+    m = mc.mode(mc.FP64, mc.RNG_MRG32K3A, seed=12345, stream=loop)
+    am, eu = engine.american_lsmc_ref(n_paths, n_steps, 100.0, 110.0, .25, 2.0, m)
+    # the same normals through buffer mode: identical results
+    z = _sample_stream_buffer(engine.mrg32k3a_gaussian, n_paths, n_steps, loop)
+    bam, beu = engine.american_lsmc_ref(n_paths, n_steps, 100.0, 110.0, .25, 2.0, mc.mode(mc.FP64, mc.RNG_BUFFER),
+                                        z)
+    assert am == bam and eu == beu
+    # and the oracle on its own MRG32k3a stream
+    oz = _sample_stream_buffer(orc.mrg32k3a_gaussian, n_paths, n_steps, loop)
+    oam, oeu = orc.lsmc_ref(n_paths, n_steps, 100.0, 110.0, .25, 2.0, oz)
+    assert abs(am - oam) <= 1e-12 * abs(oam) and abs(eu - oeu) <= 1e-12 * abs(oeu)
