@@ -53,7 +53,10 @@ def sources():
 def _cli(src, suffix, defines):
     name = os.path.basename(src)[len("cli_"):-len(".cpp")]
     out = os.path.join(BIN_DIR, f"{name}_{suffix}")
-    cmd = [GXX] + HOST_FLAGS + defines + [src, "-o", out, f"-L{LIB_DIR}", "-lfsmc", "-Wl,-rpath,$ORIGIN/../lib"]
+    # command lines that hold device buffers themselves call the HIP runtime
+    hip = ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-L/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,/opt/rocm/lib"] if "hip/hip_runtime.h" in open(src).read() else []
+    cmd = [GXX] + HOST_FLAGS + defines + [src, "-o", out, f"-L{LIB_DIR}", "-lfsmc", "-Wl,-rpath,$ORIGIN/../lib"] + hip
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"g++ failed for {src}:\n{r.stderr}")

@@ -24,6 +24,8 @@ RNG_PHILOX, RNG_BUFFER, RNG_MRG32K3A = 0, 1, 2
 QUIRK_ASIAN_VAR_AS_VOL = 1 << 0
 QUIRK_ASIAN_RUNNING_VAL2 = 1 << 1
 QUIRK_LIBOR_INPLACE_ADJ = 1 << 2
+QUIRK_BS_POLY_CND = 1 << 3
+LATTICE_AMERICAN, LATTICE_PUT = 1, 2
 
 _D = ctypes.POINTER(ctypes.c_double)
 _F = ctypes.POINTER(ctypes.c_float)
@@ -121,6 +123,9 @@ def lib():
         L.fsmc_libor_workspace.argtypes = [PL, PM]
         L.fsmc_libor_swaption_greeks_async.argtypes = [PL, PM, _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]
         L.fsmc_libor_normals.argtypes = [_VP, ctypes.c_int, PM, ctypes.c_double, ctypes.c_double, _D]
+        L.fsmc_black_scholes.argtypes = [_VP, ctypes.c_int, PM] + [_D] * 4 + [ctypes.c_double] + [_D] * 3
+        L.fsmc_lattice.argtypes = [_VP, ctypes.c_int, PM] + [_D] * 4 + [ctypes.c_double, ctypes.c_int,
+                                                                       ctypes.c_uint, _D]
         L.fsmc_mrg32k3a_uniform.argtypes = [_VP, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_longlong, _D]
         L.fsmc_mrg32k3a_gaussian.argtypes = [_VP, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_longlong,
                                              ctypes.c_double, ctypes.c_double, _D]
@@ -264,6 +269,27 @@ class Engine:
         _check(lib().fsmc_american_lsmc_ref(self._ctx, ctypes.byref(p), ctypes.byref(m), pz, ctypes.byref(am),
                                             ctypes.byref(eu)), "fsmc_american_lsmc_ref")
         return am.value, eu.value
+
+    # -- closed form and lattice ---------------------------------------------
+    def black_scholes(self, S, X, T, sigma, r, m, greeks=False):
+        """Returns (call[n], put[n]) or (call, put, greeks[n][3])."""
+        n = len(S)
+        arrs = [_d(np.broadcast_to(np.asarray(a, dtype=np.float64), (n,))) for a in (S, X, T, sigma)]
+        call, pc = _d(np.zeros(n))
+        put, pp = _d(np.zeros(n))
+        g, pg = _d(np.zeros((n, 3)))
+        _check(lib().fsmc_black_scholes(self._ctx, n, ctypes.byref(m), *[q for _, q in arrs], r, pc, pp,
+                                        pg if greeks else None), "fsmc_black_scholes")
+        return (call, put, g) if greeks else (call, put)
+
+    def lattice(self, S, X, T, sigma, r, steps, m, american=False, put=False):
+        n = len(S)
+        arrs = [_d(np.broadcast_to(np.asarray(a, dtype=np.float64), (n,))) for a in (S, X, T, sigma)]
+        out, po = _d(np.zeros(n))
+        flags = (LATTICE_AMERICAN if american else 0) | (LATTICE_PUT if put else 0)
+        _check(lib().fsmc_lattice(self._ctx, n, ctypes.byref(m), *[q for _, q in arrs], r, steps, flags, po),
+               "fsmc_lattice")
+        return out
 
     def mrg32k3a_uniform(self, seed, offset, n):
         """Positions offset .. offset+n-1 of the MKL-compatible MRG32k3a stream."""

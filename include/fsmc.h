@@ -58,6 +58,7 @@ extern "C" {
 #define FSMC_QUIRK_ASIAN_VAR_AS_VOL (1u << 0)   /* MCAsianOptions.cpp:57-60: sigma = v+ */
 #define FSMC_QUIRK_ASIAN_RUNNING_VAL2 (1u << 1) /* MCAsianOptions.cpp:191-192,199        */
 #define FSMC_QUIRK_LIBOR_INPLACE_ADJ (1u << 2)  /* liborSwaptionGreeks.cpp:173-184       */
+#define FSMC_QUIRK_BS_POLY_CND (1u << 3)        /* BlackScholesDP.cpp:84-113: 5-term CND  */
 
 typedef struct fsmc_mode {
   int precision;   /* FSMC_FP32 | FSMC_FP64                               */
@@ -223,6 +224,28 @@ typedef struct fsmc_lsmc_ref_params {
  * pricings (the sample's loop index). */
 int fsmc_american_lsmc_ref(fsmc_ctx* ctx, const fsmc_lsmc_ref_params* p, const fsmc_mode* m, const double* z,
                            double* american, double* european);
+
+/* ---- Closed form and lattice ---------------------------------------------------
+ * Black-Scholes calls and puts (BlackScholes/BlackScholesDP.cpp:177-201) for
+ * n options with per-option S, X, T, sigma and one rate r: call[n], put[n]
+ * (may be NULL), greeks[n][3] = call delta, vega, rho (may be NULL).  The
+ * normal CDF is exact (erfc) unless m->quirks has FSMC_QUIRK_BS_POLY_CND, the
+ * sample's Abramowitz-Stegun 26.2.17 polynomial (error below 7.5e-8). */
+int fsmc_black_scholes(fsmc_ctx* ctx, int n, const fsmc_mode* m, const double* S, const double* X, const double* T,
+                       const double* sigma, double r, double* call, double* put, double* greeks);
+int fsmc_black_scholes_async(int n, const fsmc_mode* m, const double* d_S, const double* d_X, const double* d_T,
+                             const double* d_sigma, double r, double* d_call, double* d_put, double* d_greeks,
+                             void* stream);
+
+/* Cox-Ross-Rubinstein lattice with `steps` steps (1..8000) for n options:
+ * the Binomial sample's European call (BinomialOptions/binomial_cpu.cpp:62-101)
+ * and, with the flags, puts and American exercise.  price[n]. */
+#define FSMC_LATTICE_AMERICAN 1u
+#define FSMC_LATTICE_PUT 2u
+int fsmc_lattice(fsmc_ctx* ctx, int n, const fsmc_mode* m, const double* S, const double* X, const double* T,
+                 const double* sigma, double r, int steps, unsigned flags, double* price);
+int fsmc_lattice_async(int n, const fsmc_mode* m, const double* d_S, const double* d_X, const double* d_T,
+                       const double* d_sigma, double r, int steps, unsigned flags, double* d_price, void* stream);
 
 /* ---- Mixed request batches (serving) ----------------------------------------
  * No counterpart in the samples, which price one workload per process: many
