@@ -1,12 +1,20 @@
 // Longstaff-Schwartz American puts.
 //
-// Batched "correct" LSMC (many options, uneven step and path counts):
-//   * lsmc_paths_kernel: one path per lane per round, GBM in the working
-//     precision, the whole path stored to a per-option FP32/FP64 matrix
-//     [date][path] in HBM (the backward sweep revisits every date); the
-//     terminal cash flow e^{-rT} (K - S_T)^+ is kept discounted to time 0.
+// Batched "correct" LSMC (many options, uneven step and path counts).  Paths
+// are built backwards in time by the Brownian bridge, so no path matrix is
+// stored: with W the driving Brownian motion at dates t_k = k dt,
+//   W_N = sqrt(T) Z_{N-1},  W_k = k/(k+1) W_{k+1} + sqrt(dt k/(k+1)) Z_{k-1},
+//   S_k = S0 exp((r - sigma^2/2) t_k + sigma W_k),
+// exactly the law of the forward GBM path, Z_i of path p being component p & 3
+// of Philox counter (p/4, i, option): one Philox call per four paths per date.  One state word per
+// path (W at the current date) replaces the [date][path] matrix: a pass
+// reads and rewrites it, so the 4 B per path-step matrix write and a second
+// row read per pass disappear.
+//   * lsmc_init_kernel: W_N, the terminal cash flow e^{-rT} (K - S_T)^+
+//     (kept discounted to time 0) and the European sum;
 //   * lsmc_pass_kernel, once per backward date: applies the exercise decision
-//     of date d with the coefficients fitted at d, then accumulates the
+//     of date d (spot from the stored W_d) with the coefficients fitted at d,
+//     steps the bridge to W_{d-1} (stored back) and accumulates the
 //     regression of the discounted cash flow on {1, x, x^2, x^3} over the
 //     in-the-money paths of date d-1.  x = 1 + (S/K - 1)/(sigma sqrt(t)) is
 //     an affine map of S/K (same fitted values as the plain monomials of S/K)
@@ -49,8 +57,7 @@ constexpr int kNS = 12;  // partial sums: m0..m6, c0..c3, spare
 constexpr double kLog2e = 1.4426950408889634074;
 
 struct OptMeta {
-  long long mat_off;  // first element of the option's path matrix (row k-1 = date k)
-  long long cf_off;   // first cash flow
+  long long cf_off;   // first cash flow and bridge state of the option
   int first_block, nblk, steps, paths;
   double S0, K, sigma, r, T;
 };
@@ -58,7 +65,7 @@ struct OptMeta {
 struct LsmcArgs {
   const OptMeta* meta;
   const int2* table;  // workgroup -> (option, chunk)
-  void* mat;
+  void* state;  // bridge state: W at the current date, one word per path (cf layout)
   void* cf;
   double* partials;    // [workgroup][kNS]
   const int* order;    // option indices sorted by steps (descending)
@@ -77,51 +84,72 @@ __device__ __forceinline__ Real pick4(const Real (&z)[4], int j) {
 // ---------------------------------------------------------------------------
 // Batched correct LSMC
 // ---------------------------------------------------------------------------
+// Normal i of paths 4g .. 4g+3 (path p takes component p & 3 of Philox
+// counter (p / 4, i, option)): one call serves a thread's four consecutive
+// paths at one date.
 template <typename Real>
-__global__ __launch_bounds__(kBlock) void lsmc_paths_kernel(LsmcArgs a) {
+__device__ __forceinline__ void group_normals(int g, int i, uint32_t o, const LsmcArgs& a, Real (&z)[4]) {
+  normals4<Real>(U4{(uint32_t)g, (uint32_t)i, o, a.w}, a.k0, a.k1, z[0], z[1], z[2], z[3]);
+}
+
+// A thread owns kPPT / 4 groups of four consecutive paths; per-option cash
+// flow and state segments start on a multiple of 4 elements and are padded
+// to one, so a live group is one aligned vector load / store.
+constexpr int kGroups = kPPT / 4;
+template <typename Real>
+struct Quad {
+  typedef Real V __attribute__((ext_vector_type(4)));
+};
+template <typename Real>
+__device__ __forceinline__ void load4(const Real* src, Real (&v)[4]) {
+  const typename Quad<Real>::V q = *reinterpret_cast<const typename Quad<Real>::V*>(src);
+  v[0] = q[0];
+  v[1] = q[1];
+  v[2] = q[2];
+  v[3] = q[3];
+}
+template <typename Real>
+__device__ __forceinline__ void store4(Real* dst, const Real (&v)[4]) {
+  typename Quad<Real>::V q;
+  q[0] = v[0];
+  q[1] = v[1];
+  q[2] = v[2];
+  q[3] = v[3];
+  *reinterpret_cast<typename Quad<Real>::V*>(dst) = q;
+}
+__device__ __forceinline__ int group_first_path(int chunk, int h) {
+  return chunk * kChunk + 4 * (h * kBlock + (int)threadIdx.x);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void lsmc_init_kernel(LsmcArgs a) {
   using O = MathOps<Real>;
   __shared__ double lds[kBlock / 64];
   const int2 e = a.table[blockIdx.x];
   const OptMeta m = a.meta[e.x];
-  const double dt = m.T / m.steps;
-  const Real drift = (Real)((m.r - 0.5 * m.sigma * m.sigma) * dt * O::kE);
-  const Real vol = (Real)(m.sigma * sqrt(dt) * O::kE);
+  const Real sqT = (Real)sqrt(m.T);
+  const Real muT = (Real)((m.r - 0.5 * m.sigma * m.sigma) * m.T * O::kE), sg = (Real)(m.sigma * O::kE);
   const Real discT = (Real)exp(-m.r * m.T);
-  const Real K = (Real)m.K;
-  Real* mat = (Real*)a.mat + m.mat_off;
+  const Real K = (Real)m.K, S0 = (Real)m.S0;
+  Real* w = (Real*)a.state + m.cf_off;
   Real* cf = (Real*)a.cf + m.cf_off;
-  const size_t P = (size_t)m.paths;
   double eu[1] = {0.0};
-  // four paths per thread advance together (independent exp chains and
-  // Philox calls in flight), each storing its row entry per date
-  constexpr int kG = 4;
-  for (int q0 = 0; q0 < kPPT; q0 += kG) {
-    int pp[kG];
-    Real S[kG], z[kG][4];
 #pragma unroll
-    for (int g = 0; g < kG; ++g) {
-      pp[g] = e.y * kChunk + (q0 + g) * kBlock + threadIdx.x;
-      S[g] = (Real)m.S0;
-    }
-    for (int j = 0; j < m.steps; ++j) {
+  for (int h = 0; h < kGroups; ++h) {
+    const int p0 = group_first_path(e.y, h);
+    if (p0 >= m.paths) continue;
+    Real z[4];
+    group_normals<Real>(p0 >> 2, m.steps - 1, (uint32_t)e.x, a, z);
+    Real W[4], c[4];
 #pragma unroll
-      for (int g = 0; g < kG; ++g) {
-        if (pp[g] >= m.paths) continue;
-        if ((j & 3) == 0)
-          normals4<Real>(U4{(uint32_t)pp[g], (uint32_t)(j >> 2), (uint32_t)e.x, a.w}, a.k0, a.k1, z[g][0], z[g][1],
-                         z[g][2], z[g][3]);
-        S[g] = S[g] * O::ex(drift + vol * pick4(z[g], j & 3));
-        mat[(size_t)j * P + pp[g]] = S[g];
-      }
+    for (int g = 0; g < 4; ++g) {
+      W[g] = sqT * z[g];
+      const Real ST = S0 * O::ex(muT + sg * W[g]);
+      c[g] = p0 + g < m.paths ? discT * (K - ST > Real(0) ? K - ST : Real(0)) : Real(0);
+      eu[0] += (double)c[g];
     }
-#pragma unroll
-    for (int g = 0; g < kG; ++g) {
-      if (pp[g] >= m.paths) continue;
-      const Real pay = K - S[g] > Real(0) ? K - S[g] : Real(0);
-      const Real c = discT * pay;
-      cf[pp[g]] = c;
-      eu[0] += (double)c;
-    }
+    store4(w + p0, W);  // padding entries are written too (never read as paths)
+    store4(cf + p0, c);
   }
   block_sum<1>(eu, lds);
   if (threadIdx.x == 0) a.eur[blockIdx.x] = eu[0];
@@ -199,6 +227,7 @@ __device__ void lsmc_solve(const double (&s)[kNS], double* beta_out) {
 
 template <typename Real>
 __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
+  using O = MathOps<Real>;
   __shared__ double lds[kNS * (kBlock / 64)];
   const int2 e = a.table[blockIdx.x];
   const int o = e.x;
@@ -209,47 +238,65 @@ __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
   const double* bin = a.beta + (size_t)((a.pass & 1) * a.n_options + o) * 5;
   const bool decide = a.pass >= 1 && bin[4] != 0.0;
   const Real b0 = (Real)bin[0], b1 = (Real)bin[1], b2 = (Real)bin[2], b3 = (Real)bin[3];
-  const Real K = (Real)m.K, invK = (Real)(1.0 / m.K);
+  const Real K = (Real)m.K, invK = (Real)(1.0 / m.K), S0 = (Real)m.S0;
+  const double mu = (m.r - 0.5 * m.sigma * m.sigma) * dt;
+  const Real mu_d = (Real)(mu * d * O::kE), mu_k = (Real)(mu * k * O::kE), sg = (Real)(m.sigma * O::kE);
   const Real isw_d = (Real)(1.0 / (m.sigma * sqrt(d * dt))), disc_d = (Real)exp(-m.r * d * dt);
   const Real isw_k = k >= 1 ? (Real)(1.0 / (m.sigma * sqrt(k * dt))) : Real(0);
-  Real* mat = (Real*)a.mat + m.mat_off;
+  // bridge from date d back to k = d - 1: W_k = k/d W_d + sqrt(dt k/d) Z_{k-1}
+  const Real ba = (Real)((double)k / d), bb = (Real)sqrt(dt * k / d);
+  Real* wst = (Real*)a.state + m.cf_off;
   Real* cf = (Real*)a.cf + m.cf_off;
-  const size_t P = (size_t)m.paths;
   double s[kNS];
 #pragma unroll
   for (int q = 0; q < kNS; ++q) s[q] = 0.0;
   // every load of the workgroup's paths is issued before any use: a pass is
-  // one short stream per path, so memory-level parallelism sets its speed.
-  // A spot of K marks "not in the money" (dead lanes, no decision date).
-  Real cv[kPPT], sd[kPPT], sk[kPPT];
+  // one short stream per path, so memory-level parallelism sets its speed
+  Real cv[kGroups][4], wd[kGroups][4];
 #pragma unroll
-  for (int q = 0; q < kPPT; ++q) {
-    const int p = e.y * kChunk + q * kBlock + threadIdx.x;
-    const bool live = p < m.paths;
-    cv[q] = live ? cf[p] : Real(0);
-    sd[q] = live && a.pass >= 1 ? mat[(size_t)(d - 1) * P + p] : K;  // decide is known later
-    sk[q] = live && k >= 1 ? mat[(size_t)(k - 1) * P + p] : K;
+  for (int h = 0; h < kGroups; ++h) {
+    const int p0 = group_first_path(e.y, h);
+    if (p0 < m.paths) {
+      load4(cf + p0, cv[h]);
+      load4(wst + p0, wd[h]);
+    } else {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) cv[h][g] = wd[h][g] = Real(0);
+    }
   }
 #pragma unroll
-  for (int q = 0; q < kPPT; ++q) {
-    const int p = e.y * kChunk + q * kBlock + threadIdx.x;
-    Real c = cv[q];
-    const Real Sd = sd[q];
-    if (decide && K - Sd > Real(0)) {
-      const Real x = Real(1) + (Sd * invK - Real(1)) * isw_d;
-      const Real cont = b0 + x * (b1 + x * (b2 + x * b3));
-      const Real ex = (K - Sd) * disc_d;
-      if (ex > cont) {
-        c = ex;
-        cf[p] = c;
+  for (int h = 0; h < kGroups; ++h) {
+    const int p0 = group_first_path(e.y, h);
+    if (p0 >= m.paths) continue;
+    Real z[4] = {0, 0, 0, 0}, wk[4];
+    if (k >= 1) group_normals<Real>(p0 >> 2, k - 1, (uint32_t)o, a, z);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int p = p0 + g;
+      const bool live = p < m.paths;
+      Real c = cv[h][g];
+      if (decide && live) {
+        const Real Sd = S0 * O::ex(mu_d + sg * wd[h][g]);
+        if (K - Sd > Real(0)) {
+          const Real x = Real(1) + (Sd * invK - Real(1)) * isw_d;
+          const Real cont = b0 + x * (b1 + x * (b2 + x * b3));
+          const Real ex = (K - Sd) * disc_d;
+          if (ex > cont) {
+            c = ex;
+            cf[p] = c;
+          }
+        }
+      }
+      if (k >= 1) {  // uniform per workgroup
+        const Real Wk = ba * wd[h][g] + bb * z[g];
+        wk[g] = Wk;
+        const Real Sk = S0 * O::ex(mu_k + sg * Wk);
+        const bool itm = live && K - Sk > Real(0);
+        const Real x = itm ? Real(1) + (Sk * invK - Real(1)) * isw_k : Real(0);
+        accum_valu(s, itm, (double)x, (double)c);
       }
     }
-    if (k >= 1) {  // uniform per workgroup
-      const Real Sk = sk[q];
-      const bool itm = K - Sk > Real(0);
-      const Real x = itm ? Real(1) + (Sk * invK - Real(1)) * isw_k : Real(0);
-      accum_valu(s, itm, (double)x, (double)c);
-    }
+    if (k >= 1) store4(wst + p0, wk);  // padding entries too (never read as paths)
   }
   if (k < 1) return;  // last pass of this option: decision only
   block_sum<kNS>(s, lds);
@@ -459,8 +506,8 @@ struct LsmcPlanImpl {
   std::vector<int> grid_of_pass;  // workgroups launched by backward pass j
   std::vector<int> live_of_pass;  // options with a date left at backward pass j
   int max_steps = 0;
-  size_t real = 4, mat_elems = 0, cf_elems = 0;
-  size_t off_meta, off_table, off_order, off_mat, off_cf, off_part, off_beta, off_eur, off_out, ws = 0;
+  size_t real = 4, cf_elems = 0;
+  size_t off_meta, off_table, off_order, off_w, off_cf, off_part, off_beta, off_eur, off_out, ws = 0;
 };
 
 int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K, const double* sigma,
@@ -482,11 +529,10 @@ int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K,
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return steps[x] > steps[y]; });
   pl->meta.resize(n);
   pl->table.clear();
-  long long mat = 0, cfo = 0;
+  long long cfo = 0;
   for (int idx = 0; idx < n; ++idx) {
     const int o = order[idx];
     OptMeta& q = pl->meta[o];
-    q.mat_off = mat;
     q.cf_off = cfo;
     q.steps = steps[o];
     q.paths = paths[o];
@@ -498,10 +544,8 @@ int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K,
     q.r = r[o];
     q.T = T[o];
     for (int c = 0; c < q.nblk; ++c) pl->table.push_back(make_int2(o, c));
-    mat += (long long)steps[o] * paths[o];
-    cfo += paths[o];
+    cfo += (paths[o] + 3) / 4 * 4;  // segments aligned and padded to whole groups of 4
   }
-  pl->mat_elems = (size_t)mat;
   pl->cf_elems = (size_t)cfo;
   pl->max_steps = steps[order[0]];
   pl->order = order;
@@ -526,7 +570,7 @@ int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K,
   pl->off_meta = take(sizeof(OptMeta) * n);
   pl->off_table = take(sizeof(int2) * pl->table.size());
   pl->off_order = take(sizeof(int) * n);
-  pl->off_mat = take(pl->real * pl->mat_elems);
+  pl->off_w = take(pl->real * pl->cf_elems);
   pl->off_cf = take(pl->real * pl->cf_elems);
   pl->off_part = take(sizeof(double) * kNS * pl->table.size());
   pl->off_beta = take(sizeof(double) * 10 * n);
@@ -541,7 +585,7 @@ int lsmc_enqueue(const LsmcPlanImpl& pl, char* ws, hipStream_t s) {
   LsmcArgs a;
   a.meta = (const OptMeta*)(ws + pl.off_meta);
   a.table = (const int2*)(ws + pl.off_table);
-  a.mat = ws + pl.off_mat;
+  a.state = ws + pl.off_w;
   a.cf = ws + pl.off_cf;
   a.partials = (double*)(ws + pl.off_part);
   a.order = (const int*)(ws + pl.off_order);
@@ -554,7 +598,7 @@ int lsmc_enqueue(const LsmcPlanImpl& pl, char* ws, hipStream_t s) {
   a.pass = 0;
   hipLaunchKernelGGL(zero_f64_kernel, dim3((10 * pl.n + 255) / 256), dim3(256), 0, s, a.beta, 10 * pl.n);
   const int nb = (int)pl.table.size();
-  hipLaunchKernelGGL(lsmc_paths_kernel<Real>, dim3(nb), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(lsmc_init_kernel<Real>, dim3(nb), dim3(kBlock), 0, s, a);
   FSMC_HIP(hipGetLastError());
   for (int j = 0; j < pl.max_steps; ++j) {
     a.pass = j;

@@ -25,7 +25,10 @@
  *   Heston  c0 = path,      c1 = step / 2, c2 = option; (z1, z2) of step 2 c1
  *           are normals 0, 1 and of step 2 c1 + 1 normals 2, 3
  *   LIBOR   c0 = path,      c1 = step / 4, c2 = 0;      normal j -> step 4 c1 + j
- *   LSMC    c0 = path,      c1 = step / 4, c2 = option; normal j -> step 4 c1 + j
+ *   LSMC    batched (Brownian bridge): c0 = path / 4, c1 = i, c2 = option;
+ *           normal j -> Z_i of path 4 c0 + j
+ *           reference algorithm: c0 = path, c1 = step / 4, c2 = 0;
+ *           normal j -> step 4 c1 + j
  */
 #ifndef FSMC_H_
 #define FSMC_H_
@@ -183,7 +186,10 @@ int fsmc_mrg32k3a_gaussian_async(uint32_t seed, uint64_t offset, long long n, do
  * exercise date per time step, cash flows discounted at r, regression of the
  * discounted cash flow on {1, x, x^2, x^3} of the in-the-money paths with x an
  * affine rescaling of S/K (moment sums per lane in FP64, solved per date by
- * FP64 Cholesky), Philox normals.  It replaces the sweep inlined in
+ * FP64 Cholesky).  Paths are built backwards by the Brownian bridge from
+ * Philox normals: Z_i of path p = component p & 3 of counter (p/4, i, option)
+ * (tag 4), W_N = sqrt(T) Z_{N-1}, W_k = k/(k+1) W_{k+1} + sqrt(dt k/(k+1)) Z_{k-1},
+ * S_k = S0 exp((r - sigma^2/2) k dt + sigma W_k).  It replaces the sweep inlined in
  * MonteCarloAmericanOptions/main.cpp:131-279 (with perform_sims,
  * utilities.cpp:42-88) for the configurations that sample cannot express
  * (rates, many options).  price[n][2] = {price, standard error}; european

@@ -635,23 +635,34 @@ int orc_chol4_solve(const double* G, const double* rhs, double* beta) {
  * LSMC tag, normal j%4 drives the step from S_j to S_{j+1}. */
 static void lsmc_one(int o, double S0, double K, double sigma, double r, double T, int n, int P, uint64_t seed,
                      uint32_t stream, double* price2, double* european) {
+  /* Paths by the Brownian bridge, backwards from T (the engine's layout,
+   * documented in include/fsmc.h): Z_i of path p = component p & 3 of
+   * counter (p / 4, i, option), W_N = sqrt(T) Z_{N-1},
+   * W_k = k/(k+1) W_{k+1} + sqrt(dt k/(k+1)) Z_{k-1},
+   * S_k = S0 exp((r - sigma^2/2) dt k + sigma W_k), S_N with (r - sigma^2/2) T. */
   const double dt = T / n;
-  const double drift = (r - 0.5 * sigma * sigma) * dt, vol = sigma * sqrt(dt);
+  const double mu = (r - 0.5 * sigma * sigma) * dt;
   double* S = (double*)malloc(sizeof(double) * (size_t)n * P); /* row j = S_{j+1} */
+  double* W = (double*)malloc(sizeof(double) * (size_t)n);     /* W[k-1] = W_k of one path */
   double* cf = (double*)malloc(sizeof(double) * (size_t)P);
   const double discT = exp(-r * T);
   double eu = 0.0;
   for (int p = 0; p < P; ++p) {
-    double x = S0, zq[4] = {0, 0, 0, 0};
-    for (int j = 0; j < n; ++j) {
-      if ((j & 3) == 0) orc_normals4(seed, stream, 4u, (uint32_t)p, (uint32_t)(j >> 2), (uint32_t)o, zq);
-      x = x * exp(drift + vol * zq[j & 3]);
-      S[(size_t)j * P + p] = x;
+    double zq[4];
+    for (int k = n; k >= 1; --k) {
+      const int i = k - 1;
+      orc_normals4(seed, stream, 4u, (uint32_t)(p >> 2), (uint32_t)i, (uint32_t)o, zq);
+      const double z = zq[p & 3];
+      W[k - 1] = k == n ? sqrt(T) * z : ((double)k / (k + 1)) * W[k] + sqrt(dt * k / (k + 1)) * z;
     }
+    for (int k = 1; k < n; ++k) S[(size_t)(k - 1) * P + p] = S0 * exp(mu * k + sigma * W[k - 1]);
+    const double x = S0 * exp((r - 0.5 * sigma * sigma) * T + sigma * W[n - 1]);
+    S[(size_t)(n - 1) * P + p] = x;
     const double pay = K - x > 0.0 ? K - x : 0.0;
     cf[p] = discT * pay;
     eu += cf[p];
   }
+  free(W);
   for (int k = n - 1; k >= 1; --k) {
     const double tk = k * dt, disc = exp(-r * tk);
     const double inv_sw = 1.0 / (sigma * sqrt(tk));

@@ -428,3 +428,18 @@ def test_icdf_matches_exact_inverse():
     for v in (0.075, 0.5 - 0.425, np.exp(-25.0)):
         assert orc.icdf(v) == pytest.approx(ndtri(v), rel=1e-15)
     assert orc.icdf(0.3) == -orc.icdf(0.7) or abs(orc.icdf(0.3) + orc.icdf(0.7)) < 1e-15
+
+
+def test_lsmc_oracle_bridge_paths_have_gbm_law():
+    # The batched LSMC builds paths backwards by the Brownian bridge: its
+    # European leg must be the Black-Scholes put, for short and long maturities
+    S0, K, sig, r = np.array([90.0, 100.0, 110.0]), np.full(3, 100.0), np.array([.2, .3, .4]), np.full(3, .03)
+    T, steps, paths = np.array([.5, 1.0, 2.0]), np.array([17, 64, 9]), np.full(3, 200000)
+    pr, se, eu = orc.lsmc(S0, K, sig, r, T, steps, paths, seed=5)
+    bs_put = np.array([orc.bs_call(*a) for a in zip(S0, K, T, r, sig)]) - S0 + K * np.exp(-r * T)
+    sd = np.array([np.sqrt(np.var(np.maximum(K[i] - S0[i] * np.exp((r[i] - sig[i] ** 2 / 2) * T[i] + sig[i]
+                                                                        * np.sqrt(T[i]) * np.random.default_rng(i)
+                                                                        .standard_normal(200000)), 0))
+                           * np.exp(-2 * r[i] * T[i]) / 200000) for i in range(3)])
+    assert np.all(np.abs(eu - bs_put) < 4 * sd)
+    assert np.all(pr >= eu - 1e-12)
