@@ -51,7 +51,7 @@ namespace fsmc {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kPPT = 8;  // paths per thread per workgroup
+constexpr int kPPT = 16;  // paths per thread per workgroup
 constexpr int kChunk = kBlock * kPPT;
 constexpr int kNS = 12;  // partial sums: m0..m6, c0..c3, spare
 constexpr double kLog2e = 1.4426950408889634074;
@@ -64,7 +64,7 @@ struct OptMeta {
 
 struct LsmcArgs {
   const OptMeta* meta;
-  const int2* table;  // workgroup -> (option, chunk)
+  const int4* table;  // workgroup -> (option, chunk, first cash-flow element, paths in the chunk)
   void* state;  // bridge state: W at the current date, one word per path (cf layout)
   void* cf;
   double* partials;    // [workgroup][kNS]
@@ -125,7 +125,7 @@ template <typename Real>
 __global__ __launch_bounds__(kBlock) void lsmc_init_kernel(LsmcArgs a) {
   using O = MathOps<Real>;
   __shared__ double lds[kBlock / 64];
-  const int2 e = a.table[blockIdx.x];
+  const int4 e = a.table[blockIdx.x];
   const OptMeta m = a.meta[e.x];
   const Real sqT = (Real)sqrt(m.T);
   const Real muT = (Real)((m.r - 0.5 * m.sigma * m.sigma) * m.T * O::kE), sg = (Real)(m.sigma * O::kE);
@@ -229,8 +229,24 @@ template <typename Real>
 __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
   using O = MathOps<Real>;
   __shared__ double lds[kNS * (kBlock / 64)];
-  const int2 e = a.table[blockIdx.x];
+  // the table entry alone addresses the loads, so they leave while the
+  // option's constants and coefficients are still in flight
+  const int4 e = a.table[blockIdx.x];
   const int o = e.x;
+  Real* wst = (Real*)a.state + e.z;
+  Real* cf = (Real*)a.cf + e.z;
+  Real cv[kGroups][4], wd[kGroups][4];
+#pragma unroll
+  for (int h = 0; h < kGroups; ++h) {
+    const int l = 4 * (h * kBlock + (int)threadIdx.x);  // first path of the group within the chunk
+    if (l < e.w) {
+      load4(cf + l, cv[h]);
+      load4(wst + l, wd[h]);
+    } else {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) cv[h][g] = wd[h][g] = Real(0);
+    }
+  }
   const OptMeta m = a.meta[o];
   const int d = m.steps - a.pass;  // decision date
   const int k = d - 1;             // regression date
@@ -245,35 +261,20 @@ __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
   const Real isw_k = k >= 1 ? (Real)(1.0 / (m.sigma * sqrt(k * dt))) : Real(0);
   // bridge from date d back to k = d - 1: W_k = k/d W_d + sqrt(dt k/d) Z_{k-1}
   const Real ba = (Real)((double)k / d), bb = (Real)sqrt(dt * k / d);
-  Real* wst = (Real*)a.state + m.cf_off;
-  Real* cf = (Real*)a.cf + m.cf_off;
   double s[kNS];
 #pragma unroll
   for (int q = 0; q < kNS; ++q) s[q] = 0.0;
-  // every load of the workgroup's paths is issued before any use: a pass is
-  // one short stream per path, so memory-level parallelism sets its speed
-  Real cv[kGroups][4], wd[kGroups][4];
 #pragma unroll
   for (int h = 0; h < kGroups; ++h) {
-    const int p0 = group_first_path(e.y, h);
-    if (p0 < m.paths) {
-      load4(cf + p0, cv[h]);
-      load4(wst + p0, wd[h]);
-    } else {
-#pragma unroll
-      for (int g = 0; g < 4; ++g) cv[h][g] = wd[h][g] = Real(0);
-    }
-  }
-#pragma unroll
-  for (int h = 0; h < kGroups; ++h) {
-    const int p0 = group_first_path(e.y, h);
-    if (p0 >= m.paths) continue;
+    const int l0 = 4 * (h * kBlock + (int)threadIdx.x);
+    if (l0 >= e.w) continue;
+    const int p0 = e.y * kChunk + l0;  // path index of the group's first path
     Real z[4] = {0, 0, 0, 0}, wk[4];
     if (k >= 1) group_normals<Real>(p0 >> 2, k - 1, (uint32_t)o, a, z);
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-      const int p = p0 + g;
-      const bool live = p < m.paths;
+      const int p = l0 + g;  // element within the chunk
+      const bool live = p < e.w;
       Real c = cv[h][g];
       if (decide && live) {
         const Real Sd = S0 * O::ex(mu_d + sg * wd[h][g]);
@@ -296,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
         accum_valu(s, itm, (double)x, (double)c);
       }
     }
-    if (k >= 1) store4(wst + p0, wk);  // padding entries too (never read as paths)
+    if (k >= 1) store4(wst + l0, wk);  // padding entries too (never read as paths)
   }
   if (k < 1) return;  // last pass of this option: decision only
   block_sum<kNS>(s, lds);
@@ -501,7 +502,7 @@ struct LsmcPlanImpl {
   int precision;
   fsmc_mode mode;
   std::vector<OptMeta> meta;
-  std::vector<int2> table;
+  std::vector<int4> table;
   std::vector<int> order;         // option indices sorted by steps (descending)
   std::vector<int> grid_of_pass;  // workgroups launched by backward pass j
   std::vector<int> live_of_pass;  // options with a date left at backward pass j
@@ -543,7 +544,8 @@ int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K,
     q.sigma = sigma[o];
     q.r = r[o];
     q.T = T[o];
-    for (int c = 0; c < q.nblk; ++c) pl->table.push_back(make_int2(o, c));
+    for (int c = 0; c < q.nblk; ++c)
+      pl->table.push_back(make_int4(o, c, (int)(q.cf_off + (long long)c * kChunk), std::min(kChunk, paths[o] - c * kChunk)));
     cfo += (paths[o] + 3) / 4 * 4;  // segments aligned and padded to whole groups of 4
   }
   pl->cf_elems = (size_t)cfo;
@@ -568,7 +570,7 @@ int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K,
     return at;
   };
   pl->off_meta = take(sizeof(OptMeta) * n);
-  pl->off_table = take(sizeof(int2) * pl->table.size());
+  pl->off_table = take(sizeof(int4) * pl->table.size());
   pl->off_order = take(sizeof(int) * n);
   pl->off_w = take(pl->real * pl->cf_elems);
   pl->off_cf = take(pl->real * pl->cf_elems);
@@ -584,7 +586,7 @@ template <typename Real>
 int lsmc_enqueue(const LsmcPlanImpl& pl, char* ws, hipStream_t s) {
   LsmcArgs a;
   a.meta = (const OptMeta*)(ws + pl.off_meta);
-  a.table = (const int2*)(ws + pl.off_table);
+  a.table = (const int4*)(ws + pl.off_table);
   a.state = ws + pl.off_w;
   a.cf = ws + pl.off_cf;
   a.partials = (double*)(ws + pl.off_part);
@@ -640,7 +642,7 @@ int fsmc_lsmc_plan_create(fsmc_ctx* ctx, int n_options, const fsmc_mode* m, cons
       char* ws = (char*)pl->ws.ptr;
       if (hipMemcpyAsync(ws + pl->impl.off_meta, pl->impl.meta.data(), sizeof(OptMeta) * n_options,
                          hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
-          hipMemcpyAsync(ws + pl->impl.off_table, pl->impl.table.data(), sizeof(int2) * pl->impl.table.size(),
+          hipMemcpyAsync(ws + pl->impl.off_table, pl->impl.table.data(), sizeof(int4) * pl->impl.table.size(),
                          hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
           hipMemcpyAsync(ws + pl->impl.off_order, pl->impl.order.data(), sizeof(int) * n_options,
                          hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
