@@ -180,20 +180,26 @@ __global__ __launch_bounds__(kBlock) void libor_path_kernel(LiborArgs a) {
   for (int i = 0; i < NN; ++i) out[(size_t)(i + 1) * P] = Lb[i];
 }
 
+// Largest lambda for which the FP32 kernel rebuilds rows instead of storing
+// them: the third-order series below is then within 1.5e-8 of the exact
+// inverse for every rate (2.3e-5 at lambda = 0.75).
+constexpr double kRebuildMaxLambda = 0.3;
+
 // FP32 variant without the trajectory buffer.  The reverse sweep rebuilds row
 // n from row n+1 instead of reading it back: for i = n+1, n+2, ... the forward
 // map L(n+1,i) = x exp(con1 (v(i-1) + con1 x / (1 + delta x)) + c2(n)) with
 // x = L(n,i) is inverted through the fixed point
 //   x = w exp(-a t(x)),  t(x) = x / (1 + delta x),  a = con1^2,
-//   w = L(n+1,i) exp(-(con1 v(i-1) + c2(n))),
-// a contraction of factor a t'(x) x ~ 1e-4 (con1 = delta lambda): one step
-// from x = w, x1 = w exp(-a t(w)), plus its second-order term,
-// x = x1 (1 + a^2 t(w)^2 / (1 + delta w)), reaches FP32 rounding (checked
-// for L up to 1 against two full iterations).  The adjoint then runs in
-// the same ascending order: its running sum v1(i) = sum_{k >= i} lambda
-// L(n+1,k) Lb(k) is taken as the row total minus the prefix below i.  No
-// HBM traffic beyond the [81][path] outputs; the cost is about 1.7x the
-// VALU work of the stored-trajectory kernel.
+//   w = L(n+1,i) exp(-(con1 v(i-1) + c2(n))).
+// With s = a t(w) and r0 = 1 / (1 + delta w), its solution is the series
+//   x = w (1 - s + (r0 + 1/2) s^2 - (2 r0^2 + r0/2 + 1/6) s^3 + O(s^4))
+// (at delta = 0 this is Lambert's W(aw)/a = w (1 - s + 3/2 s^2 - 8/3 s^3)).
+// s < a / delta = delta lambda^2 for any rate, so the truncation stays below
+// FP32 rounding for lambda <= kRebuildMaxLambda; larger lambdas take the
+// stored-trajectory kernel.  The adjoint then runs in the same ascending
+// order: its running sum v1(i) = sum_{k >= i} lambda L(n+1,k) Lb(k) is taken
+// as the row total minus the prefix below i.  No HBM traffic beyond the
+// [81][path] outputs.
 template <bool kBuffer, bool kInplace>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void libor_path_rc_kernel(
     LiborArgs a) {  // two waves per SIMD: at most 256 registers per lane, no spills (80 rates + 80 adjoints)
@@ -205,7 +211,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   const float lam = (float)a.lambda;
   const float con1 = delta * lam;
   const float kk = con1 * con1 * O::kE;  // log2 exponent per unit of sum L / (1 + delta L)
-  const float a2 = (float)((double)kk * 0.69314718055994531 * (double)kk * 0.69314718055994531);
   const float L0 = (float)a.L0;
 
   // ---- forward: path_calc_b1 (no stores) ---------------------------------------
@@ -229,12 +234,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     }
   }
 
-  float Lb[NN];
-  const float val = portfolio_adjoint<float, kInplace>(L, Lb);
+  // The reverse sweep carries M(i) = Lb(i) L(n+1,i) instead of Lb(i): the
+  // reference's update Lb(n,i) = Lb(n+1,i) L(n+1,i) / L(n,i) + v1 lambda faci^2
+  // becomes M(n,i) = M(n+1,i) + L(n,i) lambda v1 faci^2, with no division, and
+  // the row total S = sum_{k>n} L(n+1,k) Lb(k) is the plain sum of M.  Every
+  // rate starts at L0, so Lb(0,i) = M(0,i) / L0.
+  float M[NN];
+  const float val = portfolio_adjoint<float, kInplace>(L, M);
+#pragma unroll
+  for (int i = 1; i < NN; ++i) M[i] *= L[i];
 
   // ---- adjoint: path_calc_b2, rows rebuilt in registers ------------------------
-  // With S = sum_{k>n} L(n+1,k) Lb(k) and P its prefix below i, the
-  // reference's v1 lambda faci^2 is con1^2 (S - P) / (1 + delta L(n,i))^2.
+  // v1(i) lambda faci^2 = con1^2 (S - P) / (1 + delta L(n,i))^2, with P the
+  // prefix of the row total below i.
   const float con1sq = con1 * con1;
   for (int n = NMAT - 1; n >= 0; --n) {
     float z;
@@ -252,31 +264,36 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     float S = 0.f;
 #pragma unroll
     for (int i = 1; i < NN; ++i)
-      if (i > n) S = __builtin_fmaf(L[i], Lb[i], S);
+      if (i > n) S += M[i];
     float vs = 0.f, P0 = 0.f;
 #pragma unroll
     for (int i = 1; i < NN; ++i) {
       if (i > n) {
-        const float y = L[i];  // L(n+1, i)
-        const float w = y * O::ex(-__builtin_fmaf(kk, vs, c2));
+        const float w = L[i] * O::ex(-__builtin_fmaf(kk, vs, c2));
         const float r0 = __builtin_amdgcn_rcpf(__builtin_fmaf(delta, w, 1.f));
-        const float q = w * r0;
-        const float x1 = w * O::ex(-kk * q);
-        const float x = __builtin_fmaf(x1, a2 * (q * q) * r0, x1);
-        const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(delta, x, 1.f));
-        vs = __builtin_fmaf(x, r, vs);
+        // x = w (1 + e), e = -s + c2 s^2 + c3 s^3, s = con1^2 w / (1 + delta w)
+        const float s = con1sq * (w * r0);
+        const float c3 = __builtin_fmaf(r0, __builtin_fmaf(-2.f, r0, -0.5f), -1.f / 6.f);
+        const float h = __builtin_fmaf(s, __builtin_fmaf(s, c3, r0 + 0.5f), -1.f);
+        const float x = __builtin_fmaf(w, s * h, w);
+        // 1 / (1 + delta x) by one Newton step from r0: |delta (x - w) r0| < 1e-4
+        const float r = __builtin_fmaf(r0, __builtin_fmaf(-__builtin_fmaf(delta, x, 1.f), r0, 1.f), r0);
+        const float xr = x * r;
+        vs += xr;
         const float d = S - P0;
-        P0 = __builtin_fmaf(y, Lb[i], P0);
-        Lb[i] = __builtin_fmaf(d * (r * r), con1sq, Lb[i] * (y * __builtin_amdgcn_rcpf(x)));
+        P0 += M[i];
+        M[i] = __builtin_fmaf(xr * r, d * con1sq, M[i]);
         L[i] = x;
       }
     }
   }
 
+  const float inv_L0 = (float)(1.0 / a.L0);
   float* out = (float*)a.out + p;
   out[0] = val;
+  out[P] = M[0];
 #pragma unroll
-  for (int i = 0; i < NN; ++i) out[(size_t)(i + 1) * P] = Lb[i];
+  for (int i = 1; i < NN; ++i) out[(size_t)(i + 1) * P] = M[i] * inv_L0;
 }
 
 // Sum and sum of squares of quantity q over a strided set of paths, in FP64.
@@ -334,12 +351,16 @@ struct LiborPlan {
   size_t real, traj_bytes, out_bytes, part_bytes, res_bytes, ws;
 };
 
+bool stores_trajectory(const fsmc_libor_params* p, const fsmc_mode* m) {
+  return m->precision == FSMC_FP64 || p->lambda > kRebuildMaxLambda;
+}
+
 LiborPlan libor_plan(const fsmc_libor_params* p, const fsmc_mode* m) {
   LiborPlan l;
   l.real = m->precision == FSMC_FP64 ? sizeof(double) : sizeof(float);
   const size_t P = (size_t)p->n_paths;
-  // the FP32 kernel rebuilds the trajectory in registers; FP64 stores it
-  l.traj_bytes = m->precision == FSMC_FP64 ? align_up((size_t)kSlots * P * l.real, 256) : 0;
+  // the FP32 kernel rebuilds the trajectory in registers up to kRebuildMaxLambda
+  l.traj_bytes = stores_trajectory(p, m) ? align_up((size_t)kSlots * P * l.real, 256) : 0;
   l.out_bytes = align_up((size_t)(NN + 1) * P * l.real, 256);
   l.part_bytes = align_up((size_t)(NN + 1) * kReduceChunks * 2 * sizeof(double), 256);
   l.res_bytes = align_up(2 * (NN + 1) * sizeof(double), 256);
@@ -359,7 +380,7 @@ int libor_validate(const fsmc_libor_params* p, const fsmc_mode* m, bool mrg_allo
 template <typename Real, bool B, bool Q>
 void launch(const LiborArgs& a, hipStream_t s) {
   const dim3 grid((a.n_paths + kBlock - 1) / kBlock);
-  if constexpr (sizeof(Real) == 4)
+  if (sizeof(Real) == 4 && a.lambda <= kRebuildMaxLambda)
     hipLaunchKernelGGL((libor_path_rc_kernel<B, Q>), grid, dim3(kBlock), 0, s, a);
   else
     hipLaunchKernelGGL((libor_path_kernel<Real, B, Q>), grid, dim3(kBlock), 0, s, a);

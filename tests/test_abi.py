@@ -76,3 +76,11 @@ def test_workspace_sizes(lib):
     h = mcengine.HestonParams(64, 131072, 365, 0.03, -0.7)
     ws = lib.fsmc_heston_asian_workspace(ctypes.byref(h), ctypes.byref(m))
     assert ws > 0
+    # LIBOR: FP32 rebuilds the rows in registers up to lambda = 0.3 and
+    # stores the 2380-slot trajectory beyond it, as FP64 always does
+    for prec, lam, stored in ((mcengine.FP32, 0.2, False), (mcengine.FP32, 0.3, False),
+                              (mcengine.FP32, 0.31, True), (mcengine.FP64, 0.2, True)):
+        lp = mcengine.LiborParams(4096, 0.05, lam)
+        ws = lib.fsmc_libor_workspace(ctypes.byref(lp), ctypes.byref(mcengine.mode(prec)))
+        real = 8 if prec == mcengine.FP64 else 4
+        assert ws > 0 and (ws >= 2380 * 4096 * real) == stored

@@ -38,6 +38,21 @@ def test_buffer_fp32_matches_oracle(engine):
     assert np.all(np.abs(d - od) <= 1e-5 * np.abs(od) + 1e-3 * odse)
 
 
+@pytest.mark.parametrize("lam", [0.05, 0.3, 0.31, 0.45])
+def test_buffer_fp32_lambda_range(engine, lam):
+    # lambda <= 0.3 rebuilds the rows in registers (0.3 is the series' edge),
+    # larger lambdas store the trajectory; both agree with the oracle.  (At
+    # lambda = 0.6 FP32 rates overflow on some paths, and at lambda = 0 every
+    # path is the same, so the mean keeps the 4e-5 FP32 rounding of the
+    # swaption payoffs: neither is an FP32 workload.)
+    n = 2048
+    z = np.random.default_rng(3).standard_normal((n, 40))
+    v, vse, d, dse = engine.libor(n, mc.mode(mc.FP32, mc.RNG_BUFFER), L0=0.051, lam=lam, z=z)
+    ov, ovse, od, odse = orc.libor(n, L0=0.051, lam=lam, z=z)
+    assert abs(v - ov) <= 1e-5 * abs(ov)
+    assert np.all(np.abs(d - od) <= 1e-5 * np.abs(od) + 1e-3 * odse)
+
+
 @pytest.mark.parametrize("n", [4096, 1001])
 def test_philox_fp64_matches_oracle(engine, n):
     v, vse, d, dse = engine.libor(n, mc.mode(mc.FP64, seed=17, stream=5))
