@@ -42,16 +42,29 @@ __device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
     const uint64_t p1 = (uint64_t)kPhiloxM1 * c.z;
     const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
     const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-    // three-input XOR in one v_bitop3_b32 (truth table 0x96)
-    c.x = __builtin_amdgcn_bitop3_b32(hi1, c.y, k0, 0x96);
+    if (r < 3) {
+      // Early rounds often see wave-uniform words (a step counter, the
+      // option index, the keys): plain XORs let the uniform parts go to the
+      // scalar unit instead of being copied into vector registers.
+      c.x = hi1 ^ (c.y ^ k0);
+      c.z = hi0 ^ (c.w ^ k1);
+    } else {
+      // three-input XOR in one v_bitop3_b32 (truth table 0x96)
+      c.x = __builtin_amdgcn_bitop3_b32(hi1, c.y, k0, 0x96);
+      c.z = __builtin_amdgcn_bitop3_b32(hi0, c.w, k1, 0x96);
+    }
     c.y = lo1;
-    c.z = __builtin_amdgcn_bitop3_b32(hi0, c.w, k1, 0x96);
     c.w = lo0;
     k0 += kPhiloxW0;
     k1 += kPhiloxW1;
   }
   return c;
 }
+
+// max(v, 0).  In FP32 one v_med3_f32 (clamp to [0, FLT_MAX]): a plain max of
+// a loop-carried value costs a second instruction that quiets signalling NaNs.
+__device__ __forceinline__ float positive_part(float v) { return __builtin_amdgcn_fmed3f(v, 0.f, 3.40282347e38f); }
+__device__ __forceinline__ double positive_part(double v) { return v > 0.0 ? v : 0.0; }
 
 // Counter word w: tag in bits 28..31, caller stream id in bits 0..27.
 __host__ __device__ __forceinline__ uint32_t counter_w(uint32_t tag, uint32_t stream) {

@@ -43,7 +43,7 @@ struct HestonConsts {
 template <typename Real, bool kVarAsVol>
 __device__ __forceinline__ void heston_step(const HestonConsts<Real>& c, Real& S, Real& v, Real z1, Real z2) {
 #pragma clang fp contract(off)
-  const Real vp = v > Real(0) ? v : Real(0);
+  const Real vp = positive_part(v);
   const Real sq = fsqrt(vp);
   const Real sv = kVarAsVol ? vp : sq;
   const Real sv2 = kVarAsVol ? vp * vp : vp;
