@@ -72,25 +72,27 @@ __host__ __device__ __forceinline__ uint32_t counter_w(uint32_t tag, uint32_t st
 }
 
 // ---------------------------------------------------------------------------
-// Box-Muller.  u1 = ((a >> 8) + 1) * 2^-24 in (0, 1], u2 = (b >> 8) * 2^-24
-// in [0, 1): both are exact in FP32 and FP64, so the two precisions see the
-// same uniforms.  z0 = r cos(2 pi u2), z1 = r sin(2 pi u2), r = sqrt(-2 ln u1).
+// Box-Muller.  u1 = fl(a | 1) 2^-32 in (0, 1], u2 = fl(b) 2^-32 in [0, 1],
+// with fl() the round-to-nearest-even conversion of a 32-bit word to FP32:
+// one conversion each (the low bit of a keeps u1 off zero), and both values
+// are exact in FP32 and FP64, so the two precisions see the same uniforms.
+// z0 = r cos(2 pi u2), z1 = r sin(2 pi u2), r = sqrt(-2 ln u1) <= 6.66.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float& z0, float& z1) {
-  // u1 = f 2^-24 with f = (a >> 8) + 1 exact; v_log_f32 is log2, so
-  // -2 ln u1 = -2 ln2 (log2 f - 24) = fma(-2 ln2, log2 f, 48 ln2).
-  const float f = (float)((a >> 8) + 1u);
-  const float u2 = (float)(b >> 8) * 0x1.0p-24f;
+  // u1 = f 2^-32 with f = fl(a | 1); v_log_f32 is log2, so
+  // -2 ln u1 = -2 ln2 (log2 f - 32) = fma(-2 ln2, log2 f, 64 ln2).
+  const float f = (float)(a | 1u);
+  const float u2 = (float)b * 0x1.0p-32f;
   const float r = __builtin_amdgcn_sqrtf(__builtin_fmaf(-1.3862943611198906f, __builtin_amdgcn_logf(f),
-                                                       33.271064666877376f));
+                                                       44.361419555836500f));
   // v_sin_f32 / v_cos_f32 take their argument in revolutions.
   z0 = r * __builtin_amdgcn_cosf(u2);
   z1 = r * __builtin_amdgcn_sinf(u2);
 }
 
 __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double& z0, double& z1) {
-  const double u1 = (double)((a >> 8) + 1u) * 0x1.0p-24;
-  const double u2 = (double)(b >> 8) * 0x1.0p-24;
+  const double u1 = (double)(float)(a | 1u) * 0x1.0p-32;
+  const double u2 = (double)(float)b * 0x1.0p-32;
   const double r = sqrt(-2.0 * log(u1));
   double s, c;
   sincospi(2.0 * u2, &s, &c);

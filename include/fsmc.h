@@ -19,8 +19,9 @@
  * Philox contract (mirrored bit for bit by the CPU oracle): key =
  * (seed & 0xffffffff, seed >> 32); counter = (c0, c1, c2, tag << 28 ^ stream)
  * with tag 1 GBM, 2 Heston, 3 LIBOR, 4 LSMC.  Each call yields four words
- * (a, b, c, d) -> Box-Muller pairs (a, b) and (c, d): u1 = ((w1 >> 8) + 1)
- * 2^-24, u2 = (w2 >> 8) 2^-24, z = sqrt(-2 ln u1) (cos 2 pi u2, sin 2 pi u2).
+ * (a, b, c, d) -> Box-Muller pairs (a, b) and (c, d): u1 = fl(w1 | 1) 2^-32,
+ * u2 = fl(w2) 2^-32 with fl the round-to-nearest-even conversion of the
+ * 32-bit word to float, z = sqrt(-2 ln u1) (cos 2 pi u2, sin 2 pi u2).
  *   GBM     c0 = path / 4,  c1 = 0,        c2 = option; normal j -> path 4 c0 + j
  *   Heston  c0 = path,      c1 = step / 2, c2 = option; (z1, z2) of step 2 c1
  *           are normals 0, 1 and of step 2 c1 + 1 normals 2, 3

@@ -49,8 +49,9 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 }
 
 void orc_box_muller(uint32_t a, uint32_t b, double* z0, double* z1) {
-  const double u1 = (double)((a >> 8) + 1u) * 0x1.0p-24;
-  const double u2 = (double)(b >> 8) * 0x1.0p-24;
+  /* fl(): round-to-nearest-even conversion of the 32-bit word to float */
+  const double u1 = (double)(float)(a | 1u) * 0x1.0p-32;
+  const double u2 = (double)(float)b * 0x1.0p-32;
   const double r = sqrt(-2.0 * log(u1));
   *z0 = r * cos(2.0 * ORC_PI * u2);
   *z1 = r * sin(2.0 * ORC_PI * u2);

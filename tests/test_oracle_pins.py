@@ -28,15 +28,27 @@ def test_philox_kat(ctr, key, expect):
 
 
 def test_box_muller_identities():
-    # u2 = 0 -> angle 0: z0 = r, z1 = 0; u1 = 1 (a >> 8 == 2^24 - 1) -> r = 0.
+    # u2 = 0 -> angle 0: z0 = r, z1 = 0; a = 0 -> u1 = fl(1) 2^-32, the
+    # smallest uniform; a = 2^32 - 1 rounds up to u1 = 1 -> r = 0.
     z0, z1 = orc.box_muller(0, 0)
-    assert z1 == 0.0 and z0 == pytest.approx(math.sqrt(-2 * math.log(2.0 ** -24)), rel=1e-15)
+    assert z1 == 0.0 and z0 == pytest.approx(math.sqrt(-2 * math.log(2.0 ** -32)), rel=1e-15)
     z0, z1 = orc.box_muller(0xFFFFFFFF, 0x40000000)  # u1 = 1, u2 = 1/4
     assert z0 == 0.0 and z1 == 0.0
-    # u2 = 1/4 -> (cos, sin) = (0, 1): all radius in z1.
+    # u2 = 1/4 -> (cos, sin) = (0, 1): all radius in z1.  0x7FFFFFFF rounds
+    # to 2^31 in FP32 (24-bit significand), so u1 = 1/2.
     z0, z1 = orc.box_muller(0x7FFFFFFF, 0x40000000)
-    r = math.sqrt(-2 * math.log(((0x7FFFFFFF >> 8) + 1) * 2.0 ** -24))
+    r = math.sqrt(2 * math.log(2.0))
     assert abs(z0) < 1e-15 * r and z1 == pytest.approx(r, rel=1e-15)
+    # below 2^24 the conversion is exact: u1 = (a | 1) 2^-32
+    z0, _ = orc.box_muller(0x00ABCDEE, 0)
+    assert z0 == pytest.approx(math.sqrt(-2 * math.log(0x00ABCDEF * 2.0 ** -32)), rel=1e-15)
+    # above it, round to nearest even on 24 significant bits: 2^24 + 1 is a
+    # tie between 2^24 and 2^24 + 2 and goes to the even significand 2^24
+    z0, _ = orc.box_muller(2 ** 24, 0)
+    assert z0 == pytest.approx(math.sqrt(-2 * math.log(2.0 ** -8)), rel=1e-15)
+    # u2 = fl(b) 2^-32: b = 2^32 - 1 rounds to one full revolution
+    z0, z1 = orc.box_muller(0, 0xFFFFFFFF)
+    assert z0 == pytest.approx(math.sqrt(64 * math.log(2.0)), rel=1e-15) and abs(z1) < 1e-14
 
 
 def test_normals_moments():
