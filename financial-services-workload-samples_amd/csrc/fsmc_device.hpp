@@ -179,6 +179,49 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
   }
 }
 
+// FP32 wave sum: four adds with a DPP operand inside each row of 16 lanes
+// (xor 1, xor 2 by quad permutes, then the half-row and row mirrors), then
+// two cross-row shuffles.  Every lane ends with the total; the tree keeps the
+// rounding of a 64-term sum within 6 ulps.
+template <int kCtrl>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), kCtrl, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float wave_sum(float v) {
+  v += dpp_f32<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
+  v += dpp_f32<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
+  v += dpp_f32<0x141>(v);  // row_half_mirror: lane i <-> 7 - i within 8
+  v += dpp_f32<0x140>(v);  // row_mirror: lane i <-> 15 - i within 16
+  v += __shfl_xor(v, 16, 64);
+  v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+// Workgroup sum of NV FP32 thread values: each wave sums in FP32 (above),
+// the wave totals are combined in FP64.  Result (FP64) valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(const float (&v)[NV], double (&out)[NV], double* lds) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  float w[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) w[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lds[k * nwaves + wave] = (double)w[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double t = 0.0;
+      for (int i = 0; i < nwaves; ++i) t += lds[k * nwaves + i];
+      out[k] = t;
+    }
+  }
+}
+
 // Precision-dependent exponential and division.  FP32 uses the hardware
 // v_exp_f32 (base 2, so exponents are pre-scaled by kE = log2 e) and
 // v_rcp_f32; FP64 uses the full-precision library exp and IEEE division, so

@@ -19,8 +19,8 @@
 //     in-the-money paths of date d-1.  x = 1 + (S/K - 1)/(sigma sqrt(t)) is
 //     an affine map of S/K (same fitted values as the plain monomials of S/K)
 //     that keeps the Gram matrix well conditioned.  Each lane accumulates
-//     the seven moments and four right-hand sides of its own paths in FP64
-//     (FP64 FMA issues at the FP32 rate here, and a 4x4 Gram matrix is too
+//     the seven moments and four right-hand sides of its own paths in the
+//     working precision, the workgroup combines them in FP64 (a 4x4 Gram matrix is too
 //     small for matrix cores: a v_mfma_f32_4x4x1f32 formulation needs the
 //     path's basis values spread over four lanes and cost ~117 instructions
 //     per 64 paths against ~14 for per-lane FMAs).  Each workgroup writes its
@@ -155,10 +155,12 @@ __global__ __launch_bounds__(kBlock) void lsmc_init_kernel(LsmcArgs a) {
   if (threadIdx.x == 0) a.eur[blockIdx.x] = eu[0];
 }
 
-// Moments of one path for the FP64 (VALU) accumulation.
-__device__ __forceinline__ void accum_valu(double (&s)[kNS], bool itm, double x, double y) {
-  if (!itm) return;
-  double xp = 1.0;
+// Moments of one path: s[q] += x^q (q = 0..6), s[7 + q] += x^q y (q = 0..3)
+// for an in-the-money path; other paths add exact zeros (branch-free).
+constexpr int kNM = 11;
+template <typename Real>
+__device__ __forceinline__ void accum_moments(Real (&s)[kNM], bool itm, Real x, Real y) {
+  Real xp = itm ? Real(1) : Real(0);
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
     s[q] += xp;
@@ -261,9 +263,14 @@ __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
   const Real isw_k = k >= 1 ? (Real)(1.0 / (m.sigma * sqrt(k * dt))) : Real(0);
   // bridge from date d back to k = d - 1: W_k = k/d W_d + sqrt(dt k/d) Z_{k-1}
   const Real ba = (Real)((double)k / d), bb = (Real)sqrt(dt * k / d);
-  double s[kNS];
+  // A thread's own paths (at most kPPT) are summed in the working precision.
+  // In FP32 the thread sums (<= 16 terms) and the wave trees over them
+  // (64 terms, 6 roundings) carry ~1e-6 relative rounding, independent
+  // across waves, which are combined in FP64: the option's moments keep
+  // ~1e-8.  FP64 runs are summed in FP64 throughout.
+  Real sm[kNM];
 #pragma unroll
-  for (int q = 0; q < kNS; ++q) s[q] = 0.0;
+  for (int q = 0; q < kNM; ++q) sm[q] = Real(0);
 #pragma unroll
   for (int h = 0; h < kGroups; ++h) {
     const int l0 = 4 * (h * kBlock + (int)threadIdx.x);
@@ -294,13 +301,24 @@ __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
         const Real Sk = S0 * O::ex(mu_k + sg * Wk);
         const bool itm = live && K - Sk > Real(0);
         const Real x = itm ? Real(1) + (Sk * invK - Real(1)) * isw_k : Real(0);
-        accum_valu(s, itm, (double)x, (double)c);
+        accum_moments(sm, itm, x, c);
       }
     }
     if (k >= 1) store4(wst + l0, wk);  // padding entries too (never read as paths)
   }
   if (k < 1) return;  // last pass of this option: decision only
-  block_sum<kNS>(s, lds);
+  double s[kNS];
+  if constexpr (sizeof(Real) == 4) {
+    // FP32 thread sums: FP32 wave trees (6 roundings), FP64 from the wave totals on
+    double t[kNM];
+    block_sum<kNM>(sm, t, lds);
+#pragma unroll
+    for (int q = 0; q < kNS; ++q) s[q] = q < kNM ? t[q] : 0.0;
+  } else {
+#pragma unroll
+    for (int q = 0; q < kNS; ++q) s[q] = q < kNM ? (double)sm[q] : 0.0;
+    block_sum<kNS>(s, lds);
+  }
   if (threadIdx.x == 0) {
     double* dst = a.partials + (size_t)blockIdx.x * kNS;
     for (int q = 0; q < kNS; ++q) dst[q] = s[q];
