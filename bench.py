@@ -2,8 +2,9 @@
 
 Workload (one "step" = one pricing of the whole batch, inputs resident in
 HBM): Heston arithmetic-average Asian calls, 64 options x 131072 paths x 365
-full-truncation Euler steps, FP32 path arithmetic with compensated FP32 sums
-and an FP64 combine, Philox4x32-10 normals generated in-kernel (S0~U[80,120],
+full-truncation Euler steps, FP32 path arithmetic with the per-path time
+sums and the estimator sums in FP64, Philox4x32-10 normals generated
+in-kernel (S0~U[80,120],
 K=100, T=1, r=0.03, kappa=2, theta=0.04, xi=0.3, rho=-0.7, v0=0.04).
 
 Prints one JSON line: path-steps/s, the roofline of the path kernel against
@@ -58,9 +59,10 @@ def _params(np):
 
 
 def _profile_figures():
-    """HBM bytes and VALU wavefront instructions per launch of the path
-    kernel, from the committed rocprofv3 summary of this workload
-    (FETCH_SIZE + WRITE_SIZE scaled by a plain copy; SQ_INSTS_VALU)."""
+    """HBM bytes, VALU wavefront instructions and counted FP32 FLOP per
+    launch of the path kernel, from the committed rocprofv3 summary of this
+    workload (FETCH_SIZE + WRITE_SIZE scaled by a plain copy; SQ_INSTS_VALU;
+    64 lanes x (ADD + MUL + 2 FMA + TRANS) of SQ_INSTS_VALU_*_F32)."""
     path = os.path.join(ROOT, "profiles", "heston_asian_fp32_64x131072x365_traffic.txt")
     vals = {}
     if os.path.exists(path):
@@ -74,7 +76,7 @@ def _profile_figures():
             return float(vals[k])
         except (KeyError, ValueError):
             return None
-    return num("bytes_per_launch"), num("valu_insts_per_launch")
+    return num("bytes_per_launch"), num("valu_insts_per_launch"), num("fp32_flop_per_launch")
 
 
 def _cpu_baseline(np, P, target_s):
@@ -291,6 +293,7 @@ def main():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--workload", choices=["heston", "heston_adjoint", "copy"] + list(CONFIGS),
                     help="profiling aid: run only this workload in FP32 for --steps launches and print its timing")
+    ap.add_argument("--fp64", action="store_true", help="with --workload: run the plain FP64 path as well")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every path shards by "
@@ -303,7 +306,7 @@ def main():
     import mcengine as mc
 
     if args.workload in CONFIGS:
-        print(json.dumps(_other_configs(np, torch, mc, only=args.workload, steps=args.steps, fp64=False)),
+        print(json.dumps(_other_configs(np, torch, mc, only=args.workload, steps=args.steps, fp64=args.fp64)),
               flush=True)
         return
     if args.workload == "copy":
@@ -341,9 +344,12 @@ def main():
         print(json.dumps({"workload": "heston_adjoint", "ms": msg, "kernel_ms": kg}), flush=True)
         return
     ms, kern_ms, p32, c32 = run(m32, args.steps, args.warmup)
-    traffic, valu = _profile_figures()
+    traffic, valu, counted_flop = _profile_figures()
     if args.workload == "heston":
-        print(json.dumps({"workload": "heston", "ms": ms, "kernel_ms": kern_ms}), flush=True)
+        rec = {"workload": "heston", "ms": ms, "kernel_ms": kern_ms}
+        if args.fp64:
+            rec["ms_fp64"], rec["kernel_ms_fp64"], _, _ = run(mc.mode(mc.FP64, seed=1), args.steps, args.warmup)
+        print(json.dumps(rec), flush=True)
         return
     path_steps = N_OPT * N_PATHS * N_STEPS
     value = path_steps / (ms * 1e-3)
@@ -359,7 +365,8 @@ def main():
                      "kernel": "heston_paths_kernel<float> (+finalize)", "kernel_ms": kern_ms,
                      "flop_per_path_step": FLOP_PER_PATH_STEP,
                      # issue ceiling: one wave64 VALU instruction per SIMD per 4 cycles
-                     "valu_issue_frac": (valu / (kern_ms * 1e-3) / VALU_ISSUE_PEAK) if valu else None},
+                     "valu_issue_frac": (valu / (kern_ms * 1e-3) / VALU_ISSUE_PEAK) if valu else None,
+                     "counted_tflops": (counted_flop / (kern_ms * 1e-3) / 1e12) if counted_flop else None},
     }
     if not args.no_secondary:
         ms64, k64, p64, c64 = run(mc.mode(mc.FP64, seed=1), 3, 1)
