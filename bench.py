@@ -237,14 +237,15 @@ def _batch_config(np, torch, mc, eng, precisions, steps):
         tune_s = time.perf_counter() - t0
         ms = _timed(torch, np, lambda: b.run(mc._stream()), steps, 1)
         b.close()
-        lb = mc.Batch(eng, reqs, m, tuning=(tuning[0], 256))
+        lb = mc.Batch(eng, reqs, m, tuning=(tuning[0], 256, tuning[2]))
         lms = _timed(torch, np, lambda: lb.run(mc._stream()), steps, 1)
         lat = lb.results(mc._stream())["latency_us"] / 1e3
         lb.close()
         out.update({"ms_" + name: ms, "requests_per_s_" + name: len(reqs) / (ms * 1e-3),
                     "path_steps_per_s_" + name: ps / (ms * 1e-3),
                     "path_greeks_per_s_" + name: 3 * npaths / (ms * 1e-3),
-                    "tuning_" + name: {"chunk_paths": tuning[0], "requests_per_launch": tuning[1]},
+                    "tuning_" + name: {"chunk_paths": tuning[0], "requests_per_launch": tuning[1],
+                                       "replay_graph": tuning[2]},
                     "autotune_s_" + name: tune_s,
                     "microbatch256_ms_" + name: lms,
                     "latency_p50_ms_" + name: float(np.percentile(lat, 50)),

@@ -9,6 +9,13 @@
 // partials in item order (bitwise reproducible) into mean and standard error.
 // A HIP event after each micro-batch gives the per-request completion time;
 // an autotuner times chunk size x micro-batch size and keeps the fastest.
+// With replay_graph the whole launch sequence is captured once per tuning
+// into a hipGraph and replayed by every run: one launch from the host per
+// run whatever the micro-batch count.  Completion times then come from a
+// one-thread kernel after each micro-batch that stores the GPU's constant-
+// rate wall clock (HIP events recorded by graph nodes cannot be timed
+// against each other here).  No node clears memory (every partial, result
+// and stamp is written before it is read), so replays need no reset.
 // A request's normals and per-path arithmetic are those of the standalone
 // entry point with the request index as the option index.
 #include <algorithm>
@@ -115,6 +122,12 @@ __global__ __launch_bounds__(kBlock) void batch_heston_kernel(BatchArgs a, int f
   }
 }
 
+// Wall-clock stamp i (graph mode: i = 0 before the first micro-batch, w + 1
+// after micro-batch w).
+__global__ void batch_stamp_kernel(unsigned long long* stamps, int i) {
+  if (threadIdx.x == 0) stamps[i] = wall_clock64();
+}
+
 // One thread per request of the micro-batch [r0, r0 + n): its items are
 // first_item[r] .. first_item[r] + n_items[r] - 1 (contiguous per kind).
 __global__ void batch_finalize_kernel(const ReqDev* req, const int* first_item, const int* n_items,
@@ -151,9 +164,15 @@ struct BatchImpl {
   std::vector<MicroBatch> waves;
   int max_hes_grid = 0;
   DeviceBuffer ws;
-  size_t off_req = 0, off_items = 0, off_first = 0, off_nitems = 0, off_part = 0, off_res = 0, off_ckpt = 0;
-  std::vector<hipEvent_t> done;  // one per micro-batch
+  size_t off_req = 0, off_items = 0, off_first = 0, off_nitems = 0, off_part = 0, off_res = 0, off_ckpt = 0,
+         off_stamp = 0;
+  std::vector<hipEvent_t> done;  // one per micro-batch (stream mode)
   hipEvent_t start = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;  // autotuner's timing around a run
+  double wall_khz = 0.0;
+  hipStream_t capture = nullptr;  // private stream the graph is captured on
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
 };
 
 static std::map<std::vector<long long>, fsmc_batch_tuning>& tuning_cache() {
@@ -181,8 +200,34 @@ static std::vector<long long> batch_key(const BatchImpl& b) {
 }
 
 static void batch_release_events(BatchImpl& b) {
+  if (b.exec) (void)hipGraphExecDestroy(b.exec);
+  if (b.graph) (void)hipGraphDestroy(b.graph);
+  b.exec = nullptr;
+  b.graph = nullptr;
   for (hipEvent_t e : b.done) (void)hipEventDestroy(e);
   b.done.clear();
+}
+
+template <typename Real>
+static int batch_enqueue(BatchImpl& b, hipStream_t s, bool stamps);
+
+// Captures the launch sequence of the current tuning into b.exec.
+static int batch_capture(BatchImpl& b) {
+  if (!b.capture) FSMC_HIP(hipStreamCreateWithFlags(&b.capture, hipStreamNonBlocking));
+  FSMC_HIP(hipStreamBeginCapture(b.capture, hipStreamCaptureModeThreadLocal));
+  const int st = b.mode.precision == FSMC_FP64 ? batch_enqueue<double>(b, b.capture, true)
+                                               : batch_enqueue<float>(b, b.capture, true);
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(b.capture, &g);
+  if (st != FSMC_OK || ce != hipSuccess || g == nullptr) {
+    if (g) (void)hipGraphDestroy(g);
+    if (st != FSMC_OK) return st;
+    set_error("graph capture of the batch failed: %s", hipGetErrorString(ce));
+    return FSMC_ERR_DEVICE;
+  }
+  b.graph = g;
+  FSMC_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+  return FSMC_OK;
 }
 
 // Builds the item tables and workspace for a tuning (caller holds ctx->mu).
@@ -190,6 +235,7 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
   FSMC_REQUIRE(t.chunk_paths >= 4 * kBlock && t.chunk_paths % (4 * kBlock) == 0,
                "chunk_paths %d must be a positive multiple of %d", t.chunk_paths, 4 * kBlock);
   FSMC_REQUIRE(t.requests_per_launch >= 0, "requests_per_launch %d < 0", t.requests_per_launch);
+  FSMC_REQUIRE(t.replay_graph == 0 || t.replay_graph == 1, "replay_graph %d is not 0 or 1", t.replay_graph);
   FSMC_HIP(hipSetDevice(ctx->device));
   FSMC_HIP(hipStreamSynchronize(ctx->stream));
   batch_release_events(b);
@@ -263,6 +309,7 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
   b.off_part = take(sizeof(double) * kNV * items.size());
   b.off_res = take(sizeof(double) * 8 * b.n);
   b.off_ckpt = take(real * 2 * (size_t)b.max_seg * (size_t)std::max(1, b.max_hes_grid) * kBlock);
+  b.off_stamp = take(sizeof(unsigned long long) * (b.waves.size() + 1));
   FSMC_TRY(b.ws.reserve(off));
   char* ws = (char*)b.ws.ptr;
   hipStream_t s = ctx->stream;
@@ -271,13 +318,17 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
   FSMC_HIP(hipMemcpyAsync(ws + b.off_first, first.data(), sizeof(int) * b.n, hipMemcpyHostToDevice, s));
   FSMC_HIP(hipMemcpyAsync(ws + b.off_nitems, nitems.data(), sizeof(int) * b.n, hipMemcpyHostToDevice, s));
   FSMC_HIP(hipStreamSynchronize(s));
-  b.done.resize(b.waves.size(), nullptr);
-  for (auto& e : b.done) FSMC_HIP(hipEventCreate(&e));
+  if (t.replay_graph) {
+    FSMC_TRY(batch_capture(b));
+  } else {
+    b.done.resize(b.waves.size(), nullptr);
+    for (auto& e : b.done) FSMC_HIP(hipEventCreate(&e));
+  }
   return FSMC_OK;
 }
 
 template <typename Real>
-static int batch_enqueue(BatchImpl& b, hipStream_t s) {
+static int batch_enqueue(BatchImpl& b, hipStream_t s, bool stamps) {
   char* ws = (char*)b.ws.ptr;
   BatchArgs a;
   a.req = (const ReqDev*)(ws + b.off_req);
@@ -289,7 +340,11 @@ static int batch_enqueue(BatchImpl& b, hipStream_t s) {
   a.k1 = (uint32_t)(b.mode.seed >> 32);
   a.w_gbm = counter_w(kTagGbm, b.mode.stream);
   a.w_heston = counter_w(kTagHeston, b.mode.stream);
-  FSMC_HIP(hipEventRecord(b.start, s));
+  unsigned long long* stamp = (unsigned long long*)(ws + b.off_stamp);
+  if (stamps)
+    hipLaunchKernelGGL(batch_stamp_kernel, dim3(1), dim3(64), 0, s, stamp, 0);
+  else
+    FSMC_HIP(hipEventRecord(b.start, s));
   for (size_t w = 0; w < b.waves.size(); ++w) {
     const MicroBatch& mb = b.waves[w];
     if (mb.gbm_count > 0)
@@ -299,8 +354,10 @@ static int batch_enqueue(BatchImpl& b, hipStream_t s) {
     hipLaunchKernelGGL(batch_finalize_kernel, dim3((mb.n + 63) / 64), dim3(64), 0, s, a.req,
                        (const int*)(ws + b.off_first), (const int*)(ws + b.off_nitems), a.partials, mb.r0, mb.n,
                        (double*)(ws + b.off_res));
+    if (stamps)
+      hipLaunchKernelGGL(batch_stamp_kernel, dim3(1), dim3(64), 0, s, stamp, (int)w + 1);
     FSMC_HIP(hipGetLastError());
-    FSMC_HIP(hipEventRecord(b.done[w], s));
+    if (!stamps) FSMC_HIP(hipEventRecord(b.done[w], s));
   }
   return FSMC_OK;
 }
@@ -345,7 +402,7 @@ int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_
   bi.mode = *m;
   bi.req.assign(req, req + n);
   bi.max_seg = max_seg;
-  fsmc_batch_tuning t = {4096, 0};
+  fsmc_batch_tuning t = {4096, 0, 0};
   if (tuning) {
     t = *tuning;
   } else {
@@ -356,8 +413,14 @@ int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_
   int st;
   {
     std::lock_guard<std::mutex> lock(ctx->mu);
-    st = hipSetDevice(ctx->device) == hipSuccess && hipEventCreate(&bi.start) == hipSuccess ? FSMC_OK
-                                                                                            : FSMC_ERR_DEVICE;
+    int khz = 0;
+    st = hipSetDevice(ctx->device) == hipSuccess && hipEventCreate(&bi.start) == hipSuccess &&
+                 hipEventCreate(&bi.t0) == hipSuccess && hipEventCreate(&bi.t1) == hipSuccess &&
+                 hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) == hipSuccess && khz > 0
+             ? FSMC_OK
+             : FSMC_ERR_DEVICE;
+    if (st != FSMC_OK) set_error("event creation or wall-clock rate query failed");
+    bi.wall_khz = khz;
     if (st == FSMC_OK) st = batch_configure(ctx, bi, t);
   }
   if (st != FSMC_OK) {
@@ -372,7 +435,12 @@ int fsmc_batch_run(fsmc_batch* b, void* stream) {
   FSMC_REQUIRE(b != nullptr, "batch is NULL");
   FSMC_HIP(hipSetDevice(b->ctx->device));
   hipStream_t s = stream ? (hipStream_t)stream : b->ctx->stream;
-  return b->impl.mode.precision == FSMC_FP64 ? batch_enqueue<double>(b->impl, s) : batch_enqueue<float>(b->impl, s);
+  if (b->impl.exec) {
+    FSMC_HIP(hipGraphLaunch(b->impl.exec, s));
+    return FSMC_OK;
+  }
+  return b->impl.mode.precision == FSMC_FP64 ? batch_enqueue<double>(b->impl, s, false)
+                                             : batch_enqueue<float>(b->impl, s, false);
 }
 
 int fsmc_batch_results(fsmc_batch* b, void* stream, fsmc_result* out) {
@@ -383,12 +451,22 @@ int fsmc_batch_results(fsmc_batch* b, void* stream, fsmc_result* out) {
   FSMC_HIP(hipSetDevice(b->ctx->device));
   FSMC_HIP(hipMemcpyAsync(res.data(), (char*)bi.ws.ptr + bi.off_res, res.size() * sizeof(double),
                           hipMemcpyDeviceToHost, s));
+  std::vector<unsigned long long> stamp(bi.exec ? bi.waves.size() + 1 : 0);
+  if (bi.exec)
+    FSMC_HIP(hipMemcpyAsync(stamp.data(), (char*)bi.ws.ptr + bi.off_stamp, stamp.size() * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, s));
   FSMC_HIP(hipStreamSynchronize(s));
   for (size_t w = 0; w < bi.waves.size(); ++w) {
-    float ms = 0.0f;
-    FSMC_HIP(hipEventElapsedTime(&ms, bi.start, bi.done[w]));
+    double us = 0.0;
+    if (bi.exec) {
+      us = (double)(stamp[w + 1] - stamp[0]) / bi.wall_khz * 1e3;
+    } else {
+      float ms = 0.0f;
+      FSMC_HIP(hipEventElapsedTime(&ms, bi.start, bi.done[w]));
+      us = 1e3 * ms;
+    }
     const MicroBatch& mb = bi.waves[w];
-    for (int i = mb.r0; i < mb.r0 + mb.n; ++i) out[i].latency_us = 1e3 * ms;
+    for (int i = mb.r0; i < mb.r0 + mb.n; ++i) out[i].latency_us = us;
   }
   for (int i = 0; i < bi.n; ++i) {
     fsmc_result& r = out[i];
@@ -405,6 +483,32 @@ int fsmc_batch_results(fsmc_batch* b, void* stream, fsmc_result* out) {
   return FSMC_OK;
 }
 
+}  // extern "C"
+
+// Configures tuning t and returns the best of two timed runs (after one
+// untimed run) from the batch's start event to its last completion event.
+static int batch_time(fsmc_batch* b, fsmc_batch_tuning t, float* best) {
+  BatchImpl& bi = b->impl;
+  {
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    FSMC_TRY(batch_configure(b->ctx, bi, t));
+  }
+  *best = 1e30f;
+  hipStream_t s = b->ctx->stream;
+  for (int rep = 0; rep < 3; ++rep) {
+    FSMC_HIP(hipEventRecord(bi.t0, s));
+    FSMC_TRY(fsmc_batch_run(b, nullptr));
+    FSMC_HIP(hipEventRecord(bi.t1, s));
+    FSMC_HIP(hipEventSynchronize(bi.t1));
+    float ms = 0.0f;
+    FSMC_HIP(hipEventElapsedTime(&ms, bi.t0, bi.t1));
+    if (rep > 0) *best = std::min(*best, ms);  // first run untimed
+  }
+  return FSMC_OK;
+}
+
+extern "C" {
+
 int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen) {
   FSMC_REQUIRE(b != nullptr, "batch is NULL");
   BatchImpl& bi = b->impl;
@@ -417,33 +521,23 @@ int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen) {
   for (int c : chunks) {
     for (int per : pers) {
       if (per >= bi.n) continue;
-      {
-        std::lock_guard<std::mutex> lock(ctx->mu);
-        st = batch_configure(ctx, bi, fsmc_batch_tuning{c, per});
-      }
+      float ms = 0.0f;
+      st = batch_time(b, fsmc_batch_tuning{c, per, 0}, &ms);
       if (st != FSMC_OK) break;
-      float ms_best = 1e30f;
-      for (int rep = 0; rep < 3 && st == FSMC_OK; ++rep) {
-        st = fsmc_batch_run(b, nullptr);
-        if (st != FSMC_OK) break;
-        if (hipEventSynchronize(bi.done.back()) != hipSuccess) {
-          set_error("autotune run failed");
-          st = FSMC_ERR_DEVICE;
-          break;
-        }
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, bi.start, bi.done.back());
-        if (rep > 0) ms_best = std::min(ms_best, ms);  // first run untimed
-      }
-      if (st != FSMC_OK) break;
-      if (ms_best < best_ms) {
-        best_ms = ms_best;
-        best = fsmc_batch_tuning{c, per};
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = fsmc_batch_tuning{c, per, 0};
       }
     }
     if (st != FSMC_OK) break;
   }
   if (st != FSMC_OK) return st;
+  {
+    float ms = 0.0f;
+    const fsmc_batch_tuning g = {best.chunk_paths, best.requests_per_launch, 1};
+    FSMC_TRY(batch_time(b, g, &ms));
+    if (ms < best_ms) best = g;
+  }
   {
     std::lock_guard<std::mutex> lock(ctx->mu);
     FSMC_TRY(batch_configure(ctx, bi, best));
@@ -466,8 +560,12 @@ int fsmc_batch_destroy(fsmc_batch* b) {
   if (!b) return FSMC_OK;
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
+  if (b->impl.capture) (void)hipStreamSynchronize(b->impl.capture);
   batch_release_events(b->impl);
   if (b->impl.start) (void)hipEventDestroy(b->impl.start);
+  if (b->impl.t0) (void)hipEventDestroy(b->impl.t0);
+  if (b->impl.t1) (void)hipEventDestroy(b->impl.t1);
+  if (b->impl.capture) (void)hipStreamDestroy(b->impl.capture);
   b->impl.ws.release();
   delete b;
   return FSMC_OK;

@@ -26,6 +26,7 @@ void set_error(const char* fmt, ...);
     if (fsmc_e_ != hipSuccess) {                                                              \
       ::fsmc::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(fsmc_e_), __FILE__, \
                         __LINE__);                                                            \
+      (void)hipGetLastError(); /* reported here: not again by a later check */                \
       return FSMC_ERR_DEVICE;                                                                 \
     }                                                                                         \
   } while (0)

@@ -83,12 +83,40 @@ def test_mixed_batch_reruns_and_tuning(engine):
     b2.close()
     assert np.allclose(other["price"], first["price"], rtol=1e-9, atol=1e-12)
     chosen = b.autotune()
-    assert chosen[0] % 1024 == 0 and chosen[1] >= 0
+    assert chosen[0] % 1024 == 0 and chosen[1] >= 0 and chosen[2] in (0, 1)
     assert b.tuning() == chosen
     b.run()
     tuned = b.results()
     b.close()
     assert np.allclose(tuned["price"], first["price"], rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("prec", [mc.FP32, mc.FP64])
+def test_graph_replay_matches_stream_launches(engine, prec):
+    # the captured launch sequence, replayed with eager work and
+    # synchronisations between replays, gives the stream-launched results
+    # bit for bit on every replay, and its completion events still time the
+    # micro-batches in order
+    import torch
+    reqs = mc.mixed_requests(64, seed=8)
+    m = mc.mode(prec, seed=6)
+    ref = mc.Batch(engine, reqs, m, tuning=(2048, 16, 0))
+    ref.run()
+    want = ref.results()
+    ref.close()
+    g = mc.Batch(engine, reqs, m, tuning=(2048, 16, 1))
+    assert g.tuning() == (2048, 16, 1)
+    for _ in range(4):
+        x = torch.randn(1 << 20, device="cuda")
+        torch.cuda.synchronize()
+        del x
+        g.run()
+        got = g.results()
+        for k in ("price", "delta", "vega", "rho", "price_se", "delta_se", "vega_se", "rho_se"):
+            assert np.array_equal(got[k], want[k]), k
+        lat = got["latency_us"]
+        assert np.all(lat > 0) and np.all(np.diff(lat[::16]) >= 0)
+    g.close()
 
 
 def test_mixed_batch_statistics(engine):
@@ -114,3 +142,5 @@ def test_batch_errors(engine):
         mc.Batch(engine, [mc.gbm_request(1, 10, 10, 1, .02, .3)], mc.mode())
     with pytest.raises(mc.FsmcError):
         mc.Batch(engine, [mc.gbm_request(100, 10, 10, 1, .02, .3)], mc.mode(), tuning=(1000, 0))
+    with pytest.raises(mc.FsmcError):
+        mc.Batch(engine, [mc.gbm_request(100, 10, 10, 1, .02, .3)], mc.mode(), tuning=(1024, 0, 2))
