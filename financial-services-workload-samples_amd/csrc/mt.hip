@@ -1,0 +1,254 @@
+// MKL-compatible MT19937 stream with Box-Muller normals on the GPU: the
+// generator the AsianOptionsPricing sample draws from
+// (AsianOptionsPricing/mkl_rng.h:5-69, MCAsianOptions.cpp:92-93,134-135:
+// vslNewStream(VSL_BRNG_MT19937, seed) and vdRngGaussian with
+// VSL_RNG_METHOD_GAUSSIAN_BOXMULLER2 on consecutive 8192-normal blocks).
+//
+// MT19937 (Matsumoto and Nishimura, ACM TOMACS 8(1), 1998): 624-word state,
+// seeded here by init_by_array({seed}) (that is what MKL's single-seed
+// MT19937 does: its first outputs for seed 42 are 2746317213, 478163327,
+// 107420369), one 32-bit tempered output per state word, the whole state
+// regenerated ("twisted") every 624 outputs.  Uniforms are u = bits * 2^-32;
+// Box-Muller pairs (u1, u2) of consecutive uniforms give
+//   z1 = sqrt(-2 ln u1) sin(2 pi u2),  z2 = sqrt(-2 ln u1) cos(2 pi u2)
+// (u1 = 0 is taken as 2^-33).  The raw outputs equal MKL's bit for bit; MKL's
+// own Box-Muller arithmetic is not reproducible exactly (its normals differ
+// from this formula by up to ~2e-8), so the normals match MKL's to that.
+//
+// The stream is sequential.  A generator object keeps the state in device
+// memory; each call continues it, as successive vdRngGaussian calls on one
+// VSL stream do.  The bits kernel is one workgroup: the state sits in LDS and
+// the twist runs as three dependent phases (words [0, 227) read old words
+// i + 397; [227, 454) read the words phase one wrote; [454, 624) those of
+// phase two, the last word also the new word 0), each a read into registers,
+// a barrier and a write.  The Box-Muller transform then runs over the bits
+// in a grid-wide kernel.
+#include <algorithm>
+#include <vector>
+
+#include "fsmc_device.hpp"
+#include "fsmc_host.hpp"
+
+namespace fsmc {
+namespace {
+
+constexpr int kN = 624, kM = 397;
+constexpr uint32_t kMatrixA = 0x9908b0dfu, kUpper = 0x80000000u, kLower = 0x7fffffffu;
+constexpr int kBitsThreads = 256;
+
+struct MtState {
+  uint32_t mt[kN];
+  uint32_t pos;  // next word to temper; kN = the state must be twisted first
+};
+
+__host__ __device__ __forceinline__ uint32_t twist_word(uint32_t a, uint32_t b, uint32_t far) {
+  const uint32_t y = (a & kUpper) | (b & kLower);
+  return far ^ (y >> 1) ^ ((y & 1u) ? kMatrixA : 0u);
+}
+
+__host__ __device__ __forceinline__ uint32_t temper(uint32_t y) {
+  y ^= y >> 11;
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  y ^= y >> 18;
+  return y;
+}
+
+// init_by_array({seed}) of the reference implementation.
+void init_by_seed(uint32_t seed, MtState& s) {
+  uint32_t* mt = s.mt;
+  mt[0] = 19650218u;
+  for (int i = 1; i < kN; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
+  int i = 1;
+  for (int k = kN; k > 0; --k) {  // key length 1: j stays 0
+    mt[i] = (mt[i] ^ ((mt[i - 1] ^ (mt[i - 1] >> 30)) * 1664525u)) + seed;
+    if (++i >= kN) {
+      mt[0] = mt[kN - 1];
+      i = 1;
+    }
+  }
+  for (int k = kN - 1; k > 0; --k) {
+    mt[i] = (mt[i] ^ ((mt[i - 1] ^ (mt[i - 1] >> 30)) * 1566083941u)) - (uint32_t)i;
+    if (++i >= kN) {
+      mt[0] = mt[kN - 1];
+      i = 1;
+    }
+  }
+  mt[0] = 0x80000000u;
+  s.pos = kN;
+}
+
+// Next n outputs of the stream in st into out[0, n); one workgroup.
+__global__ __launch_bounds__(kBitsThreads) void mt_bits_kernel(MtState* st, long long n, uint32_t* out) {
+  __shared__ uint32_t mt[kN];
+  const int t = threadIdx.x;
+  for (int i = t; i < kN; i += kBitsThreads) mt[i] = st->mt[i];
+  int pos = (int)st->pos;  // uniform
+  __syncthreads();
+  long long done = 0;
+  while (done < n) {
+    if (pos == kN) {
+      // phase 1: words [0, 227) from old words i, i + 1, i + 397
+      uint32_t v = 0;
+      if (t < kN - kM) v = twist_word(mt[t], mt[t + 1], mt[t + kM]);
+      __syncthreads();
+      if (t < kN - kM) mt[t] = v;
+      __syncthreads();
+      // phase 2: words [227, 454) from old i, i + 1 and new i - 227
+      const int i2 = t + (kN - kM);
+      if (t < kN - kM) v = twist_word(mt[i2], mt[i2 + 1], mt[i2 - (kN - kM)]);
+      __syncthreads();
+      if (t < kN - kM) mt[i2] = v;
+      __syncthreads();
+      // phase 3: words [454, 624) from old i, new i - 227, and for the last
+      // word the new word 0
+      const int i3 = t + 2 * (kN - kM);
+      if (i3 < kN) v = twist_word(mt[i3], mt[i3 + 1 < kN ? i3 + 1 : 0], mt[i3 - (kN - kM)]);
+      __syncthreads();
+      if (i3 < kN) mt[i3] = v;
+      __syncthreads();
+      pos = 0;
+    }
+    const int m = (int)std::min<long long>(kN - pos, n - done);
+    for (int i = t; i < m; i += kBitsThreads) out[done + i] = temper(mt[pos + i]);
+    pos += m;
+    done += m;
+  }
+  __syncthreads();
+  for (int i = t; i < kN; i += kBitsThreads) st->mt[i] = mt[i];
+  if (t == 0) st->pos = (uint32_t)pos;
+}
+
+// Box-Muller pairs over consecutive outputs (even count).
+__global__ __launch_bounds__(256) void mt_boxmuller_kernel(const uint32_t* bits, long long pairs, double mean,
+                                                           double stddev, double* out) {
+  for (long long j = (long long)blockIdx.x * 256 + threadIdx.x; j < pairs; j += (long long)gridDim.x * 256) {
+    const uint32_t a = bits[2 * j], b = bits[2 * j + 1];
+    const double u1 = a ? (double)a * 0x1.0p-32 : 0x1.0p-33;
+    const double u2 = (double)b * 0x1.0p-32;
+    const double r = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincospi(2.0 * u2, &s, &c);
+    out[2 * j] = mean + stddev * (r * s);
+    out[2 * j + 1] = mean + stddev * (r * c);
+  }
+}
+
+}  // namespace
+}  // namespace fsmc
+
+using namespace fsmc;
+
+struct fsmc_mt19937 {
+  fsmc_ctx* ctx;
+  MtState* state;         // device
+  DeviceBuffer bits;      // scratch of the normals calls
+};
+
+namespace {
+
+int mt_validate(long long n, const void* out) {
+  FSMC_REQUIRE(n >= 0 && n <= (1LL << 40), "n %lld outside [0, 2^40]", n);
+  FSMC_REQUIRE(n == 0 || out != nullptr, "NULL output");
+  return FSMC_OK;
+}
+
+int mt_bits_enqueue(fsmc_mt19937* g, long long n, uint32_t* d_out, hipStream_t s) {
+  if (n == 0) return FSMC_OK;
+  hipLaunchKernelGGL(mt_bits_kernel, dim3(1), dim3(kBitsThreads), 0, s, g->state, n, d_out);
+  FSMC_HIP(hipGetLastError());
+  return FSMC_OK;
+}
+
+int mt_gaussian_enqueue(fsmc_mt19937* g, long long n, double mean, double stddev, double* d_out, hipStream_t s) {
+  FSMC_REQUIRE(n % 2 == 0, "Box-Muller pairs: n %lld must be even", n);
+  if (n == 0) return FSMC_OK;
+  FSMC_TRY(g->bits.reserve((size_t)n * sizeof(uint32_t)));
+  uint32_t* bits = (uint32_t*)g->bits.ptr;
+  FSMC_TRY(mt_bits_enqueue(g, n, bits, s));
+  const long long pairs = n / 2;
+  const unsigned grid = (unsigned)std::min<long long>((pairs + 255) / 256, 8192);
+  hipLaunchKernelGGL(mt_boxmuller_kernel, dim3(grid), dim3(256), 0, s, bits, pairs, mean, stddev, d_out);
+  FSMC_HIP(hipGetLastError());
+  return FSMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsmc_mt19937_create(fsmc_ctx* ctx, uint32_t seed, fsmc_mt19937** out) {
+  FSMC_REQUIRE(ctx && out, "NULL argument");
+  *out = nullptr;
+  MtState h;
+  init_by_seed(seed, h);
+  fsmc_mt19937* g = new fsmc_mt19937();
+  g->ctx = ctx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess || hipMalloc((void**)&g->state, sizeof(MtState)) != hipSuccess) {
+    (void)hipGetLastError();
+    delete g;
+    set_error("MT19937 state allocation failed");
+    return FSMC_ERR_DEVICE;
+  }
+  if (hipMemcpy(g->state, &h, sizeof(MtState), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(g->state);
+    delete g;
+    set_error("MT19937 state upload failed");
+    return FSMC_ERR_DEVICE;
+  }
+  *out = g;
+  return FSMC_OK;
+}
+
+int fsmc_mt19937_bits_async(fsmc_mt19937* g, long long n, uint32_t* d_out, void* stream) {
+  FSMC_REQUIRE(g != nullptr, "generator is NULL");
+  FSMC_TRY(mt_validate(n, d_out));
+  return mt_bits_enqueue(g, n, d_out, (hipStream_t)stream);
+}
+
+int fsmc_mt19937_gaussian_async(fsmc_mt19937* g, long long n, double mean, double stddev, double* d_out,
+                                void* stream) {
+  FSMC_REQUIRE(g != nullptr, "generator is NULL");
+  FSMC_TRY(mt_validate(n, d_out));
+  return mt_gaussian_enqueue(g, n, mean, stddev, d_out, (hipStream_t)stream);
+}
+
+int fsmc_mt19937_bits(fsmc_mt19937* g, long long n, uint32_t* out) {
+  FSMC_REQUIRE(g != nullptr, "generator is NULL");
+  FSMC_TRY(mt_validate(n, out));
+  fsmc_ctx* ctx = g->ctx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const size_t bytes = (size_t)n * sizeof(uint32_t);
+  FSMC_TRY(ctx_prepare(ctx, 0, bytes, 0, 0));
+  FSMC_TRY(mt_bits_enqueue(g, n, (uint32_t*)ctx->normals.ptr, ctx->stream));
+  if (n) FSMC_HIP(hipMemcpyAsync(out, ctx->normals.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  FSMC_HIP(hipStreamSynchronize(ctx->stream));
+  return FSMC_OK;
+}
+
+int fsmc_mt19937_gaussian(fsmc_mt19937* g, long long n, double mean, double stddev, double* out) {
+  FSMC_REQUIRE(g != nullptr, "generator is NULL");
+  FSMC_TRY(mt_validate(n, out));
+  fsmc_ctx* ctx = g->ctx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const size_t bytes = (size_t)n * sizeof(double);
+  FSMC_TRY(ctx_prepare(ctx, 0, bytes, 0, 0));
+  FSMC_TRY(mt_gaussian_enqueue(g, n, mean, stddev, (double*)ctx->normals.ptr, ctx->stream));
+  if (n) FSMC_HIP(hipMemcpyAsync(out, ctx->normals.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  FSMC_HIP(hipStreamSynchronize(ctx->stream));
+  return FSMC_OK;
+}
+
+int fsmc_mt19937_destroy(fsmc_mt19937* g) {
+  if (!g) return FSMC_OK;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipDeviceSynchronize();
+  if (g->state) (void)hipFree(g->state);
+  g->bits.release();
+  delete g;
+  return FSMC_OK;
+}
+
+}  // extern "C"

@@ -131,6 +131,14 @@ def lib():
                                              ctypes.c_double, ctypes.c_double, _D]
         L.fsmc_mrg32k3a_gaussian_async.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_longlong,
                                                    ctypes.c_double, ctypes.c_double, _VP, _VP]
+        _U32 = ctypes.POINTER(ctypes.c_uint32)
+        L.fsmc_mt19937_create.argtypes = [_VP, ctypes.c_uint32, ctypes.POINTER(_VP)]
+        L.fsmc_mt19937_bits.argtypes = [_VP, ctypes.c_longlong, _U32]
+        L.fsmc_mt19937_gaussian.argtypes = [_VP, ctypes.c_longlong, ctypes.c_double, ctypes.c_double, _D]
+        L.fsmc_mt19937_bits_async.argtypes = [_VP, ctypes.c_longlong, _VP, _VP]
+        L.fsmc_mt19937_gaussian_async.argtypes = [_VP, ctypes.c_longlong, ctypes.c_double, ctypes.c_double, _VP,
+                                                  _VP]
+        L.fsmc_mt19937_destroy.argtypes = [_VP]
         _I = ctypes.POINTER(ctypes.c_int)
         L.fsmc_lsmc_plan_create.argtypes = [_VP, ctypes.c_int, PM] + [_D] * 5 + [_I, _I, ctypes.POINTER(_VP)]
         L.fsmc_lsmc_plan_workspace.restype = ctypes.c_size_t
@@ -305,11 +313,52 @@ class Engine:
                "fsmc_mrg32k3a_gaussian")
         return z
 
+    def mt19937(self, seed):
+        """A sequential MKL-compatible MT19937 stream (each call continues it)."""
+        return MT19937(self, seed)
+
     def libor_normals(self, n_paths, m, mean=0.0, stddev=1.0):
         z, pz = _d(np.zeros(n_paths * 40))
         _check(lib().fsmc_libor_normals(self._ctx, n_paths, ctypes.byref(m), mean, stddev, pz),
                "fsmc_libor_normals")
         return z.reshape(n_paths, 40)
+
+
+class MT19937:
+    """vslNewStream(VSL_BRNG_MT19937, seed) on the GPU: raw 32-bit outputs and
+    Box-Muller normals, each call continuing the stream."""
+
+    def __init__(self, engine, seed):
+        self._g = _VP()
+        _check(lib().fsmc_mt19937_create(engine._ctx, seed, ctypes.byref(self._g)), "fsmc_mt19937_create")
+
+    def bits(self, n):
+        out = np.zeros(n, dtype=np.uint32)
+        _check(lib().fsmc_mt19937_bits(self._g, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+               "fsmc_mt19937_bits")
+        return out
+
+    def gaussian(self, n, mean=0.0, stddev=1.0):
+        out = np.zeros(n)
+        _check(lib().fsmc_mt19937_gaussian(self._g, n, mean, stddev, out.ctypes.data_as(_D)),
+               "fsmc_mt19937_gaussian")
+        return out
+
+    def gaussian_async(self, n, out, stream=None, mean=0.0, stddev=1.0):
+        _check(lib().fsmc_mt19937_gaussian_async(self._g, n, mean, stddev, out.data_ptr(),
+                                                 stream if stream is not None else _stream()),
+               "fsmc_mt19937_gaussian_async")
+
+    def close(self):
+        if self._g:
+            lib().fsmc_mt19937_destroy(self._g)
+            self._g = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class LsmcPlan:

@@ -182,6 +182,28 @@ int fsmc_mrg32k3a_gaussian(fsmc_ctx* ctx, uint32_t seed, uint64_t offset, long l
 int fsmc_mrg32k3a_gaussian_async(uint32_t seed, uint64_t offset, long long n, double mean, double stddev,
                                  double* d_z, void* stream);
 
+/* ---- MKL-compatible MT19937 stream ------------------------------------------
+ * The generator of the AsianOptionsPricing sample (mkl_rng.h:5-69: an
+ * mkl_rng object owns vslNewStream(VSL_BRNG_MT19937, seed) and get_gaussian
+ * calls vdRngGaussian(VSL_RNG_METHOD_GAUSSIAN_BOXMULLER2, ..., 0, 1);
+ * MCAsianOptions.cpp:92-93 makes two, seeds 42 and 43), on the GPU.  A
+ * generator is one sequential stream: every call continues where the last
+ * one stopped, as successive vdRngGaussian calls on one VSL stream do, so
+ * calls must be ordered (one stream, or synchronised).  The 32-bit outputs
+ * equal MKL's bit for bit (init_by_array({seed}) seeding).  Normals come in
+ * Box-Muller pairs from consecutive outputs a, b (n even): u1 = a 2^-32 (a = 0
+ * taken as 2^-33), u2 = b 2^-32, z = mean + stddev sqrt(-2 ln u1) (sin 2 pi
+ * u2, cos 2 pi u2), in FP64; MKL's own Box-Muller arithmetic is not
+ * reproducible exactly (its normals differ from this formula by ~2e-8). */
+typedef struct fsmc_mt19937 fsmc_mt19937;
+int fsmc_mt19937_create(fsmc_ctx* ctx, uint32_t seed, fsmc_mt19937** out);
+int fsmc_mt19937_bits(fsmc_mt19937* g, long long n, uint32_t* out);
+int fsmc_mt19937_gaussian(fsmc_mt19937* g, long long n, double mean, double stddev, double* out);
+int fsmc_mt19937_bits_async(fsmc_mt19937* g, long long n, uint32_t* d_out, void* stream);
+int fsmc_mt19937_gaussian_async(fsmc_mt19937* g, long long n, double mean, double stddev, double* d_out,
+                                void* stream);
+int fsmc_mt19937_destroy(fsmc_mt19937* g);
+
 /* ---- Longstaff-Schwartz American puts -------------------------------------
  * Batched LSMC for many options with uneven step and path counts: one
  * exercise date per time step, cash flows discounted at r, regression of the

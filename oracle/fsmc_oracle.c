@@ -12,6 +12,8 @@
  *   heston_*      exact martingale mean, N=1 reduction to Black-Scholes,
  *                 N=2 conditional-closed-form quadrature (rho, xi, kappa,
  *                 theta, truncation, both vol conventions)
+ *   mt19937       MKL's outputs for seed 42 and numpy's RandomState([seed])
+ *                 raw draws (the same init_by_array seeding)
  */
 #include "fsmc_oracle.h"
 
@@ -837,4 +839,75 @@ double orc_icdf(double p) {
 void orc_mrg32k3a_gaussian(uint32_t seed, uint64_t offset, int64_t n, double mean, double stddev, double* z) {
   orc_mrg32k3a_uniform(seed, offset, n, z);
   for (int64_t i = 0; i < n; ++i) z[i] = mean + stddev * orc_icdf(z[i]);
+}
+
+/* ---- MT19937 as MKL's VSL_BRNG_MT19937 (AsianOptionsPricing/mkl_rng.h:5-69,
+ * MCAsianOptions.cpp:92-93): Matsumoto-Nishimura MT19937 seeded by
+ * init_by_array({seed}); outputs offset .. offset+n-1 of the stream. */
+#define ORC_MT_N 624
+#define ORC_MT_M 397
+
+static void orc_mt_seed(uint32_t seed, uint32_t* mt) {
+  mt[0] = 19650218u;
+  for (int i = 1; i < ORC_MT_N; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
+  int i = 1;
+  for (int k = ORC_MT_N; k > 0; --k) {
+    mt[i] = (mt[i] ^ ((mt[i - 1] ^ (mt[i - 1] >> 30)) * 1664525u)) + seed;
+    if (++i >= ORC_MT_N) {
+      mt[0] = mt[ORC_MT_N - 1];
+      i = 1;
+    }
+  }
+  for (int k = ORC_MT_N - 1; k > 0; --k) {
+    mt[i] = (mt[i] ^ ((mt[i - 1] ^ (mt[i - 1] >> 30)) * 1566083941u)) - (uint32_t)i;
+    if (++i >= ORC_MT_N) {
+      mt[0] = mt[ORC_MT_N - 1];
+      i = 1;
+    }
+  }
+  mt[0] = 0x80000000u;
+}
+
+static void orc_mt_twist(uint32_t* mt) {
+  /* in place, word by word in index order: word i reads the already-updated
+   * words below it, exactly as the sequential generator does */
+  for (int i = 0; i < ORC_MT_N; ++i) {
+    const uint32_t y = (mt[i] & 0x80000000u) | (mt[(i + 1) % ORC_MT_N] & 0x7fffffffu);
+    mt[i] = mt[(i + ORC_MT_M) % ORC_MT_N] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+  }
+}
+
+void orc_mt19937_bits(uint32_t seed, uint64_t offset, int64_t n, uint32_t* out) {
+  uint32_t mt[ORC_MT_N];
+  orc_mt_seed(seed, mt);
+  int pos = ORC_MT_N;
+  for (uint64_t k = 0; k < offset + (uint64_t)n; ++k) {
+    if (pos == ORC_MT_N) {
+      orc_mt_twist(mt);
+      pos = 0;
+    }
+    uint32_t y = mt[pos++];
+    if (k < offset) continue;
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    out[k - offset] = y;
+  }
+}
+
+/* Box-Muller pairs of consecutive outputs (VSL_RNG_METHOD_GAUSSIAN_BOXMULLER2,
+ * mkl_rng.h:57-62): u = bits 2^-32 (u1 = 0 taken as 2^-33),
+ * (z0, z1) = sqrt(-2 ln u1) (sin 2 pi u2, cos 2 pi u2); offset and n even. */
+void orc_mt19937_gaussian(uint32_t seed, uint64_t offset, int64_t n, double mean, double stddev, double* z) {
+  uint32_t* bits = (uint32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(uint32_t));
+  orc_mt19937_bits(seed, offset, n, bits);
+  for (int64_t j = 0; j + 1 < n; j += 2) {
+    const double u1 = bits[j] ? (double)bits[j] * 0x1.0p-32 : 0x1.0p-33;
+    const double u2 = (double)bits[j + 1] * 0x1.0p-32;
+    const double r = sqrt(-2.0 * log(u1));
+    z[j] = mean + stddev * (r * sin(2.0 * ORC_PI * u2));
+    z[j + 1] = mean + stddev * (r * cos(2.0 * ORC_PI * u2));
+  }
+  free(bits);
 }

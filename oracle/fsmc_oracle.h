@@ -94,6 +94,8 @@ int orc_chol4_solve(const double* G, const double* rhs, double* beta);
 void orc_mrg32k3a_uniform(uint32_t seed, uint64_t offset, int64_t n, double* u);
 double orc_icdf(double u);
 void orc_mrg32k3a_gaussian(uint32_t seed, uint64_t offset, int64_t n, double mean, double stddev, double* z);
+void orc_mt19937_bits(uint32_t seed, uint64_t offset, int64_t n, uint32_t* out);
+void orc_mt19937_gaussian(uint32_t seed, uint64_t offset, int64_t n, double mean, double stddev, double* z);
 
 #ifdef __cplusplus
 }

@@ -67,6 +67,10 @@ def lib():
         L.orc_icdf.argtypes = [ctypes.c_double]
         L.orc_mrg32k3a_gaussian.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64, ctypes.c_double,
                                             ctypes.c_double, _D]
+        L.orc_mt19937_bits.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64,
+                                       ctypes.POINTER(ctypes.c_uint32)]
+        L.orc_mt19937_gaussian.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64, ctypes.c_double,
+                                           ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -260,4 +264,18 @@ def mrg32k3a_gaussian(seed, offset, n, mean=0.0, stddev=1.0):
     """generateRand's normals (MRG32k3a + inverse CDF), stream order."""
     z, pz = _d(np.zeros(n))
     lib().orc_mrg32k3a_gaussian(seed, offset, n, mean, stddev, pz)
+    return z
+
+
+def mt19937_bits(seed, offset, n):
+    """Outputs offset .. offset+n-1 of MKL's MT19937 stream for seed."""
+    out = np.zeros(n, dtype=np.uint32)
+    lib().orc_mt19937_bits(seed, offset, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    return out
+
+
+def mt19937_gaussian(seed, offset, n, mean=0.0, stddev=1.0):
+    """Box-Muller normals of the MT19937 stream (offset, n even)."""
+    z = np.zeros(n)
+    lib().orc_mt19937_gaussian(seed, offset, n, mean, stddev, z.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return z

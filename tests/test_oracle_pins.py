@@ -63,6 +63,25 @@ def test_normals_moments():
     assert orc.normals4(7, 0, 1, 0, 0, 0) != orc.normals4(7, 1, 1, 0, 0, 0)
 
 
+def test_mt19937_pins():
+    # MKL's first outputs for vslNewStream(VSL_BRNG_MT19937, 42)
+    assert list(orc.mt19937_bits(42, 0, 3)) == [2746317213, 478163327, 107420369]
+    # numpy's legacy RandomState([seed]) seeds MT19937 by the same
+    # init_by_array; its full-range 32-bit draws are the raw outputs
+    for seed in (43, 7, 2**32 - 1):
+        want = np.random.RandomState([seed]).randint(0, 2**32, size=2000, dtype=np.uint32)
+        assert np.array_equal(orc.mt19937_bits(seed, 0, 2000), want)
+        assert np.array_equal(orc.mt19937_bits(seed, 1247, 700), want[1247:1947])
+    z = orc.mt19937_gaussian(42, 0, 200000)
+    n = z.size
+    assert abs(z.mean()) < 4 / math.sqrt(n) and abs(z.var() - 1) < 4 * math.sqrt(2 / n)
+    assert abs(np.mean(z ** 4) - 3) < 4 * math.sqrt(96 / n)
+    # pairs: z0 and z1 of a pair share the radius sqrt(-2 ln u1)
+    b = orc.mt19937_bits(42, 0, 8).astype(np.float64) * 2.0 ** -32
+    r2 = z[:8:2] ** 2 + z[1:8:2] ** 2
+    assert np.allclose(r2, -2 * np.log(b[::2]), rtol=1e-13)
+
+
 # ---- Black-Scholes and lattices --------------------------------------------------
 def test_bs_textbook():
     # Hull, Options Futures and Other Derivatives, example 15.6: c = 4.76.
