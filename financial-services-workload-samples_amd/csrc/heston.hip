@@ -50,16 +50,12 @@ __global__ __launch_bounds__(kBlock) void heston_paths_kernel(HestonArgs a) {
       }
     } else {
       const int nfull = a.n_steps & ~1;
-      for (int k = 0; k < nfull; k += 2) {
-        Real n0, n1, n2, n3;
-        normals4<Real>(U4{(uint32_t)p, (uint32_t)(k >> 1), (uint32_t)o, a.w}, a.k0, a.k1, n0, n1, n2, n3);
-        heston_pair<Real, kVarAsVol>(c, S, v, avg, n0, n1, n2, n3, true);
-      }
-      if (a.n_steps & 1) {
-        Real n0, n1, n2, n3;
-        normals4<Real>(U4{(uint32_t)p, (uint32_t)(nfull >> 1), (uint32_t)o, a.w}, a.k0, a.k1, n0, n1, n2, n3);
-        heston_pair<Real, kVarAsVol>(c, S, v, avg, n0, n1, n2, n3, false);
-      }
+      for (int k = 0; k < nfull; k += 2)
+        heston_philox_pair<Real, kVarAsVol>(c, S, v, avg, U4{(uint32_t)p, (uint32_t)(k >> 1), (uint32_t)o, a.w},
+                                            a.k0, a.k1, true);
+      if (a.n_steps & 1)
+        heston_philox_pair<Real, kVarAsVol>(c, S, v, avg, U4{(uint32_t)p, (uint32_t)(nfull >> 1), (uint32_t)o, a.w},
+                                            a.k0, a.k1, false);
     }
     const double mean = avg.value() / (double)a.n_steps;
     const double x = mean - a.X[o];

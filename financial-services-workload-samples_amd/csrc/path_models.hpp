@@ -52,6 +52,20 @@ __device__ __forceinline__ void heston_step(const HestonConsts<Real>& c, Real& S
   v = ffma(sq, ffma(c.xr, z1, c.xs * z2), ffma(c.kdt, c.theta - vp, v));
 }
 
+// The FP32 Philox step with its normals in polar form, z1 = r cs, z2 = r sn
+// (box_muller_polar): both normals only ever appear multiplied by sqrt(v+),
+// so q = sqrt(v+) r takes the radius in one multiply instead of two.
+template <typename Real>
+__device__ __forceinline__ void heston_step_polar(const HestonConsts<Real>& c, Real& S, Real& v, Real r, Real cs,
+                                                  Real sn) {
+#pragma clang fp contract(off)
+  const Real vp = positive_part(v);
+  const Real q = fsqrt(vp) * r;
+  const Real e = ffma(c.c_sdt * q, cs, ffma(-c.c_hdt, vp, c.c_rdt));
+  S = S * fexp2(e);
+  v = ffma(q, ffma(c.xr, cs, c.xs * sn), ffma(c.kdt, c.theta - vp, v));
+}
+
 // Two consecutive steps sharing one Philox call (normals n0..n3); the running
 // sum of the monitored spots takes the pair's sum (one FP32 rounding) in one
 // FP64 add.
@@ -66,6 +80,34 @@ __device__ __forceinline__ void heston_pair(const HestonConsts<Real>& c, Real& S
     avg.add(s1 + S);
   } else {
     avg.add(S);
+  }
+}
+
+// Two consecutive Philox-mode steps from counter ctr: FP32 with the
+// square-root variance takes the polar form, everything else the normals.
+// Every kernel that steps Philox-mode paths goes through here, so the
+// forward, adjoint and batch kernels see the same bits.
+template <typename Real, bool kVarAsVol>
+__device__ __forceinline__ void heston_philox_pair(const HestonConsts<Real>& c, Real& S, Real& v, PathSum& avg,
+                                                   U4 ctr, uint32_t k0, uint32_t k1, bool second) {
+#pragma clang fp contract(off)
+  if constexpr (sizeof(Real) == 4 && !kVarAsVol) {
+    const U4 w = philox4x32_10(ctr, k0, k1);
+    float r0, c0, s0, r1, c1, s1;
+    box_muller_polar(w.x, w.y, r0, c0, s0);
+    box_muller_polar(w.z, w.w, r1, c1, s1);
+    heston_step_polar(c, S, v, r0, c0, s0);
+    if (second) {
+      const Real sa = S;
+      heston_step_polar(c, S, v, r1, c1, s1);
+      avg.add(sa + S);
+    } else {
+      avg.add(S);
+    }
+  } else {
+    Real n0, n1, n2, n3;
+    normals4<Real>(ctr, k0, k1, n0, n1, n2, n3);
+    heston_pair<Real, kVarAsVol>(c, S, v, avg, n0, n1, n2, n3, second);
   }
 }
 
@@ -100,11 +142,7 @@ __device__ __forceinline__ double heston_adjoint_path(const HestonConsts<Real>& 
 #pragma unroll
     for (int j = 0; j < kHestonSeg; j += 2) {
       const int k = seg * kHestonSeg + j;
-      if (k < N) {
-        Real n0, n1, n2, n3;
-        normals4<Real>(U4{p, (uint32_t)(k >> 1), c2, w}, k0, k1, n0, n1, n2, n3);
-        heston_pair<Real, false>(c, S, v, avg, n0, n1, n2, n3, k + 1 < N);
-      }
+      if (k < N) heston_philox_pair<Real, false>(c, S, v, avg, U4{p, (uint32_t)(k >> 1), c2, w}, k0, k1, k + 1 < N);
     }
   }
   g[0] = g[1] = g[2] = 0.0;
@@ -122,18 +160,36 @@ __device__ __forceinline__ double heston_adjoint_path(const HestonConsts<Real>& 
     for (int j = 0; j < kHestonSeg; j += 2) {
       const int k = seg * kHestonSeg + j;
       if (k < N) {
-        Real n0, n1, n2, n3;
-        normals4<Real>(U4{p, (uint32_t)(k >> 1), c2, w}, k0, k1, n0, n1, n2, n3);
-        Ss[j] = Sr;
-        vs[j] = vr;
-        z1s[j] = n0;
-        zcs[j] = c.rho * n0 + c.srho * n1;
-        heston_step<Real, false>(c, Sr, vr, n0, n1);
-        Ss[j + 1] = Sr;
-        vs[j + 1] = vr;
-        z1s[j + 1] = n2;
-        zcs[j + 1] = c.rho * n2 + c.srho * n3;
-        if (k + 1 < N) heston_step<Real, false>(c, Sr, vr, n2, n3);
+        // recompute exactly as the forward sweep stepped (heston_philox_pair)
+        if constexpr (sizeof(Real) == 4) {
+          const U4 q = philox4x32_10(U4{p, (uint32_t)(k >> 1), c2, w}, k0, k1);
+          float r0, c0, s0, r1, c1, s1;
+          box_muller_polar(q.x, q.y, r0, c0, s0);
+          box_muller_polar(q.z, q.w, r1, c1, s1);
+          Ss[j] = Sr;
+          vs[j] = vr;
+          z1s[j] = r0 * c0;
+          zcs[j] = r0 * (c.rho * c0 + c.srho * s0);
+          heston_step_polar(c, Sr, vr, r0, c0, s0);
+          Ss[j + 1] = Sr;
+          vs[j + 1] = vr;
+          z1s[j + 1] = r1 * c1;
+          zcs[j + 1] = r1 * (c.rho * c1 + c.srho * s1);
+          if (k + 1 < N) heston_step_polar(c, Sr, vr, r1, c1, s1);
+        } else {
+          Real n0, n1, n2, n3;
+          normals4<Real>(U4{p, (uint32_t)(k >> 1), c2, w}, k0, k1, n0, n1, n2, n3);
+          Ss[j] = Sr;
+          vs[j] = vr;
+          z1s[j] = n0;
+          zcs[j] = c.rho * n0 + c.srho * n1;
+          heston_step<Real, false>(c, Sr, vr, n0, n1);
+          Ss[j + 1] = Sr;
+          vs[j + 1] = vr;
+          z1s[j + 1] = n2;
+          zcs[j + 1] = c.rho * n2 + c.srho * n3;
+          if (k + 1 < N) heston_step<Real, false>(c, Sr, vr, n2, n3);
+        }
       }
     }
 #pragma unroll
