@@ -559,8 +559,9 @@ int fsmc_batch_tuning_get(const fsmc_batch* b, fsmc_batch_tuning* out) {
 int fsmc_batch_destroy(fsmc_batch* b) {
   if (!b) return FSMC_OK;
   (void)hipSetDevice(b->ctx->device);
-  (void)hipStreamSynchronize(b->ctx->stream);
-  if (b->impl.capture) (void)hipStreamSynchronize(b->impl.capture);
+  // runs may sit on any caller stream: wait for all work before the graph,
+  // events and workspace go
+  (void)hipDeviceSynchronize();
   batch_release_events(b->impl);
   if (b->impl.start) (void)hipEventDestroy(b->impl.start);
   if (b->impl.t0) (void)hipEventDestroy(b->impl.t0);

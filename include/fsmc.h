@@ -208,8 +208,9 @@ int fsmc_mt19937_destroy(fsmc_mt19937* g);
  * Batched LSMC for many options with uneven step and path counts: one
  * exercise date per time step, cash flows discounted at r, regression of the
  * discounted cash flow on {1, x, x^2, x^3} of the in-the-money paths with x an
- * affine rescaling of S/K (moment sums per lane in FP64, solved per date by
- * FP64 Cholesky).  Paths are built backwards by the Brownian bridge from
+ * affine rescaling of S/K (moment sums per thread and per wave in the working
+ * precision, across waves and workgroups in FP64, solved per date by FP64
+ * Cholesky).  Paths are built backwards by the Brownian bridge from
  * Philox normals: Z_i of path p = component p & 3 of counter (p/4, i, option)
  * (tag 4), W_N = sqrt(T) Z_{N-1}, W_k = k/(k+1) W_{k+1} + sqrt(dt k/(k+1)) Z_{k-1},
  * S_k = S0 exp((r - sigma^2/2) k dt + sigma W_k).  It replaces the sweep inlined in
