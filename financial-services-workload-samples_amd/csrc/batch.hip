@@ -46,10 +46,10 @@ struct BatchArgs {
   uint32_t k0, k1, w_gbm, w_heston;
 };
 
+// One GBM item (a chunk of one request's paths) by the whole workgroup:
+// pathwise price, delta, vega and rho sums into the item's partials.
 template <typename Real>
-__global__ __launch_bounds__(kBlock) void batch_gbm_kernel(BatchArgs a, int first_item) {
-  __shared__ double lds[kNV * (kBlock / 64)];
-  const int item = first_item + blockIdx.x;
+__device__ __forceinline__ void gbm_item(const BatchArgs& a, int item, double* lds) {
   const int2 e = a.items[item];
   const ReqDev q = a.req[e.x];
   const int p0 = e.y * a.chunk;
@@ -88,18 +88,16 @@ __global__ __launch_bounds__(kBlock) void batch_gbm_kernel(BatchArgs a, int firs
   }
 }
 
+// One Heston item: the adjoint path of each of the chunk's paths, checkpoints
+// in this thread's slot ck (stride = threads of the launch).
 template <typename Real>
-__global__ __launch_bounds__(kBlock) void batch_heston_kernel(BatchArgs a, int first_item) {
-  __shared__ double lds[kNV * (kBlock / 64)];
-  const int item = first_item + blockIdx.x;
+__device__ __forceinline__ void heston_item(const BatchArgs& a, int item, Real* ck, size_t stride, double* lds) {
   const int2 e = a.items[item];
   const ReqDev q = a.req[e.x];
   const int p0 = e.y * a.chunk;
   const int p1 = min(q.n_paths, p0 + a.chunk);
   const double disc = exp(-q.r * q.T);
   const HestonConsts<Real> c(q.T, q.n_steps, q.r, q.kappa, q.xi, q.theta, q.rho);
-  const size_t stride = (size_t)gridDim.x * kBlock;
-  Real* ck = (Real*)a.ckpt + (size_t)blockIdx.x * kBlock + threadIdx.x;
   double s[kNV] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int p = p0 + (int)threadIdx.x; p < p1; p += kBlock) {
     double g[3];
@@ -119,6 +117,40 @@ __global__ __launch_bounds__(kBlock) void batch_heston_kernel(BatchArgs a, int f
     double* dst = a.partials + (size_t)item * kNV;
 #pragma unroll
     for (int k = 0; k < kNV; ++k) dst[k] = s[k];
+  }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void batch_gbm_kernel(BatchArgs a, int first_item) {
+  __shared__ double lds[kNV * (kBlock / 64)];
+  gbm_item<Real>(a, first_item + blockIdx.x, lds);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void batch_heston_kernel(BatchArgs a, int first_item) {
+  __shared__ double lds[kNV * (kBlock / 64)];
+  heston_item<Real>(a, first_item + blockIdx.x, (Real*)a.ckpt + (size_t)blockIdx.x * kBlock + threadIdx.x,
+                    (size_t)gridDim.x * kBlock, lds);
+}
+
+// Persistent variant: gridDim.x = CUs x workgroups per CU; workgroup g runs
+// the items sched[off[g]] .. sched[off[g + 1] - 1] of this micro-batch, both
+// kinds, in the order the host's longest-first schedule put them.  The
+// schedule is static: no queue counter, no fence, no workgroup waits on
+// another, and every loop is bounded by the schedule.
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void batch_persistent_kernel(BatchArgs a, const int* sched, const int* off) {
+  __shared__ double lds[kNV * (kBlock / 64)];
+  Real* ck = (Real*)a.ckpt + (size_t)blockIdx.x * kBlock + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  const int j1 = off[blockIdx.x + 1];
+  for (int j = off[blockIdx.x]; j < j1; ++j) {
+    const int item = sched[j];
+    if (a.req[a.items[item].x].kind == FSMC_REQ_GBM_EUROPEAN)  // uniform per workgroup
+      gbm_item<Real>(a, item, lds);
+    else
+      heston_item<Real>(a, item, ck, stride, lds);
+    __syncthreads();  // thread 0's reads of lds in block_sum before the next item's writes
   }
 }
 
@@ -153,6 +185,7 @@ struct MicroBatch {
   int r0, n;                 // requests [r0, r0 + n)
   int gbm_first, gbm_count;  // items
   int hes_first, hes_count;
+  int off_first;             // persistent mode: this micro-batch's G + 1 schedule offsets
 };
 
 struct BatchImpl {
@@ -160,12 +193,13 @@ struct BatchImpl {
   fsmc_mode mode;
   std::vector<fsmc_request> req;
   int max_seg = 1;
-  fsmc_batch_tuning tuning = {4096, 0};
+  fsmc_batch_tuning tuning = {4096, 0, 0, 0};
   std::vector<MicroBatch> waves;
   int max_hes_grid = 0;
+  int cus = 0;  // compute units (persistent grid = cus x workgroups_per_cu)
   DeviceBuffer ws;
   size_t off_req = 0, off_items = 0, off_first = 0, off_nitems = 0, off_part = 0, off_res = 0, off_ckpt = 0,
-         off_stamp = 0;
+         off_stamp = 0, off_sched = 0, off_soff = 0;
   std::vector<hipEvent_t> done;  // one per micro-batch (stream mode)
   hipEvent_t start = nullptr;
   hipEvent_t t0 = nullptr, t1 = nullptr;  // autotuner's timing around a run
@@ -236,6 +270,8 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
                "chunk_paths %d must be a positive multiple of %d", t.chunk_paths, 4 * kBlock);
   FSMC_REQUIRE(t.requests_per_launch >= 0, "requests_per_launch %d < 0", t.requests_per_launch);
   FSMC_REQUIRE(t.replay_graph == 0 || t.replay_graph == 1, "replay_graph %d is not 0 or 1", t.replay_graph);
+  FSMC_REQUIRE(t.workgroups_per_cu >= 0 && t.workgroups_per_cu <= 16, "workgroups_per_cu %d outside [0, 16]",
+               t.workgroups_per_cu);
   FSMC_HIP(hipSetDevice(ctx->device));
   FSMC_HIP(hipStreamSynchronize(ctx->stream));
   batch_release_events(b);
@@ -277,6 +313,42 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
     }
     b.waves.push_back(mb);
   }
+  // Persistent mode: per micro-batch, items longest first (cost = paths x
+  // work per path) onto the least-loaded of G = cus x workgroups_per_cu
+  // workgroup lists; the lists are concatenated into sched, with G + 1
+  // offsets per micro-batch in soff.
+  const int G = t.workgroups_per_cu > 0 ? b.cus * t.workgroups_per_cu : 0;
+  std::vector<int> sched, soff;
+  if (G > 0) {
+    for (MicroBatch& mb : b.waves) {
+      std::vector<std::pair<double, int>> cost;
+      for (int it = mb.gbm_first; it < mb.hes_first + mb.hes_count; ++it) {
+        const fsmc_request& r = b.req[items[it].x];
+        const int paths = std::min(t.chunk_paths, r.n_paths - items[it].y * t.chunk_paths);
+        // one GBM path ~ 35 instructions, one Heston adjoint step ~ 110
+        cost.push_back({(double)paths * (r.kind == FSMC_REQ_GBM_EUROPEAN ? 1.0 : 3.0 * r.n_steps), it});
+      }
+      std::stable_sort(cost.begin(), cost.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+      std::vector<std::vector<int>> lists(G);
+      std::vector<std::pair<double, int>> load;  // min-heap of (load, workgroup)
+      for (int g = 0; g < G; ++g) load.push_back({0.0, g});
+      auto cmp = [](const auto& x, const auto& y) { return x.first > y.first || (x.first == y.first && x.second > y.second); };
+      std::make_heap(load.begin(), load.end(), cmp);
+      for (const auto& c : cost) {
+        std::pop_heap(load.begin(), load.end(), cmp);
+        auto& least = load.back();
+        lists[least.second].push_back(c.second);
+        least.first += c.first;
+        std::push_heap(load.begin(), load.end(), cmp);
+      }
+      mb.off_first = (int)soff.size();
+      for (int g = 0; g < G; ++g) {
+        soff.push_back((int)sched.size());
+        sched.insert(sched.end(), lists[g].begin(), lists[g].end());
+      }
+      soff.push_back((int)sched.size());
+    }
+  }
   for (int i = 0; i < b.n; ++i) {
     const fsmc_request& r = b.req[i];
     ReqDev& d = rd[i];
@@ -308,8 +380,11 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
   b.off_nitems = take(sizeof(int) * b.n);
   b.off_part = take(sizeof(double) * kNV * items.size());
   b.off_res = take(sizeof(double) * 8 * b.n);
-  b.off_ckpt = take(real * 2 * (size_t)b.max_seg * (size_t)std::max(1, b.max_hes_grid) * kBlock);
+  const size_t ck_groups = (size_t)std::max(1, G > 0 ? G : b.max_hes_grid);
+  b.off_ckpt = take(real * 2 * (size_t)b.max_seg * ck_groups * kBlock);
   b.off_stamp = take(sizeof(unsigned long long) * (b.waves.size() + 1));
+  b.off_sched = take(sizeof(int) * std::max<size_t>(1, sched.size()));
+  b.off_soff = take(sizeof(int) * std::max<size_t>(1, soff.size()));
   FSMC_TRY(b.ws.reserve(off));
   char* ws = (char*)b.ws.ptr;
   hipStream_t s = ctx->stream;
@@ -317,6 +392,10 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
   FSMC_HIP(hipMemcpyAsync(ws + b.off_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice, s));
   FSMC_HIP(hipMemcpyAsync(ws + b.off_first, first.data(), sizeof(int) * b.n, hipMemcpyHostToDevice, s));
   FSMC_HIP(hipMemcpyAsync(ws + b.off_nitems, nitems.data(), sizeof(int) * b.n, hipMemcpyHostToDevice, s));
+  if (G > 0) {
+    FSMC_HIP(hipMemcpyAsync(ws + b.off_sched, sched.data(), sizeof(int) * sched.size(), hipMemcpyHostToDevice, s));
+    FSMC_HIP(hipMemcpyAsync(ws + b.off_soff, soff.data(), sizeof(int) * soff.size(), hipMemcpyHostToDevice, s));
+  }
   FSMC_HIP(hipStreamSynchronize(s));
   if (t.replay_graph) {
     FSMC_TRY(batch_capture(b));
@@ -347,10 +426,16 @@ static int batch_enqueue(BatchImpl& b, hipStream_t s, bool stamps) {
     FSMC_HIP(hipEventRecord(b.start, s));
   for (size_t w = 0; w < b.waves.size(); ++w) {
     const MicroBatch& mb = b.waves[w];
-    if (mb.gbm_count > 0)
-      hipLaunchKernelGGL(batch_gbm_kernel<Real>, dim3(mb.gbm_count), dim3(kBlock), 0, s, a, mb.gbm_first);
-    if (mb.hes_count > 0)
-      hipLaunchKernelGGL(batch_heston_kernel<Real>, dim3(mb.hes_count), dim3(kBlock), 0, s, a, mb.hes_first);
+    if (b.tuning.workgroups_per_cu > 0) {
+      const int G = b.cus * b.tuning.workgroups_per_cu;
+      hipLaunchKernelGGL(batch_persistent_kernel<Real>, dim3(G), dim3(kBlock), 0, s, a,
+                         (const int*)(ws + b.off_sched), (const int*)(ws + b.off_soff) + mb.off_first);
+    } else {
+      if (mb.gbm_count > 0)
+        hipLaunchKernelGGL(batch_gbm_kernel<Real>, dim3(mb.gbm_count), dim3(kBlock), 0, s, a, mb.gbm_first);
+      if (mb.hes_count > 0)
+        hipLaunchKernelGGL(batch_heston_kernel<Real>, dim3(mb.hes_count), dim3(kBlock), 0, s, a, mb.hes_first);
+    }
     hipLaunchKernelGGL(batch_finalize_kernel, dim3((mb.n + 63) / 64), dim3(64), 0, s, a.req,
                        (const int*)(ws + b.off_first), (const int*)(ws + b.off_nitems), a.partials, mb.r0, mb.n,
                        (double*)(ws + b.off_res));
@@ -402,7 +487,7 @@ int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_
   bi.mode = *m;
   bi.req.assign(req, req + n);
   bi.max_seg = max_seg;
-  fsmc_batch_tuning t = {4096, 0, 0};
+  fsmc_batch_tuning t = {4096, 0, 0, 0};
   if (tuning) {
     t = *tuning;
   } else {
@@ -414,7 +499,9 @@ int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_
   {
     std::lock_guard<std::mutex> lock(ctx->mu);
     int khz = 0;
-    st = hipSetDevice(ctx->device) == hipSuccess && hipEventCreate(&bi.start) == hipSuccess &&
+    st = hipSetDevice(ctx->device) == hipSuccess &&
+                 hipDeviceGetAttribute(&bi.cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess &&
+                 bi.cus > 0 && hipEventCreate(&bi.start) == hipSuccess &&
                  hipEventCreate(&bi.t0) == hipSuccess && hipEventCreate(&bi.t1) == hipSuccess &&
                  hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) == hipSuccess && khz > 0
              ? FSMC_OK
@@ -522,19 +609,28 @@ int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen) {
     for (int per : pers) {
       if (per >= bi.n) continue;
       float ms = 0.0f;
-      st = batch_time(b, fsmc_batch_tuning{c, per, 0}, &ms);
+      st = batch_time(b, fsmc_batch_tuning{c, per, 0, 0}, &ms);
       if (st != FSMC_OK) break;
       if (ms < best_ms) {
         best_ms = ms;
-        best = fsmc_batch_tuning{c, per, 0};
+        best = fsmc_batch_tuning{c, per, 0, 0};
       }
     }
     if (st != FSMC_OK) break;
   }
   if (st != FSMC_OK) return st;
+  for (int wpc : {1, 2, 4}) {  // the persistent kernel at this chunk and micro-batch size
+    float ms = 0.0f;
+    const fsmc_batch_tuning p = {best.chunk_paths, best.requests_per_launch, 0, wpc};
+    FSMC_TRY(batch_time(b, p, &ms));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = p;
+    }
+  }
   {
     float ms = 0.0f;
-    const fsmc_batch_tuning g = {best.chunk_paths, best.requests_per_launch, 1};
+    const fsmc_batch_tuning g = {best.chunk_paths, best.requests_per_launch, 1, best.workgroups_per_cu};
     FSMC_TRY(batch_time(b, g, &ms));
     if (ms < best_ms) best = g;
   }

@@ -311,21 +311,27 @@ typedef struct fsmc_batch_tuning {
   int chunk_paths;         /* paths per work item, a multiple of 1024 */
   int requests_per_launch; /* micro-batch size, 0 = the whole batch   */
   int replay_graph;        /* 1: the batch's launch sequence (kernels and
-                              completion events) is captured once into a
+                              completion stamps) is captured once into a
                               hipGraph that fsmc_batch_run replays        */
+  int workgroups_per_cu;   /* 0: one grid of GBM and one of Heston items per
+                              micro-batch; 1..16: one persistent grid of
+                              CUs x this many workgroups per micro-batch,
+                              each running a static longest-first list of
+                              items of both kinds                         */
 } fsmc_batch_tuning;
 
 typedef struct fsmc_batch fsmc_batch;
 
 /* tuning NULL: the autotuner's cached choice for a batch of this shape, or
- * {4096, 0, 0}. */
+ * {4096, 0, 0, 0}. */
 int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_mode* m,
                       const fsmc_batch_tuning* tuning, fsmc_batch** out);
 int fsmc_batch_run(fsmc_batch* b, void* stream);
 int fsmc_batch_results(fsmc_batch* b, void* stream, fsmc_result* out);
-/* Times the batch over chunk sizes x micro-batch sizes, then the winner with
- * and without graph replay; keeps the fastest and caches it for batches of
- * the same shape in this process. */
+/* Times the batch over chunk sizes x micro-batch sizes (grouped grids), then
+ * the winner as a persistent grid of 1, 2 and 4 workgroups per CU, then the
+ * best with graph replay; keeps the fastest and caches it for batches of the
+ * same shape in this process. */
 int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen);
 int fsmc_batch_tuning_get(const fsmc_batch* b, fsmc_batch_tuning* out);
 int fsmc_batch_destroy(fsmc_batch* b);

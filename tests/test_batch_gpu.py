@@ -83,7 +83,7 @@ def test_mixed_batch_reruns_and_tuning(engine):
     b2.close()
     assert np.allclose(other["price"], first["price"], rtol=1e-9, atol=1e-12)
     chosen = b.autotune()
-    assert chosen[0] % 1024 == 0 and chosen[1] >= 0 and chosen[2] in (0, 1)
+    assert chosen[0] % 1024 == 0 and chosen[1] >= 0 and chosen[2] in (0, 1) and chosen[3] in (0, 1, 2, 4)
     assert b.tuning() == chosen
     b.run()
     tuned = b.results()
@@ -91,21 +91,23 @@ def test_mixed_batch_reruns_and_tuning(engine):
     assert np.allclose(tuned["price"], first["price"], rtol=1e-9, atol=1e-12)
 
 
-@pytest.mark.parametrize("prec", [mc.FP32, mc.FP64])
-def test_graph_replay_matches_stream_launches(engine, prec):
-    # the captured launch sequence, replayed with eager work and
-    # synchronisations between replays, gives the stream-launched results
-    # bit for bit on every replay, and its completion events still time the
-    # micro-batches in order
+@pytest.mark.parametrize("prec,tuning", [(mc.FP32, (2048, 16, 1, 0)), (mc.FP64, (2048, 16, 1, 0)),
+                                         (mc.FP32, (2048, 16, 0, 2)), (mc.FP32, (2048, 16, 1, 1)),
+                                         (mc.FP64, (2048, 16, 0, 1))])
+def test_graph_and_persistent_match_grouped_launches(engine, prec, tuning):
+    # graph replay (with eager work and synchronisations between replays) and
+    # the persistent static-schedule kernel give the grouped stream launches'
+    # results bit for bit on every run (an item's sums do not depend on which
+    # workgroup runs it), and their completion times order the micro-batches
     import torch
     reqs = mc.mixed_requests(64, seed=8)
     m = mc.mode(prec, seed=6)
-    ref = mc.Batch(engine, reqs, m, tuning=(2048, 16, 0))
+    ref = mc.Batch(engine, reqs, m, tuning=(2048, 16, 0, 0))
     ref.run()
     want = ref.results()
     ref.close()
-    g = mc.Batch(engine, reqs, m, tuning=(2048, 16, 1))
-    assert g.tuning() == (2048, 16, 1)
+    g = mc.Batch(engine, reqs, m, tuning=tuning)
+    assert g.tuning() == tuning
     for _ in range(4):
         x = torch.randn(1 << 20, device="cuda")
         torch.cuda.synchronize()
@@ -144,3 +146,5 @@ def test_batch_errors(engine):
         mc.Batch(engine, [mc.gbm_request(100, 10, 10, 1, .02, .3)], mc.mode(), tuning=(1000, 0))
     with pytest.raises(mc.FsmcError):
         mc.Batch(engine, [mc.gbm_request(100, 10, 10, 1, .02, .3)], mc.mode(), tuning=(1024, 0, 2))
+    with pytest.raises(mc.FsmcError):
+        mc.Batch(engine, [mc.gbm_request(100, 10, 10, 1, .02, .3)], mc.mode(), tuning=(1024, 0, 0, 17))
