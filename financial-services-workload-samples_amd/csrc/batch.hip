@@ -619,7 +619,7 @@ int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen) {
     if (st != FSMC_OK) break;
   }
   if (st != FSMC_OK) return st;
-  for (int wpc : {1, 2, 4}) {  // the persistent kernel at this chunk and micro-batch size
+  for (int wpc : {1, 2, 3, 4}) {  // the persistent kernel at this chunk and micro-batch size
     float ms = 0.0f;
     const fsmc_batch_tuning p = {best.chunk_paths, best.requests_per_launch, 0, wpc};
     FSMC_TRY(batch_time(b, p, &ms));

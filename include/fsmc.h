@@ -329,7 +329,7 @@ int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_
 int fsmc_batch_run(fsmc_batch* b, void* stream);
 int fsmc_batch_results(fsmc_batch* b, void* stream, fsmc_result* out);
 /* Times the batch over chunk sizes x micro-batch sizes (grouped grids), then
- * the winner as a persistent grid of 1, 2 and 4 workgroups per CU, then the
+ * the winner as a persistent grid of 1 to 4 workgroups per CU, then the
  * best with graph replay; keeps the fastest and caches it for batches of the
  * same shape in this process. */
 int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen);
