@@ -31,7 +31,9 @@
 //     than a last-workgroup ticket: on this GPU every device-scope release
 //     fence writes back the XCD's L2, and 35k such fences per pass cost
 //     milliseconds, while the kernel boundary costs microseconds.
-//   * lsmc_final_kernel: price = max(K - S0, mean discounted cash flow).
+//   * lsmc_final_kernel: price = max(K - S0, mean discounted cash flow), from
+//     the per-workgroup cash-flow sums the option's last pass (the decision
+//     at date 1) leaves behind, so the cash flows are not read again.
 // Options are sorted by step count (descending) so the options still active
 // at backward pass j are a prefix of the workgroup table: pass j launches
 // exactly their workgroups.
@@ -71,6 +73,7 @@ struct LsmcArgs {
   const int* order;    // option indices sorted by steps (descending)
   double* beta;        // [2][option][5]: coefficients, valid flag (double-buffered by pass)
   double* eur;         // [workgroup] European payoff sums
+  double* cfsum;       // [workgroup][2] sum and sum of squares of the final cash flows
   int n_options;
   int pass;
   uint32_t k0, k1, w;
@@ -271,6 +274,7 @@ __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
   Real sm[kNM];
 #pragma unroll
   for (int q = 0; q < kNM; ++q) sm[q] = Real(0);
+  double fin[2] = {0.0, 0.0};
 #pragma unroll
   for (int h = 0; h < kGroups; ++h) {
     const int l0 = 4 * (h * kBlock + (int)threadIdx.x);
@@ -295,6 +299,10 @@ __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
           }
         }
       }
+      if (k < 1 && live) {  // last pass: the path's final discounted cash flow
+        fin[0] += (double)c;
+        fin[1] += (double)c * (double)c;
+      }
       if (k >= 1) {  // uniform per workgroup
         const Real Wk = ba * wd[h][g] + bb * z[g];
         wk[g] = Wk;
@@ -306,7 +314,14 @@ __global__ __launch_bounds__(kBlock) void lsmc_pass_kernel(LsmcArgs a) {
     }
     if (k >= 1) store4(wst + l0, wk);  // padding entries too (never read as paths)
   }
-  if (k < 1) return;  // last pass of this option: decision only
+  if (k < 1) {  // last pass of this option: decision only, then its cash-flow sums
+    block_sum<2>(fin, lds);
+    if (threadIdx.x == 0) {
+      a.cfsum[2 * blockIdx.x] = fin[0];
+      a.cfsum[2 * blockIdx.x + 1] = fin[1];
+    }
+    return;
+  }
   double s[kNS];
   if constexpr (sizeof(Real) == 4) {
     // FP32 thread sums: FP32 wave trees (6 roundings), FP64 from the wave totals on
@@ -344,14 +359,8 @@ __global__ __launch_bounds__(kBlock) void lsmc_final_kernel(LsmcArgs a, double* 
   __shared__ double lds[2 * (kBlock / 64)];
   const int o = blockIdx.x;
   const OptMeta m = a.meta[o];
-  const Real* cf = (const Real*)a.cf + m.cf_off;
-  double s[2] = {0.0, 0.0};
-  for (int p = threadIdx.x; p < m.paths; p += kBlock) {
-    const double c = (double)cf[p];
-    s[0] += c;
-    s[1] += c * c;
-  }
-  block_sum<2>(s, lds);
+  double s[2];
+  combine_partials<2>(a.cfsum, m.first_block, m.nblk, s, lds);
   double eu[1] = {0.0};
   if (european) {  // uniform; the barrier orders thread 0's reads of lds before its reuse
     __syncthreads();
@@ -526,7 +535,7 @@ struct LsmcPlanImpl {
   std::vector<int> live_of_pass;  // options with a date left at backward pass j
   int max_steps = 0;
   size_t real = 4, cf_elems = 0;
-  size_t off_meta, off_table, off_order, off_w, off_cf, off_part, off_beta, off_eur, off_out, ws = 0;
+  size_t off_meta, off_table, off_order, off_w, off_cf, off_part, off_beta, off_eur, off_cfsum, off_out, ws = 0;
 };
 
 int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K, const double* sigma,
@@ -595,6 +604,7 @@ int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K,
   pl->off_part = take(sizeof(double) * kNS * pl->table.size());
   pl->off_beta = take(sizeof(double) * 10 * n);
   pl->off_eur = take(sizeof(double) * pl->table.size());
+  pl->off_cfsum = take(sizeof(double) * 2 * pl->table.size());
   pl->off_out = take(sizeof(double) * 3 * n);
   pl->ws = off;
   return FSMC_OK;
@@ -611,6 +621,7 @@ int lsmc_enqueue(const LsmcPlanImpl& pl, char* ws, hipStream_t s) {
   a.order = (const int*)(ws + pl.off_order);
   a.beta = (double*)(ws + pl.off_beta);
   a.eur = (double*)(ws + pl.off_eur);
+  a.cfsum = (double*)(ws + pl.off_cfsum);
   a.n_options = pl.n;
   a.k0 = (uint32_t)pl.mode.seed;
   a.k1 = (uint32_t)(pl.mode.seed >> 32);
