@@ -201,7 +201,7 @@ constexpr double kRebuildMaxLambda = 0.3;
 // as the row total minus the prefix below i.  No HBM traffic beyond the
 // [81][path] outputs.
 template <bool kBuffer, bool kInplace>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void libor_path_rc_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void libor_path_rc_kernel(
     LiborArgs a) {  // two waves per SIMD: at most 256 registers per lane, no spills (80 rates + 80 adjoints)
   using O = MathOps<float>;
   const int p = blockIdx.x * kBlock + threadIdx.x;
@@ -243,6 +243,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   const float val = portfolio_adjoint<float, kInplace>(L, M);
 #pragma unroll
   for (int i = 1; i < NN; ++i) M[i] *= L[i];
+  // Rates 1..39 move to LDS for the reverse sweep (each thread its own
+  // column, no barrier needed): with only rates 40..79 and the adjoints in
+  // registers the kernel fits three waves per SIMD.
+  __shared__ float Llo[NMAT][kBlock];
+#pragma unroll
+  for (int i = 1; i < NMAT; ++i) Llo[i][threadIdx.x] = L[i];
 
   // ---- adjoint: path_calc_b2, rows rebuilt in registers ------------------------
   // v1(i) lambda faci^2 = con1^2 (S - P) / (1 + delta L(n,i))^2, with P the
@@ -269,7 +275,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
 #pragma unroll
     for (int i = 1; i < NN; ++i) {
       if (i > n) {
-        const float w = L[i] * O::ex(-__builtin_fmaf(kk, vs, c2));
+        const float y = i < NMAT ? Llo[i][threadIdx.x] : L[i];
+        const float w = y * O::ex(-__builtin_fmaf(kk, vs, c2));
         const float r0 = __builtin_amdgcn_rcpf(__builtin_fmaf(delta, w, 1.f));
         // x = w (1 + e), e = -s + c2 s^2 + c3 s^3, s = con1^2 w / (1 + delta w)
         const float s = con1sq * (w * r0);
@@ -283,7 +290,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         const float d = S - P0;
         P0 += M[i];
         M[i] = __builtin_fmaf(xr * r, d * con1sq, M[i]);
-        L[i] = x;
+        if (i < NMAT)
+          Llo[i][threadIdx.x] = x;
+        else
+          L[i] = x;
       }
     }
   }
