@@ -202,7 +202,8 @@ constexpr double kRebuildMaxLambda = 0.3;
 // [81][path] outputs.
 template <bool kBuffer, bool kInplace>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void libor_path_rc_kernel(
-    LiborArgs a) {  // two waves per SIMD: at most 256 registers per lane, no spills (80 rates + 80 adjoints)
+    LiborArgs a) {  // three waves per SIMD: <= 168 registers per lane; the sweeps do not spill (the portfolio
+                    // section around them does, once per path)
   using O = MathOps<float>;
   const int p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= a.n_paths) return;  // no workgroup synchronisation below
