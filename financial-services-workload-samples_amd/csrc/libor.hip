@@ -284,8 +284,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         const float c3 = __builtin_fmaf(r0, __builtin_fmaf(-2.f, r0, -0.5f), -1.f / 6.f);
         const float h = __builtin_fmaf(s, __builtin_fmaf(s, c3, r0 + 0.5f), -1.f);
         const float x = __builtin_fmaf(w, s * h, w);
-        // 1 / (1 + delta x) by one Newton step from r0: |delta (x - w) r0| < 1e-4
-        const float r = __builtin_fmaf(r0, __builtin_fmaf(-__builtin_fmaf(delta, x, 1.f), r0, 1.f), r0);
+        const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(delta, x, 1.f));
         const float xr = x * r;
         vs += xr;
         const float d = S - P0;
