@@ -84,3 +84,21 @@ def test_workspace_sizes(lib):
         ws = lib.fsmc_libor_workspace(ctypes.byref(lp), ctypes.byref(mcengine.mode(prec)))
         real = 8 if prec == mcengine.FP64 else 4
         assert ws > 0 and (ws >= 2380 * 4096 * real) == stored
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # one GPU is measured; --gpus N > 1 exits with a message before touching a device
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0 and "one GPU" in r.stderr
+
+
+def test_batch_tuning_validated_without_device(lib):
+    # fsmc_batch_create rejects bad tunings before any device work; with no
+    # GPU here the context cannot be created, so check the declared layout
+    t = mcengine.BatchTuning(4096, 0, 1, 2)
+    assert ctypes.sizeof(t) == 16 and (t.chunk_paths, t.requests_per_launch, t.replay_graph,
+                                        t.workgroups_per_cu) == (4096, 0, 1, 2)
