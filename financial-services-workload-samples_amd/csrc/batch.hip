@@ -470,7 +470,8 @@ int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_
     const fsmc_request& r = req[i];
     FSMC_REQUIRE(r.kind == FSMC_REQ_GBM_EUROPEAN || r.kind == FSMC_REQ_HESTON_ASIAN, "request %d: bad kind %d", i,
                  r.kind);
-    FSMC_REQUIRE(r.n_paths >= 2, "request %d: n_paths %d < 2", i, r.n_paths);
+    FSMC_REQUIRE(r.n_paths >= 2 && r.n_paths <= (1 << 30), "request %d: n_paths %d outside [2, 2^30]", i,
+                 r.n_paths);
     FSMC_REQUIRE(r.S0 > 0 && r.K >= 0 && r.T > 0, "request %d: S0, T must be > 0 and K >= 0", i);
     if (r.kind == FSMC_REQ_GBM_EUROPEAN) {
       FSMC_REQUIRE(r.sigma > 0, "request %d: sigma must be > 0", i);

@@ -165,7 +165,7 @@ int gbm_validate(const fsmc_gbm_params* p, const fsmc_mode* m) {
   FSMC_REQUIRE(p != nullptr, "params is NULL");
   FSMC_TRY(validate_mode(m, true));
   FSMC_REQUIRE(p->n_options >= 1 && p->n_options <= 65535, "n_options %d outside [1, 65535]", p->n_options);
-  FSMC_REQUIRE(p->n_paths >= 2, "n_paths %d < 2", p->n_paths);
+  FSMC_REQUIRE(p->n_paths >= 2 && p->n_paths <= (1 << 30), "n_paths %d outside [2, 2^30]", p->n_paths);
   FSMC_REQUIRE(p->volatility > 0.0, "volatility must be > 0");
   return FSMC_OK;
 }

@@ -191,7 +191,7 @@ int heston_validate(const fsmc_heston_params* p, const fsmc_mode* m) {
   FSMC_REQUIRE(p != nullptr, "params is NULL");
   FSMC_TRY(validate_mode(m, true));
   FSMC_REQUIRE(p->n_options >= 1 && p->n_options <= 65535, "n_options %d outside [1, 65535]", p->n_options);
-  FSMC_REQUIRE(p->n_paths >= 2, "n_paths %d < 2", p->n_paths);
+  FSMC_REQUIRE(p->n_paths >= 2 && p->n_paths <= (1 << 30), "n_paths %d outside [2, 2^30]", p->n_paths);
   FSMC_REQUIRE(p->n_steps >= 1, "n_steps %d < 1", p->n_steps);
   FSMC_REQUIRE(p->rho >= -1.0 && p->rho <= 1.0, "rho %g outside [-1, 1]", p->rho);
   const bool needs64 = m->rng == FSMC_RNG_BUFFER || (m->quirks & FSMC_QUIRK_ASIAN_RUNNING_VAL2);

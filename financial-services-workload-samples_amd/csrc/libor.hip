@@ -381,7 +381,7 @@ LiborPlan libor_plan(const fsmc_libor_params* p, const fsmc_mode* m) {
 int libor_validate(const fsmc_libor_params* p, const fsmc_mode* m, bool mrg_allowed = false) {
   FSMC_REQUIRE(p != nullptr, "params is NULL");
   FSMC_TRY(validate_mode(m, true, mrg_allowed));
-  FSMC_REQUIRE(p->n_paths >= 2, "n_paths %d < 2", p->n_paths);
+  FSMC_REQUIRE(p->n_paths >= 2 && p->n_paths <= (1 << 30), "n_paths %d outside [2, 2^30]", p->n_paths);
   FSMC_REQUIRE(p->L0 > 0.0, "L0 must be > 0");
   FSMC_REQUIRE(p->lambda >= 0.0, "lambda must be >= 0");
   return FSMC_OK;
@@ -466,7 +466,8 @@ int fsmc_libor_swaption_greeks_async(const fsmc_libor_params* p, const fsmc_mode
 int fsmc_libor_normals(fsmc_ctx* ctx, int n_paths, const fsmc_mode* m, double mean, double stddev, double* z) {
   FSMC_REQUIRE(ctx != nullptr && z != nullptr, "NULL argument");
   FSMC_TRY(validate_mode(m, false));
-  FSMC_REQUIRE(n_paths >= 1, "n_paths %d < 1", n_paths);
+  // one thread per four normals, indexed in int
+  FSMC_REQUIRE(n_paths >= 1 && n_paths <= 200000000, "n_paths %d outside [1, 2e8]", n_paths);
   std::lock_guard<std::mutex> lock(ctx->mu);
   const size_t bytes = (size_t)n_paths * NMAT * sizeof(double);
   FSMC_TRY(ctx_prepare(ctx, 0, bytes, 0, 0));

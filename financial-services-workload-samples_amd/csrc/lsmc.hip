@@ -544,8 +544,8 @@ int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K,
   FSMC_TRY(validate_mode(m, false));
   FSMC_REQUIRE(S0 && K && sigma && r && T && steps && paths, "NULL host pointer");
   for (int o = 0; o < n; ++o) {
-    FSMC_REQUIRE(steps[o] >= 1 && paths[o] >= 2, "option %d: steps %d / paths %d out of range", o, steps[o],
-                 paths[o]);
+    FSMC_REQUIRE(steps[o] >= 1 && paths[o] >= 2 && paths[o] <= (1 << 30),
+                 "option %d: steps %d / paths %d out of range", o, steps[o], paths[o]);
     FSMC_REQUIRE(S0[o] > 0 && K[o] > 0 && sigma[o] > 0 && T[o] > 0, "option %d: S0, K, sigma, T must be > 0", o);
   }
   pl->n = n;
@@ -575,6 +575,8 @@ int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K,
       pl->table.push_back(make_int4(o, c, (int)(q.cf_off + (long long)c * kChunk), std::min(kChunk, paths[o] - c * kChunk)));
     cfo += (paths[o] + 3) / 4 * 4;  // segments aligned and padded to whole groups of 4
   }
+  // table entries address cash flows with int offsets
+  FSMC_REQUIRE(cfo <= (long long)0x7fffffff, "total paths %lld over 2^31", cfo);
   pl->cf_elems = (size_t)cfo;
   pl->max_steps = steps[order[0]];
   pl->order = order;
@@ -737,7 +739,8 @@ int fsmc_american_lsmc_ref(fsmc_ctx* ctx, const fsmc_lsmc_ref_params* p, const f
                            double* american, double* european) {
   FSMC_REQUIRE(ctx && p && american && european, "NULL argument");
   FSMC_TRY(validate_mode(m, true, true));
-  FSMC_REQUIRE(p->n_paths >= 2 && p->n_steps >= 2, "n_paths %d / n_steps %d too small", p->n_paths, p->n_steps);
+  FSMC_REQUIRE(p->n_paths >= 2 && p->n_steps >= 2 && p->n_paths <= (1 << 30),
+               "n_paths %d / n_steps %d out of range", p->n_paths, p->n_steps);
   const bool mrg = m->rng == FSMC_RNG_MRG32K3A;
   const bool buffer = m->rng == FSMC_RNG_BUFFER || mrg;  // both read z from the normals slot
   FSMC_REQUIRE(m->rng != FSMC_RNG_BUFFER || z, "buffer mode needs z");
