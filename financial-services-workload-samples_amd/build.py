@@ -34,7 +34,9 @@ def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
     cmd = [HIPCC] + COMMON + ["-c", src, "-o", obj]
     if src.endswith(".cpp"):
-        cmd = [GXX] + HOST_FLAGS + ["-c", src, "-o", obj]
+        # a drop-in that manages device buffers itself includes the HIP runtime
+        hip = ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"] if "hip/hip_runtime.h" in open(src).read() else []
+        cmd = [GXX] + HOST_FLAGS + hip + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     # hipcc's host pass warns that it does not know the packed-fp32 feature.
     noise = "packed-fp32-ops' is not a recognized feature"
