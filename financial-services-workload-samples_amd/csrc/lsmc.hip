@@ -822,4 +822,59 @@ int fsmc_american_lsmc_ref(fsmc_ctx* ctx, const fsmc_lsmc_ref_params* p, const f
   return FSMC_OK;
 }
 
+int fsmc_american_paths_ref(fsmc_ctx* ctx, int n_paths, int n_steps, double spot, double fwd_log2, double vsdt_log2,
+                            const fsmc_mode* m, const double* z, void* const* rows) {
+  FSMC_REQUIRE(ctx && rows, "NULL argument");
+  for (int i = 1; i <= n_steps; ++i) FSMC_REQUIRE(rows[i], "rows[%d] is NULL", i);
+  FSMC_TRY(validate_mode(m, true, true));
+  FSMC_REQUIRE(n_paths >= 1 && n_steps >= 1 && n_paths <= (1 << 30), "n_paths %d / n_steps %d out of range",
+               n_paths, n_steps);
+  const bool mrg = m->rng == FSMC_RNG_MRG32K3A;
+  const bool buffer = m->rng == FSMC_RNG_BUFFER || mrg;
+  FSMC_REQUIRE(m->rng != FSMC_RNG_BUFFER || z, "buffer mode needs z");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const size_t N = (size_t)n_paths;
+  const bool f64 = m->precision == FSMC_FP64;
+  const size_t elems = (size_t)(n_steps + 1) * N;
+  // the same stream layout as fsmc_american_lsmc_ref
+  const long long nb_skip = (long long)(N / 1024);
+  const size_t zn = mrg ? N + (size_t)nb_skip * (size_t)(n_steps - 1) : buffer ? N * (size_t)n_steps : 0;
+  FSMC_TRY(ctx_prepare(ctx, 0, zn * sizeof(double), 0, elems * (f64 ? 8 : 4)));
+  hipStream_t s = ctx->stream;
+  if (mrg)
+    FSMC_TRY(mrg_gaussian_enqueue((uint32_t)m->seed, (uint64_t)m->stream * N, (long long)zn, 0.0, 1.0,
+                                  (double*)ctx->normals.ptr, s));
+  else if (buffer)
+    FSMC_HIP(hipMemcpyAsync(ctx->normals.ptr, z, zn * sizeof(double), hipMemcpyHostToDevice, s));
+  RefArgs r{};
+  r.n_paths = n_paths;
+  r.n_steps = n_steps;
+  r.spot = spot;
+  r.fwd_l2e = fwd_log2;
+  r.vsdt_l2e = vsdt_log2;
+  r.z = (const double*)ctx->normals.ptr;
+  r.zrow = mrg ? nb_skip : (long long)N;
+  r.k0 = (uint32_t)m->seed;
+  r.k1 = (uint32_t)(m->seed >> 32);
+  r.w = counter_w(kTagLsmc, m->stream);
+  r.a = ctx->workspace.ptr;
+  const int pb = (int)((N + kBlock - 1) / kBlock);
+  if (f64 && buffer)
+    hipLaunchKernelGGL((ref_paths_kernel<double, true>), dim3(pb), dim3(kBlock), 0, s, r);
+  else if (f64)
+    hipLaunchKernelGGL((ref_paths_kernel<double, false>), dim3(pb), dim3(kBlock), 0, s, r);
+  else if (buffer)
+    hipLaunchKernelGGL((ref_paths_kernel<float, true>), dim3(pb), dim3(kBlock), 0, s, r);
+  else
+    hipLaunchKernelGGL((ref_paths_kernel<float, false>), dim3(pb), dim3(kBlock), 0, s, r);
+  FSMC_HIP(hipGetLastError());
+  // row 0 (the spot) stays the caller's, as perform_sims leaves a[0]
+  const size_t row_bytes = N * (f64 ? 8 : 4);
+  for (int i = 1; i <= n_steps; ++i)
+    FSMC_HIP(hipMemcpyAsync(rows[i], (const char*)ctx->workspace.ptr + (size_t)i * row_bytes, row_bytes,
+                            hipMemcpyDeviceToHost, s));
+  FSMC_HIP(hipStreamSynchronize(s));
+  return FSMC_OK;
+}
+
 }  // extern "C"

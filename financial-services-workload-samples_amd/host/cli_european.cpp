@@ -9,11 +9,12 @@
 // <nthreads> takes noptions / nthreads options from glibc rand_r seeded with
 // 777 + t, S ~ U[5, 50], X ~ U[10, 25], T ~ U[1, 5] (:45-48, :208-226), with
 // r = 0.06 and sigma = 0.10 (:71-72).  All of them are priced in one call of
-// fsmc_gbm_european with path_length paths each (Philox normals; the block
-// length is validated as in the sample but the GPU does not block).  As in
-// the sample the batch is priced 4 times and the last pass is timed.  With
-// verbose != 0 the Black-Scholes comparison of :275-316 is printed.
-// -DUSE_DOUBLE selects the FP64 engine path, otherwise FP32.
+// the drop-in price_european with path_length paths each (Philox normals of
+// seed 777; the block length is validated as in the sample but the GPU does
+// not block).  As in the sample the batch is priced 4 times and the last pass
+// is timed.  With verbose != 0 the Black-Scholes comparison of :275-316 is
+// printed.  -DUSE_DOUBLE selects REAL = double and the FP64 engine path,
+// otherwise REAL = float and FP32.
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -22,13 +23,13 @@
 #include <cstring>
 #include <vector>
 
-#include "fsmc.h"
+#include "fsmc_dropin.hpp"
 
 #ifdef USE_DOUBLE
-static const int kPrecision = FSMC_FP64;
+typedef double tfloat;
 static const char* kName = "Double";
 #else
-static const int kPrecision = FSMC_FP32;
+typedef float tfloat;
 static const char* kName = "Single";
 #endif
 
@@ -99,29 +100,24 @@ int main(int argc, char** argv) {
   printf("Worker Threads   = %lld\n\n", nthreads);
 
   const size_t n = (size_t)(per_thread * nthreads);
-  std::vector<double> S(n), X(n), T(n), res(n), conf(n);
+  std::vector<tfloat> S(n), X(n), T(n), res(n), conf(n);
   for (long long t = 0; t < nthreads; ++t) {
     unsigned seed = 777u + (unsigned)t;
     for (long long i = 0; i < per_thread; ++i) {
       const size_t k = (size_t)(t * per_thread + i);
-      S[k] = uniform(5.0, 50.0, &seed);
-      X[k] = uniform(10.0, 25.0, &seed);
-      T[k] = uniform(1.0, 5.0, &seed);
+      S[k] = (tfloat)uniform(5.0, 50.0, &seed);
+      X[k] = (tfloat)uniform(10.0, 25.0, &seed);
+      T[k] = (tfloat)uniform(1.0, 5.0, &seed);
     }
   }
-  fsmc_ctx* ctx = nullptr;
-  if (fsmc_ctx_create(0, &ctx) != FSMC_OK) die("fsmc_ctx_create: %s\n", fsmc_last_error());
-  fsmc_gbm_params p = {(int)n, (int)rand_n, kRiskfree, kVolatility};
-  fsmc_mode m = {kPrecision, FSMC_RNG_PHILOX, 777, 0, 0};
   double elapsed = 0.0;
   for (int it = 0; it < kIters; ++it) {
     const auto t0 = std::chrono::steady_clock::now();
-    if (fsmc_gbm_european(ctx, &p, &m, S.data(), X.data(), T.data(), nullptr, res.data(), conf.data(), nullptr) !=
-        FSMC_OK)
-      die("fsmc_gbm_european: %s\n", fsmc_last_error());
+    price_european(S.data(), X.data(), T.data(), (tfloat)kRiskfree, (tfloat)kVolatility, (int)n, (int)rand_n,
+                   (int)block, res.data(), conf.data(), 777);
     elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (res[0] != res[0]) die("price_european failed\n");
   }
-  fsmc_ctx_destroy(ctx);
 
   if (verbose) {
     double sum_delta = 0.0, sum_ref = 0.0, reserve = 0.0, max_delta = 0.0;

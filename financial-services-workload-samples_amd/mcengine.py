@@ -149,6 +149,8 @@ def lib():
         L.fsmc_lsmc_plan_destroy.argtypes = [_VP]
         L.fsmc_american_lsmc.argtypes = [_VP, ctypes.c_int, PM] + [_D] * 5 + [_I, _I, _D, _D]
         L.fsmc_american_lsmc_ref.argtypes = [_VP, ctypes.POINTER(LsmcRefParams), PM, _D, _D, _D]
+        L.fsmc_american_paths_ref.argtypes = ([_VP] + [ctypes.c_int] * 2 + [ctypes.c_double] * 3 +
+                                              [PM, _D, ctypes.POINTER(_VP)])
         L.fsmc_batch_create.argtypes = [_VP, ctypes.POINTER(Request), ctypes.c_int, PM, ctypes.POINTER(BatchTuning),
                                         ctypes.POINTER(_VP)]
         L.fsmc_batch_run.argtypes = [_VP, _VP]
@@ -278,6 +280,19 @@ class Engine:
         _check(lib().fsmc_american_lsmc_ref(self._ctx, ctypes.byref(p), ctypes.byref(m), pz, ctypes.byref(am),
                                             ctypes.byref(eu)), "fsmc_american_lsmc_ref")
         return am.value, eu.value
+
+    def american_paths_ref(self, n_paths, n_steps, spot, fwd_log2, vsdt_log2, m, z=None):
+        """The sample's path matrix (perform_sims): array [n_steps + 1][n_paths]
+        in the mode's precision, row 0 the spot."""
+        dt = np.float64 if m.precision == FP64 else np.float32
+        a = np.full((n_steps + 1, n_paths), spot, dtype=dt)
+        rows = (_VP * (n_steps + 1))(*[None] + [a[i].ctypes.data for i in range(1, n_steps + 1)])
+        pz = None
+        if z is not None:
+            z, pz = _d(np.asarray(z).reshape(-1))
+        _check(lib().fsmc_american_paths_ref(self._ctx, n_paths, n_steps, spot, fwd_log2, vsdt_log2, ctypes.byref(m),
+                                             pz, rows), "fsmc_american_paths_ref")
+        return a
 
     # -- closed form and lattice ---------------------------------------------
     def black_scholes(self, S, X, T, sigma, r, m, greeks=False):

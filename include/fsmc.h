@@ -254,6 +254,14 @@ typedef struct fsmc_lsmc_ref_params {
  * pricings (the sample's loop index). */
 int fsmc_american_lsmc_ref(fsmc_ctx* ctx, const fsmc_lsmc_ref_params* p, const fsmc_mode* m, const double* z,
                            double* american, double* european);
+/* The path matrix of the same algorithm, perform_sims (utilities.cpp:42-88,
+ * declared utilities.h:34): a[s][p] = a[s-1][p] 2^(fwd_log2 + vsdt_log2 z)
+ * from a[0][p] = spot, computed on the GPU and copied to the caller's host
+ * rows: rows[s], s = 1..n_steps, each n_paths REALs (double for FSMC_FP64,
+ * float for FSMC_FP32); rows[0] is neither read nor written.  The normals
+ * as fsmc_american_lsmc_ref takes them (buffer mode: z[(s-1)*n_paths + p]). */
+int fsmc_american_paths_ref(fsmc_ctx* ctx, int n_paths, int n_steps, double spot, double fwd_log2, double vsdt_log2,
+                            const fsmc_mode* m, const double* z, void* const* rows);
 
 /* ---- Closed form and lattice ---------------------------------------------------
  * Black-Scholes calls and puts (BlackScholes/BlackScholesDP.cpp:177-201) for

@@ -55,4 +55,67 @@ void MonteCarloTimeStep(double* h_CallResult, double* h_CallConfidence, double* 
 void MonteCarloTimeStep(float* h_CallResult, float* h_CallConfidence, float* S, float* X, float* T,
                         float* initialVol, float* kappaVals, float* thetaVals, float* xiVals);
 
+// MonteCarloEuropeanOptions: the sample prices inside main
+// (MonteCarloInsideBlockingDP.cpp:229-270), so this entry point is new: it
+// prices nOpt GBM European calls (spot S, strike X, expiry T, rate r,
+// volatility sigma) with nPaths paths each and writes the discounted mean
+// payoff and the 95 % confidence half-width, the sample's CallResultList and
+// CallConfidenceList (:251-268).  blockLen is validated as the sample
+// validates RAND_BLOCK_LENGTH (:165-174: at least 16, a multiple of 16, at
+// most nPaths, dividing nPaths); the GPU does not block.  Normals are the
+// engine's Philox stream of `seed`.  The double overload runs FP64, the float
+// one FP32.
+void price_european(const double* S, const double* X, const double* T, double r, double sigma, int nOpt,
+                    int nPaths, int blockLen, double* CallResult, double* CallConfidence,
+                    unsigned long long seed = 777);
+void price_european(const float* S, const float* X, const float* T, float r, float sigma, int nOpt, int nPaths,
+                    int blockLen, float* CallResult, float* CallConfidence, unsigned long long seed = 777);
+
+namespace fsmc_dropin {
+
+// MonteCarloAmericanOptions/utilities.h:23-24 and main.cpp:103 -- the shape
+// perform_sims fills and its streams.  `pricing` counts perform_sims calls:
+// call c reads run c of numPaths draws of every stream, as the sample's
+// streams advance by numPaths per loop; each call increments it.
+struct AmericanConfig {
+  int num_paths = 131072 * 2 * 2;  // numPaths
+  int num_steps = 256;             // numSteps
+  unsigned seed = 12345;           // vslNewStream(VSL_BRNG_MRG32K3A, 12345)
+  unsigned pricing = 0;            // perform_sims calls so far
+};
+AmericanConfig& american_config();
+
+}  // namespace fsmc_dropin
+
+// MonteCarloAmericanOptions/utilities.h:34 -- fills a[s][p], s = 1..numSteps,
+// p < numPaths, with a[s][p] = a[s-1][p] * exp2(fwdFactor + volSqrtdt * z)
+// (utilities.cpp:42-88) on the GPU.  a[0] is left as it is and must hold one
+// spot price in every column (main.cpp:84-87 fills it so); otherwise nothing
+// is written and an error is printed.  z is the sample's single-thread draw:
+// MRG32k3a seed 12345, step s on stream s-1 skipped ahead by
+// (numPaths / 1024) (s-1) (main.cpp:93-106), with ICDF normals generated on
+// the GPU.  `streams` and `nthreads` are not read: MKL stream state is opaque,
+// so the streams' position is american_config().pricing.
+void perform_sims(double** a, void** streams, int nthreads, double fwdFactor, double volSqrtdt);
+void perform_sims(float** a, void** streams, int nthreads, float fwdFactor, float volSqrtdt);
+
+// MonteCarloAmericanOptions/utilities.h:32 -- a = x^-1 for a 3x3 matrix by
+// cofactors over the determinant (utilities.cpp:90-123); prints "Trying to
+// invert non invertible matrix" when |det| < 1e-8 and divides regardless.
+// numRows and numCols are not read.
+void invert3_arr(double a[][3], double x[][3], int numRows, int numCols);
+void invert3_arr(float a[][3], float x[][3], int numRows, int numCols);
+
+// MonteCarloAmericanOptions: the sweep is inlined in the sample's main
+// (main.cpp:117-279), so this entry point is new: one pricing of the sample's
+// algorithm (paths, lagged quadratic regression with ridge 0.01, cofactor
+// inverse) for an American put, writing the sample's "American" and
+// "European" prices (main.cpp:294-295).  Normals as perform_sims draws them
+// with `pricing` loops already drawn.  The double overload runs FP64, the
+// float one FP32.
+void price_american_lsmc(int numPaths, int numSteps, double spot, double strike, double vol, double maturity,
+                         double* american, double* european, unsigned pricing = 0);
+void price_american_lsmc(int numPaths, int numSteps, float spot, float strike, float vol, float maturity,
+                         float* american, float* european, unsigned pricing = 0);
+
 #endif

@@ -63,7 +63,9 @@ def test_american_command_line(engine, build):
     # the printed prices are those of the last loop (50) on the sample's own
     # stream (its overlapping MRG32k3a streams correlate paths, so they are not
     # iid estimates of the Black-Scholes put); 9 significant digits printed
+    # of a REAL (float in the sp build: rounding to 6e-8)
     prec = mc.FP64 if build == "dp" else mc.FP32
     m = mc.mode(prec, mc.RNG_MRG32K3A, seed=12345, stream=50)
     ram, reu = engine.american_lsmc_ref(524288, 256, 100.0, 110.0, .25, 2.0, m)
-    assert am == pytest.approx(ram, rel=2e-8) and eu == pytest.approx(reu, rel=2e-8)
+    tol = 2e-8 if build == "dp" else 1e-7
+    assert am == pytest.approx(ram, rel=tol) and eu == pytest.approx(reu, rel=tol)

@@ -1,0 +1,151 @@
+"""The C++ drop-ins of the European and American samples on the GPU, called
+through their mangled C++ symbols as a sample's object files call them.
+
+perform_sims (utilities.cpp:42-88) fills the caller's rows with the path
+matrix on the sample's own MRG32k3a stream layout; it is checked against the
+oracle's MRG32k3a + inverse-CDF normals pushed through the sample's
+recursion in numpy (FP64 to 1e-11 relative: the normals agree to ~1e-15 and
+exp2 to an ulp per step), FP32 against FP64 to 2e-5.  price_american_lsmc and
+price_european must equal the C-ABI calls they wrap bit for bit."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import mcengine as mc
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+LOG2E = 1.4426950408889634
+
+
+class AmericanConfig(ctypes.Structure):
+    _fields_ = [("num_paths", ctypes.c_int), ("num_steps", ctypes.c_int), ("seed", ctypes.c_uint),
+                ("pricing", ctypes.c_uint)]
+
+
+def _cfg():
+    fn = mc.lib()._ZN11fsmc_dropin15american_configEv
+    fn.restype = ctypes.POINTER(AmericanConfig)
+    return fn().contents
+
+
+def _sample_paths(n_paths, n_steps, spot, fwd, vs, loop):
+    # utilities.cpp:52-84 single-threaded: step s of path p takes stream
+    # position (n_paths // 1024) (s - 1) + p + loop n_paths
+    nb = n_paths // 1024
+    g = orc.mrg32k3a_gaussian(12345, loop * n_paths, n_paths + nb * (n_steps - 1))
+    a = np.empty((n_steps + 1, n_paths))
+    a[0] = spot
+    for s in range(1, n_steps + 1):
+        a[s] = a[s - 1] * np.exp2(fwd + vs * g[nb * (s - 1):nb * (s - 1) + n_paths])
+    return a
+
+
+def _fwd_vs(vol=0.25, maturity=2.0, steps=256):
+    # main.cpp:55-59
+    dt = maturity / steps
+    return -0.5 * vol * vol * dt * LOG2E, vol * np.sqrt(dt) * LOG2E
+
+
+@pytest.mark.parametrize("n_paths,n_steps,loop", [(4096, 32, 0), (5120, 16, 3), (1300, 8, 1)])
+def test_paths_ref_on_sample_stream(engine, n_paths, n_steps, loop):
+    fwd, vs = _fwd_vs(steps=n_steps)
+    want = _sample_paths(n_paths, n_steps, 100.0, fwd, vs, loop)
+    m = mc.mode(mc.FP64, mc.RNG_MRG32K3A, seed=12345, stream=loop)
+    got = engine.american_paths_ref(n_paths, n_steps, 100.0, fwd, vs, m)
+    assert np.max(np.abs(got / want - 1)) < 1e-11
+    # the same normals through buffer mode give the same matrix bit for bit
+    nb = n_paths // 1024
+    z = engine.mrg32k3a_gaussian(12345, loop * n_paths, n_paths + nb * (n_steps - 1))
+    zb = np.stack([z[nb * j:nb * j + n_paths] for j in range(n_steps)])
+    assert np.array_equal(engine.american_paths_ref(n_paths, n_steps, 100.0, fwd, vs, mc.mode(mc.FP64, mc.RNG_BUFFER),
+                                                    zb), got)
+    g32 = engine.american_paths_ref(n_paths, n_steps, 100.0, fwd, vs, mc.mode(mc.FP32, mc.RNG_MRG32K3A, seed=12345,
+                                                                                stream=loop))
+    assert g32.dtype == np.float32 and np.max(np.abs(g32 / got - 1)) < 2e-5
+
+
+def test_paths_ref_matches_lsmc_ref_sweep(engine):
+    # the matrix perform_sims returns drives the sample's sweep: the oracle's
+    # sweep on the oracle's normals equals the GPU's full LSMC on the same stream
+    n_paths, n_steps = 4096, 24
+    m = mc.mode(mc.FP64, mc.RNG_MRG32K3A, seed=12345, stream=2)
+    am, eu = engine.american_lsmc_ref(n_paths, n_steps, 100.0, 110.0, .25, 2.0, m)
+    nb = n_paths // 1024
+    g = orc.mrg32k3a_gaussian(12345, 2 * n_paths, n_paths + nb * (n_steps - 1))
+    oam, oeu = orc.lsmc_ref(n_paths, n_steps, 100.0, 110.0, .25, 2.0,
+                            np.stack([g[nb * j:nb * j + n_paths] for j in range(n_steps)]))
+    assert abs(am - oam) <= 1e-12 * abs(oam) and abs(eu - oeu) <= 1e-12 * abs(oeu)
+
+
+@pytest.mark.parametrize("real", [np.float64, np.float32])
+def test_perform_sims_dropin(engine, real):
+    cfg = _cfg()
+    saved = (cfg.num_paths, cfg.num_steps, cfg.pricing)
+    n_paths, n_steps = 2048, 12
+    cfg.num_paths, cfg.num_steps, cfg.pricing = n_paths, n_steps, 0
+    try:
+        ct = ctypes.c_double if real == np.float64 else ctypes.c_float
+        sym = "_Z12perform_simsPPdPPvidd" if real == np.float64 else "_Z12perform_simsPPfPPviff"
+        fn = getattr(mc.lib(), sym)
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ct, ct]
+        fn.restype = None
+        fwd, vs = _fwd_vs(steps=n_steps)
+        prec = mc.FP64 if real == np.float64 else mc.FP32
+        # the sample's main calls it once per loop on one stream table
+        # (main.cpp:117-121): loop l reads run l of every stream
+        for loop in range(2):
+            a = np.full((n_steps + 1, n_paths), 100.0, dtype=real)
+            rows = (ctypes.c_void_p * (n_steps + 1))(*[a[i].ctypes.data for i in range(n_steps + 1)])
+            fn(rows, None, 8, fwd, vs)
+            assert cfg.pricing == loop + 1
+            want = engine.american_paths_ref(n_paths, n_steps, 100.0, float(real(fwd)), float(real(vs)),
+                                             mc.mode(prec, mc.RNG_MRG32K3A, seed=12345, stream=loop))
+            assert np.array_equal(a, want)
+            assert np.all(a[0] == 100.0)
+    finally:
+        cfg.num_paths, cfg.num_steps, cfg.pricing = saved
+
+
+@pytest.mark.parametrize("real", [np.float64, np.float32])
+def test_price_american_lsmc_dropin(engine, real):
+    ct = ctypes.c_double if real == np.float64 else ctypes.c_float
+    sym = "_Z19price_american_lsmciiddddPdS_j" if real == np.float64 else "_Z19price_american_lsmciiffffPfS_j"
+    fn = getattr(mc.lib(), sym)
+    fn.argtypes = [ctypes.c_int, ctypes.c_int] + [ct] * 4 + [ctypes.POINTER(ct)] * 2 + [ctypes.c_uint]
+    fn.restype = None
+    am, eu = ct(), ct()
+    fn(8192, 32, 100.0, 110.0, 0.25, 2.0, ctypes.byref(am), ctypes.byref(eu), 1)
+    prec = mc.FP64 if real == np.float64 else mc.FP32
+    wam, weu = engine.american_lsmc_ref(8192, 32, 100.0, 110.0, .25, 2.0,
+                                        mc.mode(prec, mc.RNG_MRG32K3A, seed=12345, stream=1))
+    assert am.value == real(wam) and eu.value == real(weu)
+    # no rate in the sample's dynamics (main.cpp:55-59): early exercise is
+    # worth nothing, so the American price is the European one less the
+    # regression's low bias
+    assert 0.95 * eu.value < am.value < 1.01 * eu.value and 15.0 < eu.value < 25.0
+
+
+@pytest.mark.parametrize("real", [np.float64, np.float32])
+def test_price_european_dropin(engine, real):
+    ct = ctypes.c_double if real == np.float64 else ctypes.c_float
+    sym = "_Z14price_europeanPKdS0_S0_ddiiiPdS1_y" if real == np.float64 else "_Z14price_europeanPKfS0_S0_ffiiiPfS1_y"
+    fn = getattr(mc.lib(), sym)
+    P = ctypes.POINTER(ct)
+    fn.argtypes = [P, P, P, ct, ct, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_ulonglong]
+    fn.restype = None
+    rng = np.random.default_rng(7)
+    n, paths = 64, 65536
+    S, X, T = rng.uniform(5, 50, n), rng.uniform(10, 25, n), rng.uniform(1, 5, n)
+    S, X, T = (v.astype(real) for v in (S, X, T))
+    res, conf = np.zeros(n, dtype=real), np.zeros(n, dtype=real)
+    fn(*[v.ctypes.data_as(P) for v in (S, X, T)], 0.06, 0.10, n, paths, 1024, res.ctypes.data_as(P),
+       conf.ctypes.data_as(P), 777)
+    prec = mc.FP64 if real == np.float64 else mc.FP32
+    wr, wc = engine.gbm_european(S.astype(np.float64), X.astype(np.float64), T.astype(np.float64), float(real(0.06)),
+                                 float(real(0.10)), paths, mc.mode(prec, mc.RNG_PHILOX, seed=777))
+    assert np.array_equal(res, wr.astype(real)) and np.array_equal(conf, wc.astype(real))
+    bs = np.array([orc.bs_call(float(a), float(b), float(c), 0.06, 0.10) for a, b, c in zip(S, X, T)])
+    assert np.all(np.abs(res - bs) <= 5 * np.maximum(conf, 1e-6) / 1.96 + 1e-4)
