@@ -166,6 +166,24 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Sums n records of nv <= NV doubles (record b at p[b * nv]) with one
+// wavefront: lane l adds records l, l + 64, ... in order, then the xor tree
+// (every lane ends with the same bits).  The order depends only on n, so the
+// result is reproducible; called by whole waves.
+template <int NV>
+__device__ __forceinline__ void wave_combine(const double* p, int n, int nv, double (&s)[NV]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k] = 0.0;
+  for (int b = lane; b < n; b += 64) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      if (k < nv) s[k] += p[(size_t)b * nv + k];
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k] = wave_sum(s[k]);
+}
+
 // Sums NV values across a workgroup of blockDim.x threads (multiple of 64).
 // Result valid in thread 0.  `lds` must hold NV * (blockDim.x / 64) doubles.
 template <int NV>

@@ -114,16 +114,16 @@ __device__ __forceinline__ void mean_se(double s, double s2, double N, double f,
   *se = fabs(f) * sqrt(var > 0.0 ? var : 0.0) / sqrt(N);
 }
 
-// One thread per option; partials combined in block order.
-__global__ void gbm_finalize_kernel(int n_options, int nbx, int nv, int n_paths, double r,
-                                    const double* S, const double* X, const double* T, const double* partials,
-                                    double* result, double* confidence, double* greeks) {
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= n_options) return;
-  double s[7] = {0, 0, 0, 0, 0, 0, 0};
-  const double* p = partials + (size_t)o * nbx * nv;
-  for (int b = 0; b < nbx; ++b)
-    for (int k = 0; k < nv; ++k) s[k] += p[b * nv + k];
+// One wavefront per option (workgroup of 64): the option's workgroup
+// partials combined by wave_combine (fixed order).
+__global__ __launch_bounds__(64) void gbm_finalize_kernel(int nbx, int nv, int n_paths, double r, const double* S,
+                                                         const double* X, const double* T,
+                                                         const double* partials, double* result,
+                                                         double* confidence, double* greeks) {
+  const int o = blockIdx.x;
+  double s[7];
+  wave_combine<7>(partials + (size_t)o * nbx * nv, nbx, nv, s);
+  if (threadIdx.x != 0) return;
   const double N = (double)n_paths;
   // MonteCarloInsideBlockingDP.cpp:266-269.
   const double exprt = exp2(-r * kLog2e * T[o]);
@@ -222,9 +222,8 @@ int fsmc_gbm_european_async(const fsmc_gbm_params* p, const fsmc_mode* m, const 
     else greeks ? launch<float, false, true>(a, p->n_options, g.nbx, s) : launch<float, false, false>(a, p->n_options, g.nbx, s);
   }
   FSMC_HIP(hipGetLastError());
-  const int fb = 128;
-  hipLaunchKernelGGL(gbm_finalize_kernel, dim3((p->n_options + fb - 1) / fb), dim3(fb), 0, s, p->n_options, g.nbx,
-                     g.nv, p->n_paths, p->riskfree, d_S, d_X, d_T, a.partials, d_result, d_conf, d_greeks);
+  hipLaunchKernelGGL(gbm_finalize_kernel, dim3(p->n_options), dim3(64), 0, s, g.nbx, g.nv, p->n_paths,
+                     p->riskfree, d_S, d_X, d_T, a.partials, d_result, d_conf, d_greeks);
   FSMC_HIP(hipGetLastError());
   return FSMC_OK;
 }

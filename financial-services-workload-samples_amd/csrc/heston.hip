@@ -84,12 +84,15 @@ __device__ __forceinline__ void mean_se(double s, double s2, double N, double f,
   *se = fabs(f) * sqrt(var > 0.0 ? var : 0.0) / sqrt(N);
 }
 
-__global__ void heston_finalize_kernel(int n_options, int nbx, int n_paths, double r, const double* T,
-                                       const double* partials, const double* chunks, int running_val2,
-                                       double* result, double* confidence) {
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= n_options) return;
+// One wavefront per option (workgroup of 64).
+__global__ __launch_bounds__(64) void heston_finalize_kernel(int nbx, int n_paths, double r, const double* T,
+                                                            const double* partials, const double* chunks,
+                                                            int running_val2, double* result, double* confidence) {
+  const int o = blockIdx.x;
   const double N = (double)n_paths;
+  double s[2];
+  if (!running_val2) wave_combine<2>(partials + (size_t)o * nbx * 2, nbx, 2, s);  // uniform branch
+  if (threadIdx.x != 0) return;
   const double exprt = exp(-r * T[o]);
   if (running_val2) {
     // MCAsianOptions.cpp:184-201 verbatim in meaning: val accumulates chunk
@@ -105,12 +108,7 @@ __global__ void heston_finalize_kernel(int n_options, int nbx, int n_paths, doub
     const double stddev = sqrt((N * val2 - val * val) * (1.0 / (N * (N - 1.0))));
     confidence[o] = exprt * 1.96 * stddev * (1.0 / sqrt(N));
   } else {
-    double s0 = 0.0, s1 = 0.0;
-    const double* p = partials + (size_t)o * nbx * 2;
-    for (int b = 0; b < nbx; ++b) {
-      s0 += p[2 * b];
-      s1 += p[2 * b + 1];
-    }
+    const double s0 = s[0], s1 = s[1];
     result[o] = exprt * s0 / N;
     const double stddev = sqrt((N * s1 - s0 * s0) * (1.0 / (N * (N - 1.0))));
     confidence[o] = exprt * 1.96 * stddev * (1.0 / sqrt(N));
@@ -149,15 +147,16 @@ __global__ __launch_bounds__(kBlock) void heston_adjoint_kernel(HestonArgs a, Re
   }
 }
 
-__global__ void heston_greeks_finalize_kernel(int n_options, int nbx, int n_paths, double r, const double* T,
-                                              const double* partials, double* result, double* confidence,
-                                              double* greeks) {
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= n_options) return;
-  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const double* p = partials + (size_t)o * nbx * 8;
-  for (int b = 0; b < nbx; ++b)
-    for (int k = 0; k < 8; ++k) s[k] += p[8 * b + k];
+// One wavefront per option (workgroup of 64); the price sums are combined
+// in the same order as heston_finalize_kernel's, so both report the same
+// price for the same paths.
+__global__ __launch_bounds__(64) void heston_greeks_finalize_kernel(int nbx, int n_paths, double r, const double* T,
+                                                                   const double* partials, double* result,
+                                                                   double* confidence, double* greeks) {
+  const int o = blockIdx.x;
+  double s[8];
+  wave_combine<8>(partials + (size_t)o * nbx * 8, nbx, 8, s);
+  if (threadIdx.x != 0) return;
   const double N = (double)n_paths;
   const double exprt = exp(-r * T[o]);
   double se;
@@ -274,8 +273,8 @@ int fsmc_heston_asian_async(const fsmc_heston_params* p, const fsmc_mode* m, con
       hipLaunchKernelGGL(heston_adjoint_kernel<float>, dim3(h.nbx, p->n_options), dim3(kBlock), 0, s, a,
                          (float*)((char*)d_ws + h.gpartial_bytes));
     FSMC_HIP(hipGetLastError());
-    hipLaunchKernelGGL(heston_greeks_finalize_kernel, dim3((p->n_options + 63) / 64), dim3(64), 0, s,
-                       p->n_options, h.nbx, p->n_paths, p->riskfree, d_T, a.partials, d_result, d_conf, d_greeks);
+    hipLaunchKernelGGL(heston_greeks_finalize_kernel, dim3(p->n_options), dim3(64), 0, s, h.nbx, p->n_paths,
+                       p->riskfree, d_T, a.partials, d_result, d_conf, d_greeks);
     FSMC_HIP(hipGetLastError());
     return FSMC_OK;
   }
@@ -289,9 +288,8 @@ int fsmc_heston_asian_async(const fsmc_heston_params* p, const fsmc_mode* m, con
            : launch_q<float, false>(a, q4, ch, p->n_options, h.nbx, s);
   }
   FSMC_HIP(hipGetLastError());
-  const int fb = 64;
-  hipLaunchKernelGGL(heston_finalize_kernel, dim3((p->n_options + fb - 1) / fb), dim3(fb), 0, s, p->n_options,
-                     h.nbx, p->n_paths, p->riskfree, d_T, a.partials, a.chunks, ch ? 1 : 0, d_result, d_conf);
+  hipLaunchKernelGGL(heston_finalize_kernel, dim3(p->n_options), dim3(64), 0, s, h.nbx, p->n_paths, p->riskfree,
+                     d_T, a.partials, a.chunks, ch ? 1 : 0, d_result, d_conf);
   FSMC_HIP(hipGetLastError());
   return FSMC_OK;
 }

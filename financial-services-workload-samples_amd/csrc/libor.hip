@@ -326,18 +326,22 @@ __global__ __launch_bounds__(256) void libor_reduce_kernel(int n_paths, const Re
   }
 }
 
-__global__ void libor_finalize_kernel(int n_paths, int nchunks, const double* partials, double* mean, double* se) {
-  const int q = threadIdx.x;
-  if (q > NN) return;
-  double s0 = 0.0, s1 = 0.0;
-  for (int c = 0; c < nchunks; ++c) {
-    s0 += partials[((size_t)q * nchunks + c) * 2];
-    s1 += partials[((size_t)q * nchunks + c) * 2 + 1];
-  }
+// One wavefront per output q (workgroup of 64; q = 0 the value, q = 1..NN
+// the deltas): the chunk partials combined by wave_combine, mean and standard
+// error written straight into the caller's value[2] / deltas[2 NN].
+__global__ __launch_bounds__(64) void libor_finalize_kernel(int n_paths, int nchunks, const double* partials,
+                                                           double* value, double* deltas) {
+  const int q = blockIdx.x;
+  double s[2];
+  wave_combine<2>(partials + (size_t)q * nchunks * 2, nchunks, 2, s);
+  if (threadIdx.x != 0) return;
   const double N = (double)n_paths;
-  mean[q] = s0 / N;
-  double var = (N * s1 - s0 * s0) / (N * (N - 1.0));
-  se[q] = sqrt(var > 0.0 ? var : 0.0) / sqrt(N);
+  const double mean = s[0] / N;
+  const double var = (N * s[1] - s[0] * s[0]) / (N * (N - 1.0));
+  const double se = sqrt(var > 0.0 ? var : 0.0) / sqrt(N);
+  double* dst = q == 0 ? value : deltas + (q - 1);
+  dst[0] = mean;
+  dst[q == 0 ? 1 : NN] = se;
 }
 
 // Normals of the LIBOR Philox layout written out as a [n_paths][40] buffer,
@@ -358,7 +362,7 @@ __global__ void libor_normals_kernel(int n_paths, double mean, double stddev, ui
 constexpr int kReduceChunks = 64;
 
 struct LiborPlan {
-  size_t real, traj_bytes, out_bytes, part_bytes, res_bytes, ws;
+  size_t real, traj_bytes, out_bytes, part_bytes, ws;
 };
 
 bool stores_trajectory(const fsmc_libor_params* p, const fsmc_mode* m) {
@@ -373,8 +377,7 @@ LiborPlan libor_plan(const fsmc_libor_params* p, const fsmc_mode* m) {
   l.traj_bytes = stores_trajectory(p, m) ? align_up((size_t)kSlots * P * l.real, 256) : 0;
   l.out_bytes = align_up((size_t)(NN + 1) * P * l.real, 256);
   l.part_bytes = align_up((size_t)(NN + 1) * kReduceChunks * 2 * sizeof(double), 256);
-  l.res_bytes = align_up(2 * (NN + 1) * sizeof(double), 256);
-  l.ws = l.traj_bytes + l.out_bytes + l.part_bytes + l.res_bytes;
+  l.ws = l.traj_bytes + l.out_bytes + l.part_bytes;
   return l;
 }
 
@@ -439,7 +442,6 @@ int fsmc_libor_swaption_greeks_async(const fsmc_libor_params* p, const fsmc_mode
   a.traj = ws;
   a.out = ws + l.traj_bytes;
   double* part = (double*)(ws + l.traj_bytes + l.out_bytes);
-  double* res = (double*)(ws + l.traj_bytes + l.out_bytes + l.part_bytes);
   const bool q = (m->quirks & FSMC_QUIRK_LIBOR_INPLACE_ADJ) != 0;
   const bool f64 = m->precision == FSMC_FP64;
   if (f64) launch_all<double>(a, buffer, q, s);
@@ -452,14 +454,10 @@ int fsmc_libor_swaption_greeks_async(const fsmc_libor_params* p, const fsmc_mode
     hipLaunchKernelGGL(libor_reduce_kernel<float>, dim3(kReduceChunks, NN + 1), dim3(256), 0, s, p->n_paths,
                        (const float*)a.out, part);
   FSMC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(libor_finalize_kernel, dim3(1), dim3(128), 0, s, p->n_paths, kReduceChunks, part, res,
-                     res + NN + 1);
+  // value and its standard error, the 80 deltas and their standard errors
+  hipLaunchKernelGGL(libor_finalize_kernel, dim3(NN + 1), dim3(64), 0, s, p->n_paths, kReduceChunks, part, d_value,
+                     d_deltas);
   FSMC_HIP(hipGetLastError());
-  // value, its standard error, the 80 deltas and their standard errors
-  FSMC_HIP(hipMemcpyAsync(d_value, res, sizeof(double), hipMemcpyDeviceToDevice, s));
-  FSMC_HIP(hipMemcpyAsync(d_value + 1, res + NN + 1, sizeof(double), hipMemcpyDeviceToDevice, s));
-  FSMC_HIP(hipMemcpyAsync(d_deltas, res + 1, NN * sizeof(double), hipMemcpyDeviceToDevice, s));
-  FSMC_HIP(hipMemcpyAsync(d_deltas + NN, res + NN + 2, NN * sizeof(double), hipMemcpyDeviceToDevice, s));
   return FSMC_OK;
 }
 
