@@ -150,8 +150,8 @@ def test_batch_errors(engine):
         mc.Batch(engine, [mc.gbm_request(100, 10, 10, 1, .02, .3)], mc.mode(), tuning=(1024, 0, 0, 17))
 
 
-def test_config4_full_batch(engine):
-    # Configuration 4 at its full size (4096 requests, FP32, Greeks): GBM
+def test_serving_batch_4096_requests(engine):
+    # The serving batch at its full size (4096 requests, FP32, Greeks): GBM
     # requests against Black-Scholes price, delta, vega and rho in units of
     # their standard errors; Heston requests against the FP64 batch on the
     # same stream; one micro-batch, so every request completes together

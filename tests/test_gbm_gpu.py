@@ -89,8 +89,8 @@ def test_philox_fp32_vs_fp64(engine):
     assert np.all(err <= 1e-5 * np.abs(g64[:, :3]) + 1e-3 * g64[:, 3:])
 
 
-def test_config0_against_black_scholes(engine):
-    # Config 0 at full shape: 1024 options x 262144 paths, FP32 Philox.
+def test_1024x262144_against_black_scholes(engine):
+    # Full shape: 1024 options x 262144 paths, FP32 Philox.
     S, X, T = _opts(1024, 10)
     r, sig, n = 0.02, 0.3, 262144
     res, conf, g = engine.gbm_european(S, X, T, r, sig, n, mc.mode(mc.FP32, seed=2024), greeks=True)

@@ -82,8 +82,8 @@ def test_philox_fp64_matches_oracle(engine, n_paths, n_steps):
     assert _rel(conf, oconf) <= 1e-10
 
 
-def test_philox_fp32_vs_fp64_config1(engine):
-    # Config 1 at full shape: 64 options x 131072 paths x 365 steps.
+def test_philox_fp32_vs_fp64_64x131072x365(engine):
+    # Full shape: 64 options x 131072 paths x 365 steps.
     P = _params(64, 6)
     r32, c32 = _call(engine, P, .03, -.7, 131072, 365, mc.mode(mc.FP32, seed=1))
     r64, c64 = _call(engine, P, .03, -.7, 131072, 365, mc.mode(mc.FP64, seed=1))

@@ -103,8 +103,8 @@ def test_normals_seam_equals_philox(engine):
     assert np.allclose(zs, 1.0 + 2.0 * z, rtol=1e-15, atol=1e-15)
 
 
-def test_config2_fp32_vs_fp64(engine):
-    # Config 2: 262144 paths, L0 = 0.05, lambda = 0.2, FP32 vs FP64 on the
+def test_portfolio_262144_paths_fp32_vs_fp64(engine):
+    # 262144 paths, L0 = 0.05, lambda = 0.2, FP32 vs FP64 on the
     # same Philox stream; sum of deltas vs a flat-curve central difference.
     n = 262144
     v32, s32, d32, ds32 = engine.libor(n, mc.mode(mc.FP32, seed=1), L0=0.05)

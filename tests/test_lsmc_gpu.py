@@ -84,8 +84,8 @@ def test_batched_longstaff_schwartz_table(engine):
         assert p == pytest.approx([4.472, 2.313, 1.110], abs=0.03)
 
 
-def test_config3_against_bermudan_lattice(engine):
-    # Config 3: 256 puts, steps ~U{50..250}, paths ~U{16384..262144}, FP32.
+def test_256_uneven_puts_against_bermudan_lattice(engine):
+    # 256 puts, steps ~U{50..250}, paths ~U{16384..262144}, FP32.
     P = _opts(256, 3)
     p, se, eu = _call(engine, P, mc.mode(mc.FP32, seed=3))
     p64, se64, eu64 = _call(engine, P, mc.mode(mc.FP64, seed=3))
