@@ -149,3 +149,68 @@ def test_price_european_rejects_block_lengths(lib, capfd, block):
        conf.ctypes.data_as(D), 777)
     assert np.isnan(res[0]) and np.isnan(conf[0])
     assert "blockLen" in capfd.readouterr().err
+
+
+# The samples' own prototypes (signatures as their headers declare them,
+# MKL's stream pointer being void *), compiled beside fsmc_dropin.hpp where
+# the sample would include it, and linked against the library: a sample built
+# this way resolves every call to the drop-ins.
+SAMPLE_TUS = {
+    "libor": """
+        void generateRand(REAL* z, int num, REAL mean, REAL stddev, int seed = 12345);
+        void liborSwaptionGreeks(REAL *z, int num, REAL mean, REAL std_dev, int seed, REAL *d_v, REAL *d_Lb,
+                                 int numPaths);
+        int main(int argc, char**) {
+          if (argc > 99) {
+            REAL z[40], v, lb;
+            generateRand(z, 40, 0, 1);
+            liborSwaptionGreeks(z, 40, 0, 1, 0, &v, &lb, 1);
+          }
+          return 0;
+        }""",
+    "american": """
+        typedef void* VSLStreamStatePtr;
+        void invert3_arr(REAL a[][3], REAL x[][3], int numRows, int numCols);
+        void perform_sims(REAL **a, VSLStreamStatePtr *streams, int nthreads, REAL fwdFactor, REAL volSqrtdt);
+        #include "fsmc_dropin.hpp"
+        int main(int argc, char**) {
+          if (argc > 99) {
+            REAL x[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, y[3][3], am, eu;
+            REAL* rows[2] = {nullptr, nullptr};
+            invert3_arr(y, x, 3, 3);
+            perform_sims(rows, nullptr, 1, 0, 0);
+            price_american_lsmc(1024, 2, REAL(100), REAL(110), REAL(.25), REAL(2), &am, &eu);
+          }
+          return 0;
+        }""",
+    "asian_european": """
+        #include "fsmc_dropin.hpp"
+        int main(int argc, char**) {
+          if (argc > 99) {
+            REAL r[1], c[1], p[1] = {1};
+            MonteCarloTimeStep(r, c, p, p, p, p, p, p, p);
+            price_european(p, p, p, REAL(.02), REAL(.3), 1, 1024, 1024, r, c);
+          }
+          return 0;
+        }""",
+}
+
+
+@pytest.mark.parametrize("real", ["double", "float"])
+@pytest.mark.parametrize("sample", sorted(SAMPLE_TUS))
+def test_sample_prototypes_link(lib, sample, real):
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no host C++ compiler")
+    out_dir = os.path.join(os.path.dirname(mcengine.LIB_PATH), "..", "_build", "sample_link")
+    os.makedirs(out_dir, exist_ok=True)
+    src = os.path.join(out_dir, f"{sample}_{real}.cpp")
+    with open(src, "w") as f:
+        f.write(f"typedef {real} REAL;\n" + SAMPLE_TUS[sample] + "\n")
+    exe = src[:-4]
+    lib_dir = os.path.dirname(mcengine.LIB_PATH)
+    r = subprocess.run([gxx, "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), src, "-o", exe,
+                        f"-L{lib_dir}", "-lfsmc", f"-Wl,-rpath,{lib_dir}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
