@@ -162,6 +162,20 @@ def _gbm_config(np, torch, mc, eng, precisions, steps):
                     "fp64_vs_black_scholes": {"options_with_1000_itm_paths": int(ok.sum()),
                                               "max_abs_z": float(np.max(np.abs(z))),
                                               "mean_z2": float(np.mean(z * z))}})
+    if "fp32" in res:
+        # the same paths with S_T as control variate: time and standard error
+        m = dict(precisions)["fp32"]
+        ws = torch.empty(mc.gbm_european_workspace(n, paths, r, sig, m, control_variate=True), dtype=torch.uint8,
+                         device=dev)
+        o = torch.zeros(n, dtype=torch.float64, device=dev)
+        c = torch.zeros_like(o)
+        ms = _timed(torch, np, lambda: mc.gbm_european_async(tS, tX, tT, r, sig, paths, m, o, c, ws,
+                                                             control_variate=True), steps, 2)
+        ratio = (c.cpu().numpy() / 1.96) / np.maximum(res["fp32"][2], 1e-300)
+        live = res["fp32"][2] > 0
+        out["control_variate_fp32"] = {"ms": ms, "path_steps_per_s": n * paths / (ms * 1e-3),
+                                       "median_se_ratio": float(np.median(ratio[live])),
+                                       "median_se_ratio_s_over_1_2k": float(np.median(ratio[live & (S > 1.2 * X)]))}
     return out
 
 

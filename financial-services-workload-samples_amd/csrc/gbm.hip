@@ -5,6 +5,12 @@
 // optional pathwise Greeks) per workgroup in FP64.  A second tiny kernel
 // combines the per-workgroup partials in a fixed order and applies the
 // estimator of MonteCarloInsideBlockingDP.cpp:266-269.
+//
+// Optional control variate (not in the sample): u = S_T - S e^{rT} has known
+// mean 0 under the pricing measure, so with y = (S_T - K)^+ the estimator
+// e^{-rT} (mean(y) - beta mean(u)), beta = cov(y, u) / var(u) from the same
+// paths, has variance var(y) (1 - corr(y, u)^2) / N.  The kernel adds the
+// sums of u, u^2 and u y over every path.
 #include "fsmc_device.hpp"
 #include "fsmc_host.hpp"
 
@@ -23,12 +29,22 @@ struct GbmArgs {
   double* partials;      // [n_options][gridDim.x][NV]
 };
 
-template <typename Real, bool kGreeks>
+template <typename Real, bool kGreeks, bool kCV>
 struct GbmAcc {
   // Payoff sum and sum of squares; with Greeks also the pathwise delta and
-  // vega kernels (and their squares) and the in-the-money count (rho).
-  CompSum<Real> v0, v1, dl, dl2, vg, vg2;
+  // vega kernels (and their squares) and the in-the-money count (rho); with
+  // the control variate the sums of u, u^2 and u y.
+  CompSum<Real> v0, v1, dl, dl2, vg, vg2, cu, cuu, cuy;
   uint32_t itm = 0;
+  Real F = 0;  // S e^{rT}, the mean of S_T
+  __device__ __forceinline__ void control(Real st, Real c) {
+    if constexpr (kCV) {
+      const Real u = st - F;
+      cu.add(u);
+      cuu.add(u * u);
+      cuy.add(c > Real(0) ? u * c : Real(0));
+    }
+  }
   __device__ __forceinline__ void hit(Real c, Real st, Real dlog) {
     v0.add(c);
     v1.add(c * c);
@@ -45,12 +61,14 @@ struct GbmAcc {
     const Real st = Y * fexp2(mu + vol * z);
     const Real c = st - Z;
     if (c > Real(0)) hit(c, st, sqrtT * z - sigT);
+    control(st, c);
   }
 };
 
-template <typename Real, bool kBuffer, bool kGreeks>
+template <typename Real, bool kBuffer, bool kGreeks, bool kCV>
 __global__ __launch_bounds__(kBlock) void gbm_paths_kernel(GbmArgs a) {
-  constexpr int NV = kGreeks ? 7 : 2;
+  constexpr int NG = kGreeks ? 7 : 2;  // the control-variate sums follow at NG
+  constexpr int NV = NG + (kCV ? 3 : 0);
   __shared__ double lds[NV * (kBlock / 64)];
   const int o = blockIdx.y;
   const double Td = a.T[o];
@@ -61,7 +79,8 @@ __global__ __launch_bounds__(kBlock) void gbm_paths_kernel(GbmArgs a) {
   const Real Z = (Real)a.X[o];
   const Real sqrtT = (Real)sqrtTd;
   const Real sigT = (Real)(a.sigma * Td);
-  GbmAcc<Real, kGreeks> acc;
+  GbmAcc<Real, kGreeks, kCV> acc;
+  acc.F = (Real)(a.S[o] * exp(a.r * Td));
   const int t = blockIdx.x * kBlock + threadIdx.x;
   const int stride = gridDim.x * kBlock;
   if (kBuffer) {
@@ -72,6 +91,7 @@ __global__ __launch_bounds__(kBlock) void gbm_paths_kernel(GbmArgs a) {
       const Real st = Y * fexp2(s);
       const Real c = st - Z;
       if (c > Real(0)) acc.hit(c, st, kGreeks ? sqrtT * ((s - mu) / vol) - sigT : Real(0));
+      acc.control(st, c);
     }
   } else {
     const int groups = (a.n_paths + 3) >> 2;
@@ -98,6 +118,11 @@ __global__ __launch_bounds__(kBlock) void gbm_paths_kernel(GbmArgs a) {
     v[5] = acc.vg2.value();
     v[6] = (double)acc.itm;
   }
+  if constexpr (kCV) {
+    v[NG] = acc.cu.value();
+    v[NG + 1] = acc.cuu.value();
+    v[NG + 2] = acc.cuy.value();
+  }
   block_sum<NV>(v, lds);
   if (threadIdx.x == 0) {
     double* out = a.partials + ((size_t)o * gridDim.x + blockIdx.x) * NV;
@@ -115,21 +140,32 @@ __device__ __forceinline__ void mean_se(double s, double s2, double N, double f,
 }
 
 // One wavefront per option (workgroup of 64): the option's workgroup
-// partials combined by wave_combine (fixed order).
-__global__ __launch_bounds__(64) void gbm_finalize_kernel(int nbx, int nv, int n_paths, double r, const double* S,
-                                                         const double* X, const double* T,
+// partials combined by wave_combine (fixed order).  cv: the last three
+// partials are the control-variate sums.
+__global__ __launch_bounds__(64) void gbm_finalize_kernel(int nbx, int nv, int cv, int n_paths, double r,
+                                                         const double* S, const double* X, const double* T,
                                                          const double* partials, double* result,
                                                          double* confidence, double* greeks) {
   const int o = blockIdx.x;
-  double s[7];
-  wave_combine<7>(partials + (size_t)o * nbx * nv, nbx, nv, s);
+  double s[10];
+  wave_combine<10>(partials + (size_t)o * nbx * nv, nbx, nv, s);
   if (threadIdx.x != 0) return;
   const double N = (double)n_paths;
   // MonteCarloInsideBlockingDP.cpp:266-269.
   const double exprt = exp2(-r * kLog2e * T[o]);
-  result[o] = exprt * s[0] * (1.0 / N);
-  const double stddev = sqrt((N * s[1] - s[0] * s[0]) * (1.0 / (N * (N - 1.0))));
-  confidence[o] = exprt * stddev * (1.96 / sqrt(N));
+  const double vy = (N * s[1] - s[0] * s[0]) * (1.0 / (N * (N - 1.0)));
+  if (cv) {
+    const double su = s[nv - 3], suu = s[nv - 2], suy = s[nv - 1];
+    const double vu = (N * suu - su * su) * (1.0 / (N * (N - 1.0)));
+    const double cov = (N * suy - su * s[0]) * (1.0 / (N * (N - 1.0)));
+    const double beta = vu > 0.0 ? cov / vu : 0.0;
+    const double v = vy - beta * cov;
+    result[o] = exprt * (s[0] - beta * su) * (1.0 / N);
+    confidence[o] = exprt * sqrt(v > 0.0 ? v : 0.0) * (1.96 / sqrt(N));
+  } else {
+    result[o] = exprt * s[0] * (1.0 / N);
+    confidence[o] = exprt * sqrt(vy) * (1.96 / sqrt(N));
+  }
   if (greeks) {
     double* g = greeks + 6 * (size_t)o;
     // delta_p = e^{-rT} 1{S_T>K} S_T / S0
@@ -149,7 +185,7 @@ struct GbmPlan {
 
 GbmPlan gbm_plan(const fsmc_gbm_params* p, const fsmc_mode* m, bool greeks) {
   GbmPlan g;
-  g.nv = greeks ? 7 : 2;
+  g.nv = (greeks ? 7 : 2) + (p->control_variate ? 3 : 0);
   // About 16 paths per thread: enough to amortise the per-thread setup.
   const long long per_block = (long long)kBlock * 16;
   long long nbx = (p->n_paths + per_block - 1) / per_block;
@@ -167,12 +203,22 @@ int gbm_validate(const fsmc_gbm_params* p, const fsmc_mode* m) {
   FSMC_REQUIRE(p->n_options >= 1 && p->n_options <= 65535, "n_options %d outside [1, 65535]", p->n_options);
   FSMC_REQUIRE(p->n_paths >= 2 && p->n_paths <= (1 << 30), "n_paths %d outside [2, 2^30]", p->n_paths);
   FSMC_REQUIRE(p->volatility > 0.0, "volatility must be > 0");
+  FSMC_REQUIRE(p->control_variate == 0 || p->control_variate == 1, "control_variate %d is neither 0 nor 1",
+               p->control_variate);
   return FSMC_OK;
 }
 
-template <typename Real, bool B, bool G>
+template <typename Real, bool B, bool G, bool C>
 void launch(const GbmArgs& a, int n_options, int nbx, hipStream_t s) {
-  hipLaunchKernelGGL((gbm_paths_kernel<Real, B, G>), dim3(nbx, n_options), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((gbm_paths_kernel<Real, B, G, C>), dim3(nbx, n_options), dim3(kBlock), 0, s, a);
+}
+
+template <typename Real, bool B>
+void launch_gc(const GbmArgs& a, bool greeks, bool cv, int n_options, int nbx, hipStream_t s) {
+  if (greeks)
+    cv ? launch<Real, B, true, true>(a, n_options, nbx, s) : launch<Real, B, true, false>(a, n_options, nbx, s);
+  else
+    cv ? launch<Real, B, false, true>(a, n_options, nbx, s) : launch<Real, B, false, false>(a, n_options, nbx, s);
 }
 
 }  // namespace
@@ -214,16 +260,17 @@ int fsmc_gbm_european_async(const fsmc_gbm_params* p, const fsmc_mode* m, const 
   a.w = counter_w(kTagGbm, m->stream);
   a.partials = (double*)d_ws;
   const bool f64 = m->precision == FSMC_FP64;
+  const bool cv = p->control_variate != 0;
   if (f64) {
-    if (buffer) greeks ? launch<double, true, true>(a, p->n_options, g.nbx, s) : launch<double, true, false>(a, p->n_options, g.nbx, s);
-    else greeks ? launch<double, false, true>(a, p->n_options, g.nbx, s) : launch<double, false, false>(a, p->n_options, g.nbx, s);
+    if (buffer) launch_gc<double, true>(a, greeks, cv, p->n_options, g.nbx, s);
+    else launch_gc<double, false>(a, greeks, cv, p->n_options, g.nbx, s);
   } else {
-    if (buffer) greeks ? launch<float, true, true>(a, p->n_options, g.nbx, s) : launch<float, true, false>(a, p->n_options, g.nbx, s);
-    else greeks ? launch<float, false, true>(a, p->n_options, g.nbx, s) : launch<float, false, false>(a, p->n_options, g.nbx, s);
+    if (buffer) launch_gc<float, true>(a, greeks, cv, p->n_options, g.nbx, s);
+    else launch_gc<float, false>(a, greeks, cv, p->n_options, g.nbx, s);
   }
   FSMC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(gbm_finalize_kernel, dim3(p->n_options), dim3(64), 0, s, g.nbx, g.nv, p->n_paths,
-                     p->riskfree, d_S, d_X, d_T, a.partials, d_result, d_conf, d_greeks);
+  hipLaunchKernelGGL(gbm_finalize_kernel, dim3(p->n_options), dim3(64), 0, s, g.nbx, g.nv, cv ? 1 : 0,
+                     p->n_paths, p->riskfree, d_S, d_X, d_T, a.partials, d_result, d_conf, d_greeks);
   FSMC_HIP(hipGetLastError());
   return FSMC_OK;
 }

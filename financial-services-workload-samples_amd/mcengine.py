@@ -47,7 +47,7 @@ class Mode(ctypes.Structure):
 
 class GbmParams(ctypes.Structure):
     _fields_ = [("n_options", ctypes.c_int), ("n_paths", ctypes.c_int), ("riskfree", ctypes.c_double),
-                ("volatility", ctypes.c_double)]
+                ("volatility", ctypes.c_double), ("control_variate", ctypes.c_int)]
 
 
 class LiborParams(ctypes.Structure):
@@ -201,9 +201,11 @@ class Engine:
         self.close()
 
     # -- GBM European --------------------------------------------------------
-    def gbm_european(self, S, X, T, r, sigma, n_paths, m, samples=None, greeks=False):
+    def gbm_european(self, S, X, T, r, sigma, n_paths, m, samples=None, greeks=False, control_variate=False):
+        """Returns (price[n], confidence[n]) or (price, confidence, greeks[n][6]);
+        control_variate prices with S_T as control variate."""
         n = len(S)
-        p = GbmParams(n, n_paths, r, sigma)
+        p = GbmParams(n, n_paths, r, sigma, 1 if control_variate else 0)
         S, pS = _d(S)
         X, pX = _d(X)
         T, pT = _d(T)
@@ -495,14 +497,14 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def gbm_european_workspace(n_options, n_paths, r, sigma, m):
-    p = GbmParams(n_options, n_paths, r, sigma)
+def gbm_european_workspace(n_options, n_paths, r, sigma, m, control_variate=False):
+    p = GbmParams(n_options, n_paths, r, sigma, 1 if control_variate else 0)
     return lib().fsmc_gbm_european_workspace(ctypes.byref(p), ctypes.byref(m))
 
 
 def gbm_european_async(S, X, T, r, sigma, n_paths, m, out_res, out_conf, workspace, samples=None,
-                       out_greeks=None):
-    p = GbmParams(S.numel(), n_paths, r, sigma)
+                       out_greeks=None, control_variate=False):
+    p = GbmParams(S.numel(), n_paths, r, sigma, 1 if control_variate else 0)
     _check(lib().fsmc_gbm_european_async(ctypes.byref(p), ctypes.byref(m), _ptr(S), _ptr(X), _ptr(T),
                                          _ptr(samples), _ptr(out_res), _ptr(out_conf), _ptr(out_greeks),
                                          _ptr(workspace), workspace.numel() * workspace.element_size(),

@@ -88,12 +88,18 @@ const char* fsmc_last_error(void);
  * :266-269).  Buffer mode consumes the reference's float samples
  * s = MuByT + VBySqrtT*z (log2 domain, vsRngGaussian output, :249) laid out
  * [n_options][n_paths].  greeks (optional, [n_options][6]) receives the
- * pathwise delta, vega, rho followed by their standard errors. */
+ * pathwise delta, vega, rho followed by their standard errors.
+ * control_variate = 1 (no counterpart in the sample, which uses 0) prices
+ * with S_T as control variate: e^{-rT}(mean(y) - beta mean(S_T - S e^{rT})),
+ * beta = cov(y, S_T) / var(S_T) from the same paths, y = (S_T - K)^+; the
+ * confidence is then 1.96 sqrt(var(y) - beta cov(y, S_T)) / sqrt(N) e^{-rT}.
+ * The Greeks are unchanged by it. */
 typedef struct fsmc_gbm_params {
   int n_options;
   int n_paths; /* per option */
   double riskfree;
   double volatility;
+  int control_variate; /* 0: the sample's estimator; 1: S_T as control variate */
 } fsmc_gbm_params;
 
 int fsmc_gbm_european(fsmc_ctx* ctx, const fsmc_gbm_params* p, const fsmc_mode* m, const double* S,

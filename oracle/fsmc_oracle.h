@@ -43,6 +43,12 @@ void orc_gbm_buffer(int n_opt, int n_paths, double r, double sigma, const double
 void orc_gbm_philox(int n_opt, int n_paths, double r, double sigma, const double* S, const double* X,
                     const double* T, uint64_t seed, uint32_t stream, double* res, double* conf,
                     double* greeks6);
+/* The same paths priced with S_T as control variate (an engine option, no
+ * counterpart in the samples). */
+void orc_gbm_buffer_cv(int n_opt, int n_paths, double r, const double* S, const double* X, const double* T,
+                       const float* samples, double* res, double* conf);
+void orc_gbm_philox_cv(int n_opt, int n_paths, double r, double sigma, const double* S, const double* X,
+                       const double* T, uint64_t seed, uint32_t stream, double* res, double* conf);
 
 /* ---- Heston Asian --------------------------------------------------------- */
 #define ORC_Q_VAR_AS_VOL 1u

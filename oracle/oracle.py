@@ -47,6 +47,9 @@ def lib():
                                      _D, _D, _D, _F, _D, _D]
         L.orc_gbm_philox.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                      _D, _D, _D, ctypes.c_uint64, ctypes.c_uint32, _D, _D, _D]
+        L.orc_gbm_buffer_cv.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D, _D, _F, _D, _D]
+        L.orc_gbm_philox_cv.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        _D, _D, _D, ctypes.c_uint64, ctypes.c_uint32, _D, _D]
         L.orc_heston_buffer.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 2 + [_D] * 7 +
                                         [_D, _D, ctypes.c_uint32, _D, _D])
         L.orc_heston_philox.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 2 + [_D] * 7 +
@@ -158,6 +161,32 @@ def gbm_philox(S, X, T, r, sigma, n_paths, seed, stream=0, greeks=False):
     lib().orc_gbm_philox(n_opt, n_paths, r, sigma, pS, pX, pT, seed, stream, pres, pconf,
                          pg if greeks else None)
     return (res, conf, g) if greeks else (res, conf)
+
+
+def gbm_buffer_cv(S, X, T, r, samples):
+    """The buffer-mode paths priced with S_T as control variate."""
+    n_opt = len(S)
+    n_paths = samples.shape[-1]
+    S, pS = _d(S)
+    X, pX = _d(X)
+    T, pT = _d(T)
+    sm, psm = _f(samples.reshape(n_opt, n_paths))
+    res, pres = _d(np.zeros(n_opt))
+    conf, pconf = _d(np.zeros(n_opt))
+    lib().orc_gbm_buffer_cv(n_opt, n_paths, r, pS, pX, pT, psm, pres, pconf)
+    return res, conf
+
+
+def gbm_philox_cv(S, X, T, r, sigma, n_paths, seed, stream=0):
+    """The Philox-mode paths priced with S_T as control variate."""
+    n_opt = len(S)
+    S, pS = _d(S)
+    X, pX = _d(X)
+    T, pT = _d(T)
+    res, pres = _d(np.zeros(n_opt))
+    conf, pconf = _d(np.zeros(n_opt))
+    lib().orc_gbm_philox_cv(n_opt, n_paths, r, sigma, pS, pX, pT, seed, stream, pres, pconf)
+    return res, conf
 
 
 # ---- Heston ------------------------------------------------------------------

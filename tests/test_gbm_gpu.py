@@ -156,3 +156,70 @@ def test_errors(engine):
         engine.gbm_european(S, X, T, .02, .3, 1, mc.mode())
     with pytest.raises(mc.FsmcError):  # buffer mode without samples
         engine.gbm_european(S, X, T, .02, .3, 1000, mc.mode(rng=mc.RNG_BUFFER))
+
+
+# ---- control variate (S_T; an engine option, no counterpart in the sample) ----
+def test_control_variate_fp64_matches_oracle(engine):
+    S, X, T = _opts(16, 7)
+    r, sig, n_paths = 0.02, 0.3, 10000  # not a multiple of 4: a partial Philox group
+    res, conf = engine.gbm_european(S, X, T, r, sig, n_paths, mc.mode(mc.FP64, seed=8), control_variate=True)
+    ores, oconf = orc.gbm_philox_cv(S, X, T, r, sig, n_paths, seed=8)
+    _, pconf = orc.gbm_philox(S, X, T, r, sig, n_paths, seed=8)
+    nz = ores > 0
+    assert _rel(res[nz], ores[nz]) <= 1e-12 and np.all(res[~nz] == 0)
+    # var(y) - beta cov cancels deep in the money, and the confidence is its
+    # square root: a summation-order difference of 1e-16 var(y) becomes
+    # 1e-8 of the plain estimator's confidence
+    assert np.all(np.abs(conf - oconf) <= 2e-7 * pconf + 1e-13 * S)
+    sm = _samples(S, X, T, r, sig, 4096, 9)
+    res, conf = engine.gbm_european(S, X, T, r, sig, 4096, mc.mode(mc.FP64, mc.RNG_BUFFER), samples=sm,
+                                    control_variate=True)
+    ores, oconf = orc.gbm_buffer_cv(S, X, T, r, sm)
+    _, pconf = orc.gbm_buffer(S, X, T, r, sig, sm)
+    nz = ores > 0
+    assert _rel(res[nz], ores[nz]) <= 1e-12 and np.all(res[~nz] == 0)
+    assert np.all(np.abs(conf - oconf) <= 2e-7 * pconf + 1e-13 * S)
+
+
+def test_control_variate_small_strike_exact_and_greeks_unchanged(engine):
+    S, X, T = np.array([20.0, 7.5, 12.0]), np.array([1e-3, 2e-3, 5e-3]), np.array([1.0, 3.0, .5])
+    r, sig = 0.02, 0.3
+    exact = S - X * np.exp(-r * T)
+    res, conf, g = engine.gbm_european(S, X, T, r, sig, 65536, mc.mode(mc.FP64, seed=2), greeks=True,
+                                       control_variate=True)
+    assert np.allclose(res, exact, rtol=1e-12, atol=0) and np.all(conf < 1e-9 * S)
+    _, _, g0 = engine.gbm_european(S, X, T, r, sig, 65536, mc.mode(mc.FP64, seed=2), greeks=True)
+    assert np.array_equal(g, g0)
+    res32, conf32 = engine.gbm_european(S, X, T, r, sig, 65536, mc.mode(mc.FP32, seed=2), control_variate=True)
+    assert np.allclose(res32, exact, rtol=1e-5, atol=0)
+
+
+def test_control_variate_fp32_and_variance_reduction(engine):
+    # the 1024 x 262144 workload: FP32 CV vs FP64 CV on the same stream, CV vs
+    # Black-Scholes, and the standard error against the plain estimator's
+    rng = np.random.default_rng(11)
+    n = 1024
+    S, X, T = rng.uniform(5, 30, n), rng.uniform(1, 100, n), rng.uniform(.25, 10, n)
+    r, sig, n_paths = 0.02, 0.3, 262144
+    c64, f64 = engine.gbm_european(S, X, T, r, sig, n_paths, mc.mode(mc.FP64, seed=4), control_variate=True)
+    c32, f32 = engine.gbm_european(S, X, T, r, sig, n_paths, mc.mode(mc.FP32, seed=4), control_variate=True)
+    p64, g64 = engine.gbm_european(S, X, T, r, sig, n_paths, mc.mode(mc.FP64, seed=4))
+    se, pse = f64 / 1.96, g64 / 1.96
+    bs = np.array([orc.bs_call(*a, r, sig) for a in zip(S, X, T)])
+    # the variance of the CV estimator comes from the paths on both sides of
+    # the strike; its standard error is a fair yardstick only with enough of
+    # each (expected counts from P(S_T > K) = N(d2))
+    d2 = (np.log(S / X) + (r - .5 * sig * sig) * T) / (sig * np.sqrt(T))
+    p_itm = np.array([.5 * math.erfc(-x / math.sqrt(2)) for x in d2])
+    both = (n_paths * p_itm >= 1000) & (n_paths * (1 - p_itm) >= 1000)
+    z = (c64[both] - bs[both]) / se[both]
+    assert both.sum() > 300 and np.max(np.abs(z)) < 5.0 and abs(np.mean(z * z) - 1.0) < 0.2
+    # far from the money one side may see no path at all (either estimator
+    # then misses the call value, or the put value deep in the money)
+    put = bs - (S - X * np.exp(-r * T))
+    slack = np.where(p_itm < .5, bs, np.maximum(put, 0.0))
+    assert np.all(np.abs(c64[~both] - bs[~both]) <= 5 * pse[~both] + slack[~both] + 1e-9 * S[~both])
+    assert np.all(np.abs(c32 - c64) <= 0.05 * se + 2e-6 * S)
+    assert np.all(f64 <= g64 * (1 + 1e-9))
+    itm = S > 1.2 * X
+    assert np.median(f64[itm] / g64[itm]) < 0.3

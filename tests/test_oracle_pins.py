@@ -167,6 +167,48 @@ def test_gbm_buffer_oracle_matches_philox_oracle():
     assert np.allclose(cb, cp, rtol=2e-5)
 
 
+def test_gbm_control_variate_small_strike_is_exact():
+    # K far below every S_T: y = (S_T - K) = u + S e^{rT} - K on every path,
+    # so beta = 1 and the estimator is S - K e^{-rT} with zero variance.  A
+    # wrong sign of beta, an uncentred control or a dropped discount fails.
+    S, X, T = np.array([20.0, 7.5]), np.array([1e-3, 2e-3]), np.array([1.0, 3.0])
+    r = .02
+    res, conf = orc.gbm_philox_cv(S, X, T, r, .3, 8192, seed=3)
+    assert np.allclose(res, S - X * np.exp(-r * T), rtol=1e-12, atol=0)
+    assert np.all(conf < 1e-9 * S)
+    # the buffer variant on float samples
+    rng = np.random.default_rng(4)
+    L2E = 1.4426950408889634074
+    z = rng.standard_normal((2, 4096))
+    samples = ((L2E * (r - .045) * T)[:, None] + (L2E * .3 * np.sqrt(T))[:, None] * z).astype(np.float32)
+    rb, cb = orc.gbm_buffer_cv(S, X, T, r, samples)
+    # the float samples' mean of S_T is not exactly S e^{rT}: the estimator
+    # is S - K e^{-rT} plus the sample mean's deviation times (1 - beta) = 0
+    assert np.allclose(rb, S - X * np.exp(-r * T), rtol=1e-12, atol=0)
+
+
+def test_gbm_control_variate_vs_black_scholes():
+    rng = np.random.default_rng(9)
+    n = 24
+    S = rng.uniform(5, 30, n)
+    X = rng.uniform(1, 40, n)
+    T = rng.uniform(.25, 5, n)
+    res, conf = orc.gbm_philox_cv(S, X, T, .02, .3, 1 << 15, seed=21)
+    pres, pconf = orc.gbm_philox(S, X, T, .02, .3, 1 << 15, seed=21)
+    se = conf / 1.96
+    for i in range(n):
+        bs = orc.bs_call(S[i], X[i], T[i], .02, .3)
+        # deep in the money every path may finish above K: the estimator is
+        # then exactly S - K e^{-rT} (zero variance) and misses the put value
+        # bs - (S - K e^{-rT}) that no path saw
+        put = bs - (S[i] - X[i] * np.exp(-.02 * T[i]))
+        assert abs(res[i] - bs) < 4 * se[i] + 1e-8 + (put if put < 1e-4 else 0.0)
+    # never worse than the plain estimator on the same paths, much better in the money
+    assert np.all(conf <= pconf * (1 + 1e-12))
+    itm = S > 1.5 * X
+    assert itm.any() and np.all(conf[itm] < 0.5 * pconf[itm])
+
+
 # ---- Heston oracle -------------------------------------------------------------------
 HP = dict(S=[100.0], X=[100.0], T=[1.0], v0=[0.09], kappa=[2.0], theta=[0.04], xi=[1.0])
 
