@@ -94,6 +94,15 @@ def _cpu_baseline(np, P, target_s):
                       seed=5)
     el = time.perf_counter() - t0
     steps = N_OPT * paths * N_STEPS
+    # the same oracle on one thread (the per-core rate an MPTest-style run of
+    # one process per core multiplies), on 1/threads of the paths
+    p1 = max(64, paths // max(threads, 1) // 64 * 64)
+    prev = orc.lib().orc_set_threads(1)
+    t0 = time.perf_counter()
+    orc.heston_philox(P["S"], P["X"], P["T"], P["v0"], P["kappa"], P["theta"], P["xi"], R, RHO, p1, N_STEPS,
+                      seed=5)
+    el1 = time.perf_counter() - t0
+    orc.lib().orc_set_threads(prev)
     model = None
     try:
         for line in open("/proc/cpuinfo"):
@@ -106,7 +115,9 @@ def _cpu_baseline(np, P, target_s):
             "host": {"cpu": model, "logical_cpus": os.cpu_count(),
                      "usable_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None},
             "sample": f"FP64 oracle (C, OpenMP over options), {N_OPT} options x {paths} paths x {N_STEPS} steps "
-                      f"of the same workload, {el:.1f} s"}
+                      f"of the same workload, {el:.1f} s",
+            "single_thread": {"value": N_OPT * p1 * N_STEPS / el1, "unit": "path-steps/s", "cores": 1,
+                              "sample": f"{N_OPT} options x {p1} paths x {N_STEPS} steps, {el1:.1f} s"}}
 
 
 def _timed(torch, np, fn, steps, warmup):

@@ -19,6 +19,9 @@
  */
 #include "fsmc_oracle.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -982,4 +985,17 @@ void orc_mt19937_gaussian(uint32_t seed, uint64_t offset, int64_t n, double mean
     z[j + 1] = mean + stddev * (r * cos(2.0 * ORC_PI * u2));
   }
   free(bits);
+}
+
+/* Threads of the OpenMP loops over options (bench.py's CPU baseline times
+ * all host threads and one thread); returns the previous setting. */
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  const int prev = omp_get_max_threads();
+  if (n > 0) omp_set_num_threads(n);
+  return prev;
+#else
+  (void)n;
+  return 1;
+#endif
 }

@@ -103,6 +103,9 @@ void orc_mrg32k3a_gaussian(uint32_t seed, uint64_t offset, int64_t n, double mea
 void orc_mt19937_bits(uint32_t seed, uint64_t offset, int64_t n, uint32_t* out);
 void orc_mt19937_gaussian(uint32_t seed, uint64_t offset, int64_t n, double mean, double stddev, double* z);
 
+/* Threads of the OpenMP loops over options; returns the previous setting. */
+int orc_set_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif
