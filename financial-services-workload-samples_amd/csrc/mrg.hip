@@ -13,7 +13,7 @@
 //   z_n = mean + stddev * icdf(u_n)                    (Wichura's AS241, FP64)
 //
 // The uniforms equal MKL's bit for bit; MKL's own ICDF is not reproducible
-// exactly (it differs from the exact inverse by up to ~2.4e-9 relative), so
+// exactly (it differs from the exact inverse by up to ~7.3e-9 relative), so
 // the normals agree with MKL's to that and with the exact inverse to ~1e-16.
 #include "fsmc_device.hpp"
 #include "fsmc_host.hpp"

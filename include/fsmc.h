@@ -181,7 +181,7 @@ int fsmc_libor_swaption_greeks_async(const fsmc_libor_params* p, const fsmc_mode
  * Uniforms equal MKL's bit for bit: ((x_k - y_k) mod m1) * (1 / m1).  Normals
  * are mean + stddev * icdf(u_k) with an FP64 inverse normal CDF accurate to
  * ~1e-16 relative; MKL's own ICDF differs from the exact inverse by up to
- * ~2.4e-9 relative, so the normals match MKL's to that. */
+ * ~7.3e-9 relative, so the normals match MKL's to that. */
 int fsmc_mrg32k3a_uniform(fsmc_ctx* ctx, uint32_t seed, uint64_t offset, long long n, double* u);
 int fsmc_mrg32k3a_gaussian(fsmc_ctx* ctx, uint32_t seed, uint64_t offset, long long n, double mean,
                            double stddev, double* z);

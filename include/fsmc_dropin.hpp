@@ -11,7 +11,7 @@
 // LiborMarketModel/mklRng.h:9 -- z[k] = mean + stddev * icdf(u_k), k < num,
 // with u_k the MRG32k3a stream of vslNewStream(VSL_BRNG_MRG32K3A, seed)
 // (mklRng.cpp:44-139), generated on the GPU: uniforms as MKL's bit for bit,
-// normals within ~2.4e-9 relative of MKL's ICDF (fsmc_mrg32k3a_gaussian).
+// normals within ~7.3e-9 relative of MKL's ICDF (fsmc_mrg32k3a_gaussian).
 void generateRand(double* z, int num, double mean, double stddev, int seed = 12345);
 void generateRand(float* z, int num, float mean, float stddev, int seed = 12345);
 

@@ -308,3 +308,80 @@ def mt19937_gaussian(seed, offset, n, mean=0.0, stddev=1.0):
     z = np.zeros(n)
     lib().orc_mt19937_gaussian(seed, offset, n, mean, stddev, z.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return z
+
+
+# ---- SFMT19937 (the European sample's stream) ----------------------------------
+# vslNewStream(VSL_BRNG_SFMT19937, seed) (MonteCarloInsideBlockingDP.cpp:229)
+# is Saito and Matsumoto's SIMD-oriented Mersenne Twister (SFMT19937: 156
+# 128-bit words, recursion w[i] = w[i] ^ (w[i] <<8 1 byte) ^ ((w[i+122] >> 11)
+# & MSK) ^ (w[i-2] >>128 1 byte) ^ (w[i-1] << 18)) seeded by init_by_array
+# ({seed}); vsRngGaussian's ICDF method maps each 32-bit output b, read as a
+# signed integer, to u = int32(b) 2^-32 + 1/2.  Pure Python (small n only).
+_SFMT_MSK = (0xdfffffef, 0xddfecb7f, 0xbffaffff, 0xbffffff6)
+_SFMT_PARITY = (0x00000001, 0x00000000, 0x00000000, 0x13c9e684)
+_M32 = 0xffffffff
+
+
+def _sfmt_words(seed):
+    """init_by_array({seed}) of the 624-word state, then the period check."""
+    n, lag = 624, 11
+    mid = (n - lag) // 2
+    s = [0x8b8b8b8b] * n
+    mix1 = lambda x: ((x ^ (x >> 27)) * 1664525) & _M32  # noqa: E731
+    mix2 = lambda x: ((x ^ (x >> 27)) * 1566083941) & _M32  # noqa: E731
+    key = [seed & _M32]
+    i = 0
+    for k in range(n):  # first pass: the key, then the index
+        r = mix1(s[i] ^ s[(i + mid) % n] ^ s[(i + n - 1) % n])
+        s[(i + mid) % n] = (s[(i + mid) % n] + r) & _M32
+        r = (r + (len(key) if k == 0 else 0) + (key[k - 1] if 1 <= k <= len(key) else 0) + i) & _M32
+        s[(i + mid + lag) % n] = (s[(i + mid + lag) % n] + r) & _M32
+        s[i] = r
+        i = (i + 1) % n
+    for _ in range(n):  # second pass
+        r = mix2((s[i] + s[(i + mid) % n] + s[(i + n - 1) % n]) & _M32)
+        s[(i + mid) % n] ^= r
+        r = (r - i) & _M32
+        s[(i + mid + lag) % n] ^= r
+        s[i] = r
+        i = (i + 1) % n
+    inner = 0
+    for k in range(4):
+        inner ^= s[k] & _SFMT_PARITY[k]
+    for sh in (16, 8, 4, 2, 1):
+        inner ^= inner >> sh
+    if not inner & 1:  # fix the first parity bit found
+        for k in range(4):
+            bit = next((1 << b for b in range(32) if _SFMT_PARITY[k] >> b & 1), 0)
+            if bit:
+                s[k] ^= bit
+                break
+    return s
+
+
+def sfmt19937_bits(seed, n):
+    """The first n 32-bit outputs of the SFMT19937 stream of `seed`."""
+    w = [sum(v << (32 * j) for j, v in enumerate(_sfmt_words(seed)[4 * q:4 * q + 4])) for q in range(156)]
+    full = (1 << 128) - 1
+    msk = sum(m << (32 * j) for j, m in enumerate(_SFMT_MSK))
+    out = []
+    while len(out) < n:
+        r1, r2 = w[154], w[155]
+        for q in range(156):
+            a, b = w[q], w[(q + 122) % 156]
+            b_sr = 0
+            for j in range(4):  # 32-bit lanes shifted right by 11 within each lane
+                b_sr |= (((b >> (32 * j)) & _M32) >> 11) << (32 * j)
+            d_sl = 0
+            for j in range(4):  # 32-bit lanes shifted left by 18 within each lane
+                d_sl |= ((((r2 >> (32 * j)) & _M32) << 18) & _M32) << (32 * j)
+            w[q] = a ^ ((a << 8) & full) ^ (b_sr & msk) ^ (r1 >> 8) ^ d_sl
+            r1, r2 = r2, w[q]
+        for v in w:
+            out.extend((v >> (32 * j)) & _M32 for j in range(4))
+    return np.array(out[:n], dtype=np.uint32)
+
+
+def sfmt19937_icdf_uniforms(seed, n):
+    """The uniforms vsRngGaussian's ICDF method draws from that stream."""
+    return sfmt19937_bits(seed, n).view(np.int32).astype(np.float64) * 2.0 ** -32 + 0.5

@@ -104,3 +104,22 @@ def test_asian_command_line_prices(build, tol):
                                     orc.mt19937_gaussian(42, k, per), orc.mt19937_gaussian(43, k, per),
                                     quirks=orc.Q_VAR_AS_VOL | orc.Q_RUNNING_VAL2)
         assert prices[o] == pytest.approx(ores[0], rel=tol, abs=tol)
+
+
+def test_streams_against_mkl_fixture(engine):
+    # MKL 2021.4's own outputs (tests/golden/mkl_streams.npz): MT19937 bits
+    # bitwise and BoxMuller2 normals to MKL's arithmetic; MRG32k3a uniforms
+    # bitwise and ICDF normals to MKL's inverse-CDF accuracy
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mkl_streams.npz"))
+    for seed in (42, 43):
+        g = engine.mt19937(seed)
+        assert np.array_equal(g.bits(2048), d[f"mt{seed}_bits"])
+        g.close()
+        g = engine.mt19937(seed)
+        assert np.max(np.abs(g.gaussian(2048) - d[f"mt{seed}_boxmuller2"])) < 3e-8
+        g.close()
+    assert np.array_equal(engine.mrg32k3a_uniform(12345, 0, 2048), d["mrg12345_uniform"])
+    f = d["mrg12345_icdf"]
+    assert np.max(np.abs(engine.mrg32k3a_gaussian(12345, 0, 2048) - f) / np.abs(f)) < 8e-9
+    f = d["mrg12345_skip1000003_icdf"]
+    assert np.max(np.abs(engine.mrg32k3a_gaussian(12345, 1000003, 256) - f) / np.abs(f)) < 8e-9

@@ -7,6 +7,7 @@ closed-form quadrature -- each chosen so that a dropped term, a wrong sign or
 a swapped operand in the oracle fails at least one check.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -467,7 +468,7 @@ def test_mrg32k3a_reproduces_mkl_uniforms_and_normals():
     u = orc.mrg32k3a_uniform(12345, 0, 3)
     assert list(u) == MKL_MRG_U  # bit for bit
     z = orc.mrg32k3a_gaussian(12345, 0, 6)
-    # MKL's ICDF itself is off the exact inverse by up to ~2.4e-9 relative
+    # MKL's ICDF itself is off the exact inverse by up to ~7.3e-9 relative
     assert np.max(np.abs(z - MKL_MRG_Z) / np.abs(MKL_MRG_Z)) < 3e-9
     # vslSkipAheadStream(4) continues with normals 4 and 5
     assert np.array_equal(orc.mrg32k3a_gaussian(12345, 4, 2), z[4:6])
@@ -516,3 +517,38 @@ def test_lsmc_oracle_bridge_paths_have_gbm_law():
                            * np.exp(-2 * r[i] * T[i]) / 200000) for i in range(3)])
     assert np.all(np.abs(eu - bs_put) < 4 * sd)
     assert np.all(pr >= eu - 1e-12)
+
+
+# ---- MKL stream fixtures (tests/golden/mkl_streams.npz, written by
+# tests/golden/make_mkl_fixtures.py from Intel oneAPI MKL 2021.4) -------------
+def _mkl():
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mkl_streams.npz")
+    return np.load(path)
+
+
+def test_mrg32k3a_against_mkl_fixture():
+    d = _mkl()
+    assert np.array_equal(orc.mrg32k3a_uniform(12345, 0, 2048), d["mrg12345_uniform"])
+    f = d["mrg12345_icdf"]
+    assert np.max(np.abs(orc.mrg32k3a_gaussian(12345, 0, 2048) - f) / np.abs(f)) < 8e-9
+    f = d["mrg12345_skip1000003_icdf"]
+    assert np.max(np.abs(orc.mrg32k3a_gaussian(12345, 1000003, 256) - f) / np.abs(f)) < 8e-9
+
+
+def test_mt19937_against_mkl_fixture():
+    # 2048 outputs: three twists of the 624-word state
+    d = _mkl()
+    for seed in (42, 43):
+        assert np.array_equal(orc.mt19937_bits(seed, 0, 2048).astype(np.uint32), d[f"mt{seed}_bits"])
+        assert np.max(np.abs(orc.mt19937_gaussian(seed, 0, 2048) - d[f"mt{seed}_boxmuller2"])) < 3e-8
+
+
+def test_sfmt19937_against_mkl_fixture():
+    # MKL's SFMT19937 is init_by_array({seed}) SFMT; its ICDF reads each
+    # output as a signed integer (u = int32 2^-32 + 1/2): read unsigned, the
+    # normals would be off by O(1) for half the draws
+    from scipy.special import ndtri
+    d = _mkl()
+    assert np.array_equal(orc.sfmt19937_bits(777, 2048), d["sfmt777_bits"])
+    z = ndtri(orc.sfmt19937_icdf_uniforms(777, 2048))
+    assert np.max(np.abs(z - d["sfmt777_icdf_float"])) < 2e-5
