@@ -9,9 +9,10 @@
 // <nthreads> takes noptions / nthreads options from glibc rand_r seeded with
 // 777 + t, S ~ U[5, 50], X ~ U[10, 25], T ~ U[1, 5] (:45-48, :208-226), with
 // r = 0.06 and sigma = 0.10 (:71-72).  All of them are priced in one call of
-// the drop-in price_european with path_length paths each (Philox normals of
-// seed 777; the block length is validated as in the sample but the GPU does
-// not block).  As in the sample the batch is priced 4 times and the last pass
+// the drop-in price_european with path_length paths each, on the sample's own
+// normals: thread t's options draw in order from the SFMT19937 stream of seed
+// 777 + t (:229-249), generated on the GPU (the block length is validated as
+// in the sample; with inverse-CDF normals it does not change the stream).  As in the sample the batch is priced 4 times and the last pass
 // is timed.  With verbose != 0 the Black-Scholes comparison of :275-316 is
 // printed.  -DUSE_DOUBLE selects REAL = double and the FP64 engine path,
 // otherwise REAL = float and FP32.
@@ -110,6 +111,8 @@ int main(int argc, char** argv) {
       T[k] = (tfloat)uniform(1.0, 5.0, &seed);
     }
   }
+  // each sample thread's options draw from its own SFMT19937 stream (777 + thread)
+  fsmc_dropin::european_config().threads = (int)nthreads;
   double elapsed = 0.0;
   for (int it = 0; it < kIters; ++it) {
     const auto t0 = std::chrono::steady_clock::now();

@@ -140,6 +140,10 @@ def lib():
         L.fsmc_mt19937_gaussian_async.argtypes = [_VP, ctypes.c_longlong, ctypes.c_double, ctypes.c_double, _VP,
                                                   _VP]
         L.fsmc_mt19937_destroy.argtypes = [_VP]
+        L.fsmc_sfmt19937_gaussian.argtypes = [_VP, ctypes.c_uint32, ctypes.c_longlong, ctypes.c_double,
+                                              ctypes.c_double, _F]
+        L.fsmc_sfmt19937_gaussian_async.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_longlong,
+                                                    ctypes.c_longlong, _VP, _VP, _VP, _VP]
         _I = ctypes.POINTER(ctypes.c_int)
         L.fsmc_lsmc_plan_create.argtypes = [_VP, ctypes.c_int, PM] + [_D] * 5 + [_I, _I, ctypes.POINTER(_VP)]
         L.fsmc_lsmc_plan_workspace.restype = ctypes.c_size_t
@@ -331,6 +335,14 @@ class Engine:
                "fsmc_mrg32k3a_gaussian")
         return z
 
+    def sfmt19937_gaussian(self, seed, n, mean=0.0, stddev=1.0):
+        """The European sample's stream: SFMT19937 of `seed` (MKL seeding),
+        inverse-CDF normals as float32."""
+        z = np.zeros(n, dtype=np.float32)
+        _check(lib().fsmc_sfmt19937_gaussian(self._ctx, seed, n, mean, stddev, z.ctypes.data_as(_F)),
+               "fsmc_sfmt19937_gaussian")
+        return z
+
     def mt19937(self, seed):
         """A sequential MKL-compatible MT19937 stream (each call continues it)."""
         return MT19937(self, seed)
@@ -500,6 +512,13 @@ def _stream():
 def gbm_european_workspace(n_options, n_paths, r, sigma, m, control_variate=False):
     p = GbmParams(n_options, n_paths, r, sigma, 1 if control_variate else 0)
     return lib().fsmc_gbm_european_workspace(ctypes.byref(p), ctypes.byref(m))
+
+
+def sfmt19937_gaussian_async(seed0, n_streams, n_per_stream, out, seg=1, mean=None, stddev=None):
+    """Streams seed0 .. seed0 + n_streams - 1 into the float32 tensor out
+    ([n_streams * n_per_stream]); mean / stddev: float64 tensors per segment."""
+    _check(lib().fsmc_sfmt19937_gaussian_async(seed0, n_streams, n_per_stream, seg, _ptr(mean), _ptr(stddev),
+                                               _ptr(out), _stream()), "fsmc_sfmt19937_gaussian_async")
 
 
 def gbm_european_async(S, X, T, r, sigma, n_paths, m, out_res, out_conf, workspace, samples=None,

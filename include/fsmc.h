@@ -210,6 +210,23 @@ int fsmc_mt19937_gaussian_async(fsmc_mt19937* g, long long n, double mean, doubl
                                 void* stream);
 int fsmc_mt19937_destroy(fsmc_mt19937* g);
 
+/* The European sample's streams (MonteCarloInsideBlockingDP.cpp:229,249:
+ * vslNewStream(VSL_BRNG_SFMT19937, 777 + thread) and vsRngGaussian with
+ * VSL_RNG_METHOD_GAUSSIAN_ICDF): SFMT19937 seeded by init_by_array({seed})
+ * as MKL does, u = int32(output) 2^-32 + 1/2, z = AS241 inverse CDF (FP64),
+ * written as float.  n_streams streams seeded seed0, seed0 + 1, ...; stream
+ * t writes its first n_per_stream values to d_out[t * n_per_stream + i].
+ * With d_mean / d_stddev (device, one pair per segment of seg values of the
+ * whole output) value g is d_mean[g / seg] + d_stddev[g / seg] z: the
+ * sample's per-option MuByT + VBySqrtT z with seg = paths per option.
+ * Raw outputs equal MKL's bit for bit; MKL's float ICDF is within 2e-5 of
+ * the exact inverse used here.  Each stream is sequential (one workgroup,
+ * seeded in place); nothing is allocated. */
+int fsmc_sfmt19937_gaussian_async(uint32_t seed0, int n_streams, long long n_per_stream, long long seg,
+                                  const double* d_mean, const double* d_stddev, float* d_out, void* stream);
+/* One stream's first n values, mean + stddev z, to host memory. */
+int fsmc_sfmt19937_gaussian(fsmc_ctx* ctx, uint32_t seed, long long n, double mean, double stddev, float* out);
+
 /* ---- Longstaff-Schwartz American puts -------------------------------------
  * Batched LSMC for many options with uneven step and path counts: one
  * exercise date per time step, cash flows discounted at r, regression of the

@@ -55,6 +55,22 @@ void MonteCarloTimeStep(double* h_CallResult, double* h_CallConfidence, double* 
 void MonteCarloTimeStep(float* h_CallResult, float* h_CallConfidence, float* S, float* X, float* T,
                         float* initialVol, float* kappaVals, float* thetaVals, float* xiVals);
 
+namespace fsmc_dropin {
+
+// MonteCarloEuropeanOptions/MonteCarloInsideBlockingDP.cpp:141-148, 229 --
+// the normals price_european draws.  sample_stream: the sample's own
+// streams, SFMT19937 seeded seed + t for sample thread t of `threads`, which
+// prices the t-th contiguous block of nOpt / threads options drawing each
+// option's paths in order (MKL's seeding and inverse-CDF mapping, generated
+// on the GPU); otherwise the engine's Philox stream of `seed`.
+struct EuropeanConfig {
+  bool sample_stream = true;
+  int threads = 1;  // NTHREADS of the sample's command line; must divide nOpt
+};
+EuropeanConfig& european_config();
+
+}  // namespace fsmc_dropin
+
 // MonteCarloEuropeanOptions: the sample prices inside main
 // (MonteCarloInsideBlockingDP.cpp:229-270), so this entry point is new: it
 // prices nOpt GBM European calls (spot S, strike X, expiry T, rate r,
@@ -62,9 +78,10 @@ void MonteCarloTimeStep(float* h_CallResult, float* h_CallConfidence, float* S, 
 // payoff and the 95 % confidence half-width, the sample's CallResultList and
 // CallConfidenceList (:251-268).  blockLen is validated as the sample
 // validates RAND_BLOCK_LENGTH (:165-174: at least 16, a multiple of 16, at
-// most nPaths, dividing nPaths); the GPU does not block.  Normals are the
-// engine's Philox stream of `seed`.  The double overload runs FP64, the float
-// one FP32.
+// most nPaths, dividing nPaths); the GPU does not block (with inverse-CDF
+// normals the stream order does not depend on it).  Normals as
+// european_config() says (by default the sample's SFMT19937 streams of
+// seed 777 + thread).  The double overload runs FP64, the float one FP32.
 void price_european(const double* S, const double* X, const double* T, double r, double sigma, int nOpt,
                     int nPaths, int blockLen, double* CallResult, double* CallConfidence,
                     unsigned long long seed = 777);

@@ -8,6 +8,7 @@ recursion in numpy (FP64 to 1e-11 relative: the normals agree to ~1e-15 and
 exp2 to an ulp per step), FP32 against FP64 to 2e-5.  price_american_lsmc and
 price_european must equal the C-ABI calls they wrap bit for bit."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -128,8 +129,96 @@ def test_price_american_lsmc_dropin(engine, real):
     assert 0.95 * eu.value < am.value < 1.01 * eu.value and 15.0 < eu.value < 25.0
 
 
+class EuropeanConfig(ctypes.Structure):
+    _fields_ = [("sample_stream", ctypes.c_bool), ("threads", ctypes.c_int)]
+
+
+def _european_cfg():
+    fn = mc.lib()._ZN11fsmc_dropin15european_configEv
+    fn.restype = ctypes.POINTER(EuropeanConfig)
+    return fn().contents
+
+
+def _price_european(real, S, X, T, r, sig, paths, block, seed):
+    ct = ctypes.c_double if real == np.float64 else ctypes.c_float
+    sym = "_Z14price_europeanPKdS0_S0_ddiiiPdS1_y" if real == np.float64 else "_Z14price_europeanPKfS0_S0_ffiiiPfS1_y"
+    fn = getattr(mc.lib(), sym)
+    P = ctypes.POINTER(ct)
+    fn.argtypes = [P, P, P, ct, ct, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_ulonglong]
+    fn.restype = None
+    S, X, T = (np.ascontiguousarray(v, dtype=real) for v in (S, X, T))
+    res, conf = np.zeros(len(S), dtype=real), np.zeros(len(S), dtype=real)
+    fn(*[v.ctypes.data_as(P) for v in (S, X, T)], r, sig, len(S), paths, block, res.ctypes.data_as(P),
+       conf.ctypes.data_as(P), seed)
+    return res, conf
+
+
+def test_price_european_sample_stream(engine):
+    # the sample's own normals: thread t of 2 prices options [2t, 2t + 2) from
+    # SFMT19937 seed 777 + t, paths in order, s = MuByT + VBySqrtT z (float);
+    # the oracle's SFMT19937 (pinned to MKL's outputs) through the oracle's
+    # buffer-mode estimator
+    cfg = _european_cfg()
+    saved = (cfg.sample_stream, cfg.threads)
+    cfg.sample_stream, cfg.threads = True, 2
+    try:
+        from scipy.special import ndtri
+        rng = np.random.default_rng(3)
+        S, X, T = rng.uniform(5, 50, 4), rng.uniform(10, 25, 4), rng.uniform(1, 5, 4)
+        r, sig, paths = 0.06, 0.10, 1024
+        res, conf = _price_european(np.float64, S, X, T, r, sig, paths, 256, 777)
+        z = np.concatenate([ndtri(orc.sfmt19937_icdf_uniforms(777 + t, 2 * paths)) for t in range(2)])
+        z = z.reshape(4, paths)
+        mu = (LOG2E * (r - .5 * sig * sig) * T)[:, None]
+        vol = (LOG2E * sig * np.sqrt(T))[:, None]
+        samples = (mu + vol * z).astype(np.float32)
+        ores, oconf = orc.gbm_buffer(S, X, T, r, sig, samples)
+        assert np.max(np.abs(res - ores) / np.maximum(ores, 1e-12)) < 1e-6
+        assert np.max(np.abs(conf - oconf) / np.maximum(oconf, 1e-12)) < 1e-5
+        cfg.threads = 3  # 4 options cannot be split over 3 sample threads
+        bad, _ = _price_european(np.float64, S, X, T, r, sig, paths, 256, 777)
+        assert np.all(np.isnan(bad))
+    finally:
+        cfg.sample_stream, cfg.threads = saved
+
+
+def test_sfmt19937_generator(engine):
+    # one stream against MKL's own float normals and the oracle's exact
+    # inverse CDF; several streams with per-segment mean and scale
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mkl_streams.npz"))
+    from scipy.special import ndtri
+    z = engine.sfmt19937_gaussian(777, 2048)
+    assert np.max(np.abs(z - d["sfmt777_icdf_float"])) < 2e-5
+    exact = ndtri(orc.sfmt19937_icdf_uniforms(777, 2048))
+    assert np.max(np.abs(z - exact) / np.maximum(np.abs(exact), 1e-3)) < 2e-7
+    import torch
+    dev = torch.device("cuda:0")
+    n_streams, per, seg = 3, 1500, 500  # 1500 = 2 x 624 + 252: a partial block
+    mean = torch.tensor([0.5 * k for k in range(9)], dtype=torch.float64, device=dev)
+    sd = torch.tensor([1.0 + k for k in range(9)], dtype=torch.float64, device=dev)
+    out = torch.zeros(n_streams * per, dtype=torch.float32, device=dev)
+    mc.sfmt19937_gaussian_async(5, n_streams, per, out, seg, mean, sd)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().reshape(n_streams, per)
+    for t in range(n_streams):
+        e = ndtri(orc.sfmt19937_icdf_uniforms(5 + t, per))
+        k = (t * per + np.arange(per)) // seg
+        want = mean.cpu().numpy()[k] + sd.cpu().numpy()[k] * e
+        assert np.max(np.abs(got[t] - want) / np.maximum(np.abs(want), 1e-3)) < 2e-7
+
+
 @pytest.mark.parametrize("real", [np.float64, np.float32])
 def test_price_european_dropin(engine, real):
+    cfg = _european_cfg()
+    saved = (cfg.sample_stream, cfg.threads)
+    cfg.sample_stream = False  # the engine's Philox normals
+    try:
+        _price_european_philox(engine, real)
+    finally:
+        cfg.sample_stream, cfg.threads = saved
+
+
+def _price_european_philox(engine, real):
     ct = ctypes.c_double if real == np.float64 else ctypes.c_float
     sym = "_Z14price_europeanPKdS0_S0_ddiiiPdS1_y" if real == np.float64 else "_Z14price_europeanPKfS0_S0_ffiiiPfS1_y"
     fn = getattr(mc.lib(), sym)

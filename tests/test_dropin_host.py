@@ -27,6 +27,7 @@ MANGLED = {
     "price_american_lsmc": ["_Z19price_american_lsmciiddddPdS_j", "_Z19price_american_lsmciiffffPfS_j"],
     "asian_config": ["_ZN11fsmc_dropin12asian_configEv"],
     "american_config": ["_ZN11fsmc_dropin15american_configEv"],
+    "european_config": ["_ZN11fsmc_dropin15european_configEv"],
 }
 
 
