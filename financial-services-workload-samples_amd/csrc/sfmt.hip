@@ -16,9 +16,10 @@
 // float normals for seed 777 pin both).  The inverse CDF is AS241 in FP64,
 // rounded to float once with the affine map mean + stddev z.
 //
-// One stream is sequential: a 64-lane workgroup per stream seeds the state
-// in LDS (one lane), runs the 156-word recursion in registers (one 32-bit
-// column per lane), then all 64 lanes turn the new block into normals.
+// One stream is sequential: a two-wave workgroup per stream seeds the state
+// in LDS (one lane); one wave runs the 156-word recursion in registers (one
+// 32-bit column per lane) while the other turns the previous block into
+// normals.
 #include <algorithm>
 
 #include "fsmc_device.hpp"
@@ -70,52 +71,64 @@ __device__ __forceinline__ uint32_t quad_mov(uint32_t v) {
 
 // n_per_stream normals of stream t into out[t * n_per_stream + i]; with mean
 // and stddev the value of global index g is mean[g / seg] + stddev[g / seg] z.
-// The state lives in registers, column j = lane & 3 of the 156 words in each
-// lane (every quad of the wave runs the same recursion); the 128-bit byte
-// shifts take the neighbouring column from the adjacent lane by a quad
-// permute.  Each new block goes through LDS to all 64 lanes for the inverse
-// CDF.
-__global__ __launch_bounds__(64) void sfmt_gaussian_kernel(uint32_t seed0, long long n_per_stream, long long seg,
-                                                           const double* mean, const double* stddev, float* out) {
+// Two waves per stream, producer and consumer: wave 0 keeps the state in
+// registers, column j = lane & 3 of the 156 words in each lane (every quad
+// runs the same recursion; the 128-bit byte shifts take the neighbouring
+// column from the adjacent lane by a quad permute) and writes block b to LDS
+// buffer b & 1, while wave 1 maps block b - 1 from the other buffer through
+// the inverse CDF; one barrier per block separates the two.
+__global__ __launch_bounds__(128) void sfmt_gaussian_kernel(uint32_t seed0, long long n_per_stream, long long seg,
+                                                            const double* mean, const double* stddev, float* out) {
   constexpr int kQ = kWords / 4;  // 128-bit words
-  __shared__ uint32_t w[kWords];
+  __shared__ uint32_t w[2][kWords];
   const int t = threadIdx.x, j = t & 3;
+  const bool producer = t < 64;  // wave-uniform
   const size_t base = (size_t)blockIdx.x * (size_t)n_per_stream;
-  if (t == 0) sfmt_seed(seed0 + blockIdx.x, w);
+  const long long n_blocks = (n_per_stream + kWords - 1) / kWords;
+  if (t == 0) sfmt_seed(seed0 + blockIdx.x, w[1]);
   __syncthreads();
   uint32_t st[kQ];
-#pragma unroll
-  for (int q = 0; q < kQ; ++q) st[q] = w[4 * q + j];
   const uint32_t msk = j == 0 ? kMsk[0] : j == 1 ? kMsk[1] : j == 2 ? kMsk[2] : kMsk[3];
-  for (long long done = 0; done < n_per_stream; done += kWords) {
+  if (producer) {
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) {  // the recursion, word by word, in place
-      const uint32_t a = st[q], b = st[(q + 122) % kQ], c = st[(q + kQ - 2) % kQ], d = st[(q + kQ - 1) % kQ];
-      const uint32_t a_lo = quad_mov<0x90>(a);  // column j - 1 (quad_perm 0,0,1,2)
-      const uint32_t c_hi = quad_mov<0xF9>(c);  // column j + 1 (quad_perm 1,2,3,3)
-      const uint32_t a_sl = (a << 8) | (j > 0 ? a_lo >> 24 : 0u);
-      const uint32_t c_sr = (c >> 8) | (j < 3 ? c_hi << 24 : 0u);
-      st[q] = a ^ a_sl ^ ((b >> 11) & msk) ^ c_sr ^ (d << 18);
-    }
-    __syncthreads();  // the previous block's conversion has read w
-    if (t < 4) {
+    for (int q = 0; q < kQ; ++q) st[q] = w[1][4 * q + j];
+  }
+  __syncthreads();  // the seed buffer is read before block 1 reuses it
+  for (long long b = 0; b <= n_blocks; ++b) {
+    if (producer && b < n_blocks) {
 #pragma unroll
-      for (int q = 0; q < kQ; ++q) w[4 * q + t] = st[q];
-    }
-    __syncthreads();
-    const int m = (int)std::min<long long>(kWords, n_per_stream - done);
-    for (int i = t; i < m; i += 64) {
-      // int32 = -2^31 gives u = 0, where the inverse CDF diverges: 2^-33 instead
-      const double u = fmax((double)(int32_t)w[i] * 0x1.0p-32 + 0.5, 0x1.0p-33);
-      const double z = icdf(u);
-      const size_t g = base + (size_t)(done + i);
-      if (mean) {
-        const size_t k = g / (size_t)seg;
-        out[g] = (float)(mean[k] + stddev[k] * z);
-      } else {
-        out[g] = (float)z;
+      for (int q = 0; q < kQ; ++q) {  // the recursion, word by word, in place
+        const uint32_t a = st[q], bb = st[(q + 122) % kQ], c = st[(q + kQ - 2) % kQ], d = st[(q + kQ - 1) % kQ];
+        const uint32_t a_lo = quad_mov<0x90>(a);  // column j - 1 (quad_perm 0,0,1,2)
+        const uint32_t c_hi = quad_mov<0xF9>(c);  // column j + 1 (quad_perm 1,2,3,3)
+        const uint32_t a_sl = (a << 8) | (j > 0 ? a_lo >> 24 : 0u);
+        const uint32_t c_sr = (c >> 8) | (j < 3 ? c_hi << 24 : 0u);
+        st[q] = a ^ a_sl ^ ((bb >> 11) & msk) ^ c_sr ^ (d << 18);
+      }
+      if (t < 4) {
+        uint32_t* dst = w[b & 1];
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) dst[4 * q + t] = st[q];
       }
     }
+    if (!producer && b >= 1) {
+      const long long done = (b - 1) * kWords;
+      const uint32_t* src = w[(b - 1) & 1];
+      const int m = (int)std::min<long long>(kWords, n_per_stream - done);
+      for (int i = t - 64; i < m; i += 64) {
+        // int32 = -2^31 gives u = 0, where the inverse CDF diverges: 2^-33 instead
+        const double u = fmax((double)(int32_t)src[i] * 0x1.0p-32 + 0.5, 0x1.0p-33);
+        const double z = icdf(u);
+        const size_t g = base + (size_t)(done + i);
+        if (mean) {
+          const size_t k = g / (size_t)seg;
+          out[g] = (float)(mean[k] + stddev[k] * z);
+        } else {
+          out[g] = (float)z;
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -141,7 +154,7 @@ int fsmc_sfmt19937_gaussian_async(uint32_t seed0, int n_streams, long long n_per
                                   const double* d_mean, const double* d_stddev, float* d_out, void* stream) {
   FSMC_TRY(sfmt_validate(n_streams, n_per_stream, seg, d_mean, d_stddev, d_out));
   if (n_per_stream == 0) return FSMC_OK;
-  hipLaunchKernelGGL(sfmt_gaussian_kernel, dim3(n_streams), dim3(64), 0, (hipStream_t)stream, seed0, n_per_stream,
+  hipLaunchKernelGGL(sfmt_gaussian_kernel, dim3(n_streams), dim3(128), 0, (hipStream_t)stream, seed0, n_per_stream,
                      d_mean ? seg : 1LL, d_mean, d_stddev, d_out);
   FSMC_HIP(hipGetLastError());
   return FSMC_OK;
