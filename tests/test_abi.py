@@ -66,6 +66,17 @@ def test_invalid_arguments_rejected_without_device(lib):
     h = mcengine.HestonParams(2, 128, 12, 0.03, -1.5)
     st = lib.fsmc_heston_asian_async(ctypes.byref(h), ctypes.byref(m), *([None] * 12), None, 0, None)
     assert st == -1 and b"rho" in lib.fsmc_last_error()
+    # SFMT19937 streams: stream count, a mean without a scale, a NULL output
+    assert lib.fsmc_sfmt19937_gaussian_async(777, 0, 10, 1, None, None, None, None) == -1
+    assert b"n_streams" in lib.fsmc_last_error()
+    buf = (ctypes.c_double * 1)()
+    assert lib.fsmc_sfmt19937_gaussian_async(777, 1, 10, 1, buf, None, None, None) == -1
+    assert b"stddev" in lib.fsmc_last_error()
+    assert lib.fsmc_sfmt19937_gaussian_async(777, 1, 10, 1, None, None, None, None) == -1
+    assert b"NULL output" in lib.fsmc_last_error()
+    # control_variate is 0 or 1
+    cv = mcengine.GbmParams(4, 1024, 0.02, 0.3, 2)
+    assert lib.fsmc_gbm_european_workspace(ctypes.byref(cv), ctypes.byref(m)) == 0
 
 
 def test_workspace_sizes(lib):
