@@ -25,13 +25,15 @@ namespace {
 struct AsianBuffers {
   double *prm = nullptr, *z1 = nullptr, *z2 = nullptr, *res = nullptr, *conf = nullptr;
   void* ws = nullptr;
-  hipStream_t s = nullptr;
+  hipStream_t s = nullptr, s2 = nullptr;  // s2 generates the second stream's normals
   std::vector<hipEvent_t> ev;
   fsmc_mt19937 *g1 = nullptr, *g2 = nullptr;
   ~AsianBuffers() {
     if (s) (void)hipStreamSynchronize(s);
+    if (s2) (void)hipStreamSynchronize(s2);
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     if (s) (void)hipStreamDestroy(s);
+    if (s2) (void)hipStreamDestroy(s2);
     for (void* p : {(void*)prm, (void*)z1, (void*)z2, (void*)res, (void*)conf, ws})
       if (p) (void)hipFree(p);
     fsmc_mt19937_destroy(g1);
@@ -53,7 +55,8 @@ bool fsmc_ok(int st, const char* where) {
 
 // Prices with the sample's loop structure (MCAsianOptions.cpp:86-212):
 // streams created per call, NUM_LOOPS pricings of all options, each option
-// drawing its paths x steps normals from both streams in turn.  Returns
+// drawing its paths x steps normals from both streams (the two independent
+// sequential generators run concurrently on two HIP streams).  Returns
 // false on a failure (results then NaN).
 bool price(const double* prm, int n, int precision, double* res, double* conf) {
   const fsmc_dropin::AsianConfig& c = fsmc_dropin::asian_config();
@@ -74,21 +77,29 @@ bool price(const double* prm, int n, int precision, double* res, double* conf) {
       !hip_ok(hipMalloc((void**)&b.res, n * sizeof(double)), "hipMalloc") ||
       !hip_ok(hipMalloc((void**)&b.conf, n * sizeof(double)), "hipMalloc") ||
       !hip_ok(hipMalloc(&b.ws, ws_bytes), "hipMalloc") || !hip_ok(hipStreamCreate(&b.s), "hipStreamCreate") ||
+      !hip_ok(hipStreamCreate(&b.s2), "hipStreamCreate") ||
       !hip_ok(hipMemcpy(b.prm, prm, 7 * (size_t)n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy"))
     return false;
-  b.ev.resize(2 * (size_t)n);
+  // per option: [0] generation start, [1] generation end (both streams), [2] s2's part done
+  b.ev.resize(3 * (size_t)n);
   for (auto& e : b.ev)
-    if (!hip_ok(hipEventCreate(&e), "hipEventCreate")) return false;
+    if (!hip_ok(hipEventCreateWithFlags(&e, hipEventDefault), "hipEventCreate")) return false;
   double total = 0.0, rng = 0.0;
   for (int l = 0; l < c.num_loops; ++l) {
     const auto t0 = std::chrono::steady_clock::now();
     for (int o = 0; o < n; ++o) {
       const double* q = b.prm + o;
       const char* gen = "fsmc_mt19937_gaussian_async";
-      if (!hip_ok(hipEventRecord(b.ev[2 * o], b.s), "hipEventRecord") ||
+      hipEvent_t* e = &b.ev[3 * (size_t)o];
+      // the two streams are independent: the second one's normals are drawn
+      // on s2, after the previous option's pricing has read z2
+      if (!hip_ok(hipEventRecord(e[0], b.s), "hipEventRecord") ||
+          !hip_ok(hipStreamWaitEvent(b.s2, e[0], 0), "hipStreamWaitEvent") ||
+          !fsmc_ok(fsmc_mt19937_gaussian_async(b.g2, per_option, 0.0, 1.0, b.z2, b.s2), gen) ||
+          !hip_ok(hipEventRecord(e[2], b.s2), "hipEventRecord") ||
           !fsmc_ok(fsmc_mt19937_gaussian_async(b.g1, per_option, 0.0, 1.0, b.z1, b.s), gen) ||
-          !fsmc_ok(fsmc_mt19937_gaussian_async(b.g2, per_option, 0.0, 1.0, b.z2, b.s), gen) ||
-          !hip_ok(hipEventRecord(b.ev[2 * o + 1], b.s), "hipEventRecord") ||
+          !hip_ok(hipStreamWaitEvent(b.s, e[2], 0), "hipStreamWaitEvent") ||
+          !hip_ok(hipEventRecord(e[1], b.s), "hipEventRecord") ||
           !fsmc_ok(fsmc_heston_asian_async(&p1, &m, q, q + n, q + 2 * n, q + 3 * n, q + 4 * n, q + 5 * n, q + 6 * n,
                                            b.z1, b.z2, b.res + o, b.conf + o, nullptr, b.ws, ws_bytes, b.s),
                    "fsmc_heston_asian_async"))
@@ -98,7 +109,8 @@ bool price(const double* prm, int n, int precision, double* res, double* conf) {
     total += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (int o = 0; o < n; ++o) {
       float ms = 0.0f;
-      if (!hip_ok(hipEventElapsedTime(&ms, b.ev[2 * o], b.ev[2 * o + 1]), "hipEventElapsedTime")) return false;
+      if (!hip_ok(hipEventElapsedTime(&ms, b.ev[3 * (size_t)o], b.ev[3 * (size_t)o + 1]), "hipEventElapsedTime"))
+        return false;
       rng += 1e-3 * ms;
     }
   }
