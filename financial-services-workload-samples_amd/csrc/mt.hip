@@ -17,12 +17,13 @@
 //
 // The stream is sequential.  A generator object keeps the state in device
 // memory; each call continues it, as successive vdRngGaussian calls on one
-// VSL stream do.  The bits kernel is one workgroup: the state sits in LDS and
-// the twist runs as three dependent phases (words [0, 227) read old words
-// i + 397; [227, 454) read the words phase one wrote; [454, 624) those of
-// phase two, the last word also the new word 0), each a read into registers,
-// a barrier and a write.  The Box-Muller transform then runs over the bits
-// in a grid-wide kernel.
+// VSL stream do.  The bits kernel is one workgroup: the state sits in LDS,
+// double-buffered, and the twist runs as three dependent phases (words
+// [0, 227) read old words i + 397; [227, 454) read the words phase one
+// wrote; [454, 624) those of phase two, the last word also the new word 0),
+// each reading the old buffer, writing the new one and ending in one
+// barrier.  The Box-Muller transform then runs over the bits in a grid-wide
+// kernel.
 #include <algorithm>
 #include <vector>
 
@@ -78,44 +79,66 @@ void init_by_seed(uint32_t seed, MtState& s) {
   s.pos = kN;
 }
 
-// Next n outputs of the stream in st into out[0, n); one workgroup.
+// Next n outputs of the stream in st into out[0, n); one workgroup.  The
+// state is double-buffered in LDS, so each twist phase reads the old words
+// of one buffer and writes the new words of the other with a single barrier
+// per phase; a block that is output whole is tempered and stored straight
+// from the registers of the phase that made it.
 __global__ __launch_bounds__(kBitsThreads) void mt_bits_kernel(MtState* st, long long n, uint32_t* out) {
-  __shared__ uint32_t mt[kN];
+  __shared__ uint32_t buf[2][kN];
   const int t = threadIdx.x;
-  for (int i = t; i < kN; i += kBitsThreads) mt[i] = st->mt[i];
+  uint32_t* A = buf[0];  // current state
+  uint32_t* B = buf[1];  // the next twist's state
+  for (int i = t; i < kN; i += kBitsThreads) A[i] = st->mt[i];
   int pos = (int)st->pos;  // uniform
   __syncthreads();
   long long done = 0;
+  constexpr int kH = kN - kM;  // 227
   while (done < n) {
     if (pos == kN) {
+      const bool whole = n - done >= kN;  // uniform: the new block goes out entirely
+      uint32_t* o = out + done;
       // phase 1: words [0, 227) from old words i, i + 1, i + 397
-      uint32_t v = 0;
-      if (t < kN - kM) v = twist_word(mt[t], mt[t + 1], mt[t + kM]);
-      __syncthreads();
-      if (t < kN - kM) mt[t] = v;
+      if (t < kH) {
+        const uint32_t v = twist_word(A[t], A[t + 1], A[t + kM]);
+        B[t] = v;
+        if (whole) o[t] = temper(v);
+      }
       __syncthreads();
       // phase 2: words [227, 454) from old i, i + 1 and new i - 227
-      const int i2 = t + (kN - kM);
-      if (t < kN - kM) v = twist_word(mt[i2], mt[i2 + 1], mt[i2 - (kN - kM)]);
-      __syncthreads();
-      if (t < kN - kM) mt[i2] = v;
+      if (t < kH) {
+        const int i = t + kH;
+        const uint32_t v = twist_word(A[i], A[i + 1], B[i - kH]);
+        B[i] = v;
+        if (whole) o[i] = temper(v);
+      }
       __syncthreads();
       // phase 3: words [454, 624) from old i, new i - 227, and for the last
       // word the new word 0
-      const int i3 = t + 2 * (kN - kM);
-      if (i3 < kN) v = twist_word(mt[i3], mt[i3 + 1 < kN ? i3 + 1 : 0], mt[i3 - (kN - kM)]);
+      if (t + 2 * kH < kN) {
+        const int i = t + 2 * kH;
+        const uint32_t v = twist_word(A[i], i + 1 < kN ? A[i + 1] : B[0], B[i - kH]);
+        B[i] = v;
+        if (whole) o[i] = temper(v);
+      }
       __syncthreads();
-      if (i3 < kN) mt[i3] = v;
-      __syncthreads();
+      uint32_t* tmp = A;
+      A = B;
+      B = tmp;
       pos = 0;
+      if (whole) {
+        pos = kN;
+        done += kN;
+        continue;
+      }
     }
     const int m = (int)std::min<long long>(kN - pos, n - done);
-    for (int i = t; i < m; i += kBitsThreads) out[done + i] = temper(mt[pos + i]);
+    for (int i = t; i < m; i += kBitsThreads) out[done + i] = temper(A[pos + i]);
     pos += m;
     done += m;
   }
   __syncthreads();
-  for (int i = t; i < kN; i += kBitsThreads) st->mt[i] = mt[i];
+  for (int i = t; i < kN; i += kBitsThreads) st->mt[i] = A[i];
   if (t == 0) st->pos = (uint32_t)pos;
 }
 
