@@ -33,16 +33,19 @@ template <typename Real, bool kGreeks, bool kCV>
 struct GbmAcc {
   // Payoff sum and sum of squares; with Greeks also the pathwise delta and
   // vega kernels (and their squares) and the in-the-money count (rho); with
-  // the control variate the sums of u, u^2 and u y.
-  CompSum<Real> v0, v1, dl, dl2, vg, vg2, cu, cuu, cuy;
+  // the control variate the sums of u, u^2 and u y, accumulated in FP64 (an
+  // FP64 add issues at the FP32 rate here: one conversion and one add per
+  // term cost less than a compensated FP32 sum).
+  CompSum<Real> v0, v1, dl, dl2, vg, vg2;
+  PathSum cu, cuu, cuy;
   uint32_t itm = 0;
   Real F = 0;  // S e^{rT}, the mean of S_T
   __device__ __forceinline__ void control(Real st, Real c) {
     if constexpr (kCV) {
       const Real u = st - F;
-      cu.add(u);
-      cuu.add(u * u);
-      cuy.add(c > Real(0) ? u * c : Real(0));
+      cu.add((double)u);
+      cuu.add((double)(u * u));
+      cuy.add(c > Real(0) ? (double)(u * c) : 0.0);
     }
   }
   __device__ __forceinline__ void hit(Real c, Real st, Real dlog) {
