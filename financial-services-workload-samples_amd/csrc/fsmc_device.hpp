@@ -100,13 +100,14 @@ __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double& z0, d
   z1 = r * s;
 }
 
-// The same pair in polar form: z0 = r c, z1 = r s returned as (r, c, s), for
-// consumers that multiply both normals by a common factor and can fold r
-// into it (same r, c, s as box_muller computes).
-__device__ __forceinline__ void box_muller_polar(uint32_t a, uint32_t b, float& r, float& c, float& s) {
+// The same pair in polar form: z0 = r c, z1 = r s returned as (r^2, c, s),
+// r^2 = -2 ln u1, for consumers that multiply both normals by a common
+// factor and can fold r into its square root (the c, s and r^2 that
+// box_muller's FP32 form uses).
+__device__ __forceinline__ void box_muller_polar(uint32_t a, uint32_t b, float& r2, float& c, float& s) {
   const float f = (float)(a | 1u);
   const float u2 = (float)b * 0x1.0p-32f;
-  r = __builtin_amdgcn_sqrtf(__builtin_fmaf(-1.3862943611198906f, __builtin_amdgcn_logf(f), 44.361419555836500f));
+  r2 = __builtin_fmaf(-1.3862943611198906f, __builtin_amdgcn_logf(f), 44.361419555836500f);
   c = __builtin_amdgcn_cosf(u2);
   s = __builtin_amdgcn_sinf(u2);
 }

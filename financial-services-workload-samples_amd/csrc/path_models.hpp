@@ -53,14 +53,15 @@ __device__ __forceinline__ void heston_step(const HestonConsts<Real>& c, Real& S
 }
 
 // The FP32 Philox step with its normals in polar form, z1 = r cs, z2 = r sn
-// (box_muller_polar): both normals only ever appear multiplied by sqrt(v+),
-// so q = sqrt(v+) r takes the radius in one multiply instead of two.
+// (box_muller_polar gives r2 = r^2): both normals only ever appear
+// multiplied by sqrt(v+), so q = sqrt(v+ r^2) takes the radius inside the
+// step's one square root (a transcendental issues at half the FMA rate).
 template <typename Real>
-__device__ __forceinline__ void heston_step_polar(const HestonConsts<Real>& c, Real& S, Real& v, Real r, Real cs,
+__device__ __forceinline__ void heston_step_polar(const HestonConsts<Real>& c, Real& S, Real& v, Real r2, Real cs,
                                                   Real sn) {
 #pragma clang fp contract(off)
   const Real vp = positive_part(v);
-  const Real q = fsqrt(vp) * r;
+  const Real q = fsqrt(vp * r2);
   const Real e = ffma(c.c_sdt * q, cs, ffma(-c.c_hdt, vp, c.c_rdt));
   S = S * fexp2(e);
   v = ffma(q, ffma(c.xr, cs, c.xs * sn), ffma(c.kdt, c.theta - vp, v));
@@ -93,13 +94,13 @@ __device__ __forceinline__ void heston_philox_pair(const HestonConsts<Real>& c, 
 #pragma clang fp contract(off)
   if constexpr (sizeof(Real) == 4 && !kVarAsVol) {
     const U4 w = philox4x32_10(ctr, k0, k1);
-    float r0, c0, s0, r1, c1, s1;
-    box_muller_polar(w.x, w.y, r0, c0, s0);
-    box_muller_polar(w.z, w.w, r1, c1, s1);
-    heston_step_polar(c, S, v, r0, c0, s0);
+    float q0, c0, s0, q1, c1, s1;  // q0, q1: the squared radii
+    box_muller_polar(w.x, w.y, q0, c0, s0);
+    box_muller_polar(w.z, w.w, q1, c1, s1);
+    heston_step_polar(c, S, v, q0, c0, s0);
     if (second) {
       const Real sa = S;
-      heston_step_polar(c, S, v, r1, c1, s1);
+      heston_step_polar(c, S, v, q1, c1, s1);
       avg.add(sa + S);
     } else {
       avg.add(S);
@@ -163,19 +164,20 @@ __device__ __forceinline__ double heston_adjoint_path(const HestonConsts<Real>& 
         // recompute exactly as the forward sweep stepped (heston_philox_pair)
         if constexpr (sizeof(Real) == 4) {
           const U4 q = philox4x32_10(U4{p, (uint32_t)(k >> 1), c2, w}, k0, k1);
-          float r0, c0, s0, r1, c1, s1;
-          box_muller_polar(q.x, q.y, r0, c0, s0);
-          box_muller_polar(q.z, q.w, r1, c1, s1);
+          float q0, c0, s0, q1, c1, s1;  // q0, q1: the squared radii
+          box_muller_polar(q.x, q.y, q0, c0, s0);
+          box_muller_polar(q.z, q.w, q1, c1, s1);
+          const float r0 = fsqrt(q0), r1 = fsqrt(q1);
           Ss[j] = Sr;
           vs[j] = vr;
           z1s[j] = r0 * c0;
           zcs[j] = r0 * (c.rho * c0 + c.srho * s0);
-          heston_step_polar(c, Sr, vr, r0, c0, s0);
+          heston_step_polar(c, Sr, vr, q0, c0, s0);
           Ss[j + 1] = Sr;
           vs[j + 1] = vr;
           z1s[j + 1] = r1 * c1;
           zcs[j + 1] = r1 * (c.rho * c1 + c.srho * s1);
-          if (k + 1 < N) heston_step_polar(c, Sr, vr, r1, c1, s1);
+          if (k + 1 < N) heston_step_polar(c, Sr, vr, q1, c1, s1);
         } else {
           Real n0, n1, n2, n3;
           normals4<Real>(U4{p, (uint32_t)(k >> 1), c2, w}, k0, k1, n0, n1, n2, n3);
