@@ -1,8 +1,8 @@
 /*
  * CPU oracle of the Monte Carlo engine -- TEST INFRASTRUCTURE ONLY.
  *
- * Used by tests/ as the checker of the GPU kernels and by bench.py as the
- * timed CPU baseline.  Plain FP64 C; each function cites the reference
+ * Used by tests/ as the results the GPU kernels are compared with and by
+ * bench.py as the timed CPU baseline.  Plain FP64 C; each function cites the reference
  * file:line whose algorithm it restates.  Parity pins of this file (see the
  * tests named in each comment):
  *   philox        Random123 known-answer vectors        tests/test_oracle_pins.py

@@ -1,7 +1,7 @@
 /*
- * CPU oracle of the Monte Carlo engine.  TEST INFRASTRUCTURE ONLY: this code
- * is the checker the GPU results are compared against (and the CPU baseline
- * that bench.py times); no product code path links or calls it.
+ * CPU oracle of the Monte Carlo engine.  TEST INFRASTRUCTURE ONLY: the GPU
+ * results are compared against it (and bench.py times it as the CPU
+ * baseline); no product code path links or calls it.
  *
  * Every function restates the reference algorithm in plain FP64 C, citing
  * the reference file:line it follows.  The Philox/Box-Muller contract is the

@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/fsmc_oracle.c).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
-as the checker, and by bench.py's ``cpu_baseline`` leg.  The product path
+TEST INFRASTRUCTURE ONLY: imported by tests/ and ``__graft_entry__.smoke()``
+(the results the GPU kernels are compared with) and by bench.py's
+``cpu_baseline`` leg.  The product path
 never imports this module.
 """
 import ctypes
