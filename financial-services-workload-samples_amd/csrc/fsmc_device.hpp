@@ -276,6 +276,8 @@ __device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(
 __device__ __forceinline__ double fexp2(double x) { return exp2(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+// 1/sqrt(x) in FP32: the hardware reciprocal square root (one instruction, ~1 ulp).
+__device__ __forceinline__ float frsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 // Explicit fused multiply-add: a * b + c with one rounding.
 __device__ __forceinline__ float ffma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
