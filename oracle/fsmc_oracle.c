@@ -16,6 +16,17 @@
  *                 theta, truncation, both vol conventions)
  *   mt19937       MKL's outputs for seed 42 and numpy's RandomState([seed])
  *                 raw draws (the same init_by_array seeding)
+ *   heston Q5     closed form of the running-val2 confidence (identical
+ *                 deterministic paths)
+ *   geometric     exact-GBM Monte Carlo for N = 4 and 12 dates
+ *   libor         adjoint = central differences; lambda = 0 closed form;
+ *                 deep-in-the-money book = the linear (martingale) value for
+ *                 lambda = 0.1, 0.2 (pins the spot-measure drift)
+ *   lsmc_ref      numSteps = 2 closed form (1/N terminal scaling, terminal
+ *                 date); a hand-placed 5-path case whose decisions need the
+ *                 lagged, ridged fit
+ * tests/test_oracle_mutants.py plants the mistakes these pins are meant to
+ * catch in copies of this file and shows that each pin fails on them.
  */
 #include "fsmc_oracle.h"
 
@@ -616,7 +627,7 @@ static void invert3(double a[3][3], double x[3][3]) {
 }
 
 void orc_lsmc_ref(int n_paths, int n_steps, double spot, double strike, double vol, double maturity,
-                  const double* z, double* american, double* european) {
+                  const double* z, double* american, double* european, double* betas) {
   const double dt = maturity / n_steps;
   const double volSqrtdt = vol * sqrt(dt);
   const double fwdFactor = vol * vol * dt * -0.5;
@@ -668,6 +679,11 @@ void orc_lsmc_ref(int n_paths, int n_steps, double spot, double strike, double v
     b0 = x0 * inv[0][0] + x1 * inv[1][0] + x2 * inv[2][0];
     b1 = x0 * inv[0][1] + x1 * inv[1][1] + x2 * inv[2][1];
     b2 = x0 * inv[0][2] + x1 * inv[1][2] + x2 * inv[2][2];
+    if (betas) {
+      betas[3 * s] = b0;
+      betas[3 * s + 1] = b1;
+      betas[3 * s + 2] = b2;
+    }
   }
   double am = 0.0, eu = 0.0;
   for (size_t p = 0; p < N; ++p) {

@@ -74,9 +74,11 @@ void orc_libor(int n_paths, double L0, double lambda, const double* z, uint64_t 
 
 /* ---- Longstaff-Schwartz American put ------------------------------------ */
 /* Reference algorithm (refcompat): one option, z[n_steps][n_paths] normals,
- * quadratic basis, ridge 0.01, cofactor inverse, lagged betas, no rate. */
+ * quadratic basis, ridge 0.01, cofactor inverse, lagged betas, no rate.
+ * betas (NULL or [n_steps][3]): row s = the coefficients regressed at date
+ * s = n_steps-1 .. 1 (the ones the decisions at date s-1 use). */
 void orc_lsmc_ref(int n_paths, int n_steps, double spot, double strike, double vol, double maturity,
-                  const double* z, double* american, double* european);
+                  const double* z, double* american, double* european, double* betas);
 /* Correct LSMC: cash flows discounted to 0, cubic basis in the standardized
  * moneyness x = 1 + (S/K - 1)/(sigma sqrt(t)), FP64 Cholesky per date.
  * price2 = [n][2] {price, se}; european may be NULL. */

@@ -5,6 +5,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/ and ``__graft_entry__.smoke()``
 ``cpu_baseline`` leg.  The product path
 never imports this module.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -31,52 +32,71 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        L.orc_philox4x32_10.argtypes = [_U32, _U32, _U32]
-        L.orc_box_muller.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _D, _D]
-        L.orc_normals4.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
-                                   ctypes.c_uint32, ctypes.c_uint32, _D]
-        for name in ("orc_bs_call", "orc_geometric_asian_call"):
-            getattr(L, name).restype = ctypes.c_double
-        L.orc_bs_call.argtypes = [ctypes.c_double] * 5
-        L.orc_bs_call_greeks.argtypes = [ctypes.c_double] * 5 + [_D]
-        L.orc_geometric_asian_call.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int]
-        for name in ("orc_crr_european_call", "orc_crr_american_put"):
-            getattr(L, name).restype = ctypes.c_double
-            getattr(L, name).argtypes = [ctypes.c_double] * 5 + [ctypes.c_int]
-        L.orc_gbm_buffer.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                     _D, _D, _D, _F, _D, _D]
-        L.orc_gbm_philox.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                     _D, _D, _D, ctypes.c_uint64, ctypes.c_uint32, _D, _D, _D]
-        L.orc_gbm_buffer_cv.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D, _D, _F, _D, _D]
-        L.orc_gbm_philox_cv.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                        _D, _D, _D, ctypes.c_uint64, ctypes.c_uint32, _D, _D]
-        L.orc_heston_buffer.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 2 + [_D] * 7 +
-                                        [_D, _D, ctypes.c_uint32, _D, _D])
-        L.orc_heston_philox.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 2 + [_D] * 7 +
-                                        [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _D, _D, _D])
-        L.orc_libor_path.restype = ctypes.c_double
-        L.orc_libor_path.argtypes = [_D, _D, ctypes.c_double, ctypes.c_uint32, _D]
-        L.orc_libor.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, ctypes.c_uint64,
-                                ctypes.c_uint32, ctypes.c_uint32, _D, _D]
-        L.orc_crr_bermudan_put.restype = ctypes.c_double
-        L.orc_crr_bermudan_put.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int, ctypes.c_int]
-        L.orc_lsmc_ref.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [_D, _D, _D]
-        _I = ctypes.POINTER(ctypes.c_int)
-        L.orc_lsmc.argtypes = [ctypes.c_int] + [_D] * 5 + [_I, _I, ctypes.c_uint64, ctypes.c_uint32, _D, _D]
-        L.orc_chol4_solve.restype = ctypes.c_int
-        L.orc_chol4_solve.argtypes = [_D, _D, _D]
-        L.orc_mrg32k3a_uniform.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64, _D]
-        L.orc_icdf.restype = ctypes.c_double
-        L.orc_icdf.argtypes = [ctypes.c_double]
-        L.orc_mrg32k3a_gaussian.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64, ctypes.c_double,
-                                            ctypes.c_double, _D]
-        L.orc_mt19937_bits.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64,
-                                       ctypes.POINTER(ctypes.c_uint32)]
-        L.orc_mt19937_gaussian.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64, ctypes.c_double,
-                                           ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def use(path):
+    """Route every call of this module to another build of the oracle (the
+    mutated copies the pin tests compile to show that each pin rejects a
+    planted mistake); the default build comes back on exit."""
+    global _lib
+    saved = lib()
+    _lib = load(path)
+    try:
+        yield _lib
+    finally:
+        _lib = saved
+
+
+def load(path):
+    """ctypes handle of an oracle shared library with every signature set."""
+    L = ctypes.CDLL(path)
+    L.orc_philox4x32_10.argtypes = [_U32, _U32, _U32]
+    L.orc_box_muller.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _D, _D]
+    L.orc_normals4.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                               ctypes.c_uint32, ctypes.c_uint32, _D]
+    for name in ("orc_bs_call", "orc_geometric_asian_call"):
+        getattr(L, name).restype = ctypes.c_double
+    L.orc_bs_call.argtypes = [ctypes.c_double] * 5
+    L.orc_bs_call_greeks.argtypes = [ctypes.c_double] * 5 + [_D]
+    L.orc_geometric_asian_call.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int]
+    for name in ("orc_crr_european_call", "orc_crr_american_put"):
+        getattr(L, name).restype = ctypes.c_double
+        getattr(L, name).argtypes = [ctypes.c_double] * 5 + [ctypes.c_int]
+    L.orc_gbm_buffer.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                 _D, _D, _D, _F, _D, _D]
+    L.orc_gbm_philox.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                 _D, _D, _D, ctypes.c_uint64, ctypes.c_uint32, _D, _D, _D]
+    L.orc_gbm_buffer_cv.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D, _D, _F, _D, _D]
+    L.orc_gbm_philox_cv.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                    _D, _D, _D, ctypes.c_uint64, ctypes.c_uint32, _D, _D]
+    L.orc_heston_buffer.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 2 + [_D] * 7 +
+                                    [_D, _D, ctypes.c_uint32, _D, _D])
+    L.orc_heston_philox.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 2 + [_D] * 7 +
+                                    [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _D, _D, _D])
+    L.orc_libor_path.restype = ctypes.c_double
+    L.orc_libor_path.argtypes = [_D, _D, ctypes.c_double, ctypes.c_uint32, _D]
+    L.orc_libor.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, ctypes.c_uint64,
+                            ctypes.c_uint32, ctypes.c_uint32, _D, _D]
+    L.orc_crr_bermudan_put.restype = ctypes.c_double
+    L.orc_crr_bermudan_put.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int, ctypes.c_int]
+    L.orc_lsmc_ref.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [_D, _D, _D, _D]
+    _I = ctypes.POINTER(ctypes.c_int)
+    L.orc_lsmc.argtypes = [ctypes.c_int] + [_D] * 5 + [_I, _I, ctypes.c_uint64, ctypes.c_uint32, _D, _D]
+    L.orc_chol4_solve.restype = ctypes.c_int
+    L.orc_chol4_solve.argtypes = [_D, _D, _D]
+    L.orc_mrg32k3a_uniform.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64, _D]
+    L.orc_icdf.restype = ctypes.c_double
+    L.orc_icdf.argtypes = [ctypes.c_double]
+    L.orc_mrg32k3a_gaussian.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64, ctypes.c_double,
+                                        ctypes.c_double, _D]
+    L.orc_mt19937_bits.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64,
+                                   ctypes.POINTER(ctypes.c_uint32)]
+    L.orc_mt19937_gaussian.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64, ctypes.c_double,
+                                       ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+    return L
 
 
 def _d(a):
@@ -249,12 +269,16 @@ def libor(n_paths, L0=0.05, lam=0.2, z=None, seed=0, stream=0, quirks=0):
 
 
 # ---- Longstaff-Schwartz ---------------------------------------------------------
-def lsmc_ref(n_paths, n_steps, spot, strike, vol, maturity, z):
+def lsmc_ref(n_paths, n_steps, spot, strike, vol, maturity, z, betas=False):
+    """(American, European); with betas=True also the [n_steps][3]
+    coefficients regressed at each date (row 0 unused)."""
     z, pz = _d(np.asarray(z).reshape(-1))
     assert z.size == n_paths * n_steps
     am, eu = ctypes.c_double(), ctypes.c_double()
-    lib().orc_lsmc_ref(n_paths, n_steps, spot, strike, vol, maturity, pz, ctypes.byref(am), ctypes.byref(eu))
-    return am.value, eu.value
+    b, pb = _d(np.full((n_steps, 3), np.nan))
+    lib().orc_lsmc_ref(n_paths, n_steps, spot, strike, vol, maturity, pz, ctypes.byref(am), ctypes.byref(eu),
+                       pb if betas else None)
+    return (am.value, eu.value, b) if betas else (am.value, eu.value)
 
 
 def lsmc(S0, K, sigma, r, T, steps, paths, seed, stream=0):

@@ -552,3 +552,193 @@ def test_sfmt19937_against_mkl_fixture():
     assert np.array_equal(orc.sfmt19937_bits(777, 2048), d["sfmt777_bits"])
     z = ndtri(orc.sfmt19937_icdf_uniforms(777, 2048))
     assert np.max(np.abs(z - d["sfmt777_icdf_float"])) < 2e-5
+
+
+# ---- Pins of the reference-algorithm quirks -----------------------------------------
+# Each pin_* function raises AssertionError when the oracle disagrees with what
+# the reference's code fixes; tests/test_oracle_mutants.py plants the named
+# mistakes in copies of the oracle and shows that each pin rejects them.
+LSMC_K, LSMC_SPOT, LSMC_VOL, LSMC_T = 110.0, 100.0, .25, 2.0
+
+
+def _lsmc_z_from_spots(S):
+    """Normals that drive the reference's log2-domain path recursion
+    (utilities.cpp:70-83) through the given spots S[date][path], date 0 = spot."""
+    n = S.shape[0] - 1
+    dt = LSMC_T / n
+    return ((np.log(S[1:] / S[:-1]) + .5 * LSMC_VOL ** 2 * dt) / (LSMC_VOL * math.sqrt(dt))).ravel()
+
+
+def _ridge_fit(x, cf):
+    """The sample's regression (main.cpp:236-261): normal equations of the
+    quadratic basis over the given in-the-money points, 0.01 added to each of
+    the five moments, solved here by LU instead of the cofactor inverse."""
+    B = np.stack([np.ones_like(x), x, x * x], axis=1)
+    return np.linalg.solve(B.T @ B + 0.01, B.T @ cf)
+
+
+def pin_lsmc_ref_two_dates():
+    """numSteps = 2: the backward loop (main.cpp:131) visits only s = 1 =
+    numSteps - 1, where every cash flow is the terminal payoff max(K - S_1, 0)
+    scaled by 1/numPaths (:195-199); the final average (:274-278) divides by
+    numPaths again.  So American = sum_p max(K - S_1p, 0) / N^2 exactly and
+    European = sum_p max(K - S_2p, 0) / N.  A dropped 1/N is off by a factor N,
+    a terminal date taken at numSteps uses S_2 (and a second regression)."""
+    n, N = 2, 4096
+    z = np.random.default_rng(17).standard_normal(n * N)
+    am, eu, beta = orc.lsmc_ref(N, n, LSMC_SPOT, LSMC_K, LSMC_VOL, LSMC_T, z, betas=True)
+    dt = LSMC_T / n
+    S1 = LSMC_SPOT * np.exp(-.5 * LSMC_VOL ** 2 * dt + LSMC_VOL * math.sqrt(dt) * z[:N])
+    S2 = S1 * np.exp(-.5 * LSMC_VOL ** 2 * dt + LSMC_VOL * math.sqrt(dt) * z[N:])
+    assert am == pytest.approx(np.sum(np.maximum(LSMC_K - S1, 0)) / N ** 2, rel=1e-12)
+    assert eu == pytest.approx(np.sum(np.maximum(LSMC_K - S2, 0)) / N, rel=1e-12)
+    itm = S1 <= LSMC_K
+    want = _ridge_fit(S1[itm], np.maximum(LSMC_K - S1[itm], 0) / N)
+    assert np.allclose(beta[1], want, rtol=1e-6, atol=0)
+
+
+# Five hand-placed paths over four dates (rows: date 1..4), strike 110.
+LSMC_CASE = np.array([[108., 88., 106., 116., 119.],
+                      [85., 111., 86., 103., 119.],
+                      [103., 80., 81., 87., 97.],
+                      [89., 128., 123., 85., 107.]])
+
+
+def pin_lsmc_ref_lag_and_ridge():
+    """numSteps = 4, the five paths of LSMC_CASE, strike 110.
+
+    Date 3 (the sweep's terminal date): every path is in the money, cash flows
+    (110 - S_3)/5 = 1.4, 6, 5.8, 4.6, 2.6; their ridged fit beta(3) is what
+    date 2 decides with.  Date 2: paths 0, 2, 3 are in the money (85, 86,
+    103); beta(3) gives them continuation values 5.1, 4.9, 1.3 against
+    payoffs 25, 24, 7, so all three exercise.  Their cash flows 110 - x are
+    exactly linear in the spot, so without the ridge the date-2 fit would be
+    (110, -1, 0) exactly and every later decision a rounding tie; the ridge
+    (main.cpp:244-246) on this ill-conditioned 3-point Gram matrix (condition
+    2e10) moves it to (-1.349, 1.384, -0.0127).  Date 1 decides with that
+    lagged fit (main.cpp:176-190 use the betas of :259-261 from the previous
+    iteration): for the in-the-money paths 0 (108), 1 (88), 2 (106) the fit
+    minus the payoff is -1.43, +0.48, -0.79, so paths 0 and 2 exercise
+    (payoffs 2 and 4) and path 1 (out of the money at date 2) keeps its
+    terminal cash flow 6.
+    American = (2 + 6 + 4 + 7 + 2.6)/5 = 4.32; European = (21 + 0 + 0 + 25 +
+    3)/5 = 9.8.  Planted mistakes move it: the date-1 decision with a fit of
+    date 1 itself gives 16.12, no 1/N on the terminal flows 15.6, the terminal
+    at date 4 15.6, and no ridge leaves beta(2) at (110, -1, 0)."""
+    N, n = LSMC_CASE.shape[1], LSMC_CASE.shape[0]
+    S = np.vstack([np.full(N, LSMC_SPOT), LSMC_CASE])
+    am, eu, beta = orc.lsmc_ref(N, n, LSMC_SPOT, LSMC_K, LSMC_VOL, LSMC_T, _lsmc_z_from_spots(S), betas=True)
+    x3 = LSMC_CASE[2]
+    b3 = _ridge_fit(x3, (LSMC_K - x3) / N)
+    assert np.allclose(beta[3], b3, rtol=1e-6, atol=0)
+    x2 = LSMC_CASE[1][[0, 2, 3]]
+    cont = b3[0] + b3[1] * x2 + b3[2] * x2 ** 2
+    assert np.all(LSMC_K - x2 > cont + 5)  # all three exercise at date 2
+    b2 = _ridge_fit(x2, LSMC_K - x2)
+    assert np.allclose(beta[2], b2, rtol=1e-4, atol=0)
+    assert abs(b2[0] - 110) > 50  # the ridge matters here
+    x1 = LSMC_CASE[0][:3]
+    gap = b2[0] + b2[1] * x1 + b2[2] * x1 ** 2 - (LSMC_K - x1)
+    assert gap[0] < -1 and gap[1] > 0.4 and gap[2] < -0.7
+    assert am == pytest.approx((2 + 6 + 4 + 7 + 2.6) / 5, rel=1e-10)
+    assert eu == pytest.approx((21 + 25 + 3) / 5, rel=1e-12)
+
+
+def _libor_linear_value(L0):
+    """Every payer swaption exercised: the spot-measure value is linear in
+    bond prices, (P_40 - P_{41+m} - K d sum_k P_{41+k}) per swaption, with
+    P_j = (1 + d L0)^-j on the flat initial curve."""
+    d = 0.25
+    B = (1 + d * L0) ** -np.arange(1, 41)
+    S = d * np.cumsum(B)
+    mats = [4, 4, 4, 8, 8, 8, 20, 20, 20, 28, 28, 28, 40, 40, 40]
+    rates = np.array([.045, .05, .055] * 5, dtype=np.float32).astype(np.float64)
+    val = sum(100 * max(-(B[m - 1] + K * S[m - 1] - 1), 0.0) for m, K in zip(mats, rates))
+    return val * (1 + d * L0) ** -40
+
+
+def pin_libor_spot_measure_drift():
+    """At L0 = 0.3 every swaption of the book is deep in the money on every
+    path, so the payoff is linear in the time-T_40 bond prices and the
+    discounted value must equal the lambda = 0 closed form: bonds deflated by
+    the spot numeraire are martingales under the spot-measure drift of
+    liborSwaptionGreeks.cpp:70-71.  2^17 paths, seed 3, within 3 SE.  The
+    log-Euler scheme's own bias is +0.07 % (lambda 0.1) and +0.26 % (lambda
+    0.2) of the value, measured at 2^20 paths: about 0.7 and 1.2 SE here.  The
+    drift dropped lands at 4.8 and -30 SE, its sign flipped at -21 and -250
+    SE, the -con1/2 term's sign flipped at -81 and -172 SE."""
+    want = _libor_linear_value(0.3)
+    for lam in (0.1, 0.2):
+        v, se, _, _ = orc.libor(1 << 17, L0=0.3, lam=lam, seed=3)
+        assert abs(v - want) < 3 * se, (lam, v, want, se)
+
+
+def pin_heston_running_val2():
+    """MCAsianOptions.cpp:184-199 with xi = 0, X = 0 and all normals zero:
+    every path is the same deterministic average A, each 64-path chunk adds A
+    to the running val, and val2 += val^2 adds (kA)^2 after chunk k.  After c
+    chunks (N = 64c paths) val = cA and val2 = A^2 c(c+1)(2c+1)/6, so the
+    sample's confidence is e^{-rT} 1.96 A sqrt((N c(c+1)(2c+1)/6 - c^2) /
+    (N(N-1))) / sqrt(N), while the price is e^{-rT} A and a per-path standard
+    error would be 0.  With v0 = theta the variance stays at theta and A =
+    S0 mean_k e^{(r - theta/2) k dt}; with v0 != theta it follows
+    v' = v + kappa dt (theta - v)."""
+    n_steps, c, r = 6, 5, .03
+    N = 64 * c
+    S, T = np.array([100.0, 90.0]), np.array([1.0, 0.5])
+    v0, kap, th = np.array([.04, .09]), np.array([2.0, 3.0]), np.array([.04, .05])
+    z = np.zeros((2, c, n_steps, 64))
+    res, conf = orc.heston_buffer(S, np.zeros(2), T, v0, kap, th, np.zeros(2), r, -.7, N, n_steps, z, z,
+                                  quirks=orc.Q_RUNNING_VAL2)
+    for o in range(2):
+        dt = T[o] / n_steps
+        v, s, avg = v0[o], S[o], 0.0
+        for _ in range(n_steps):
+            s *= math.exp((r - .5 * v) * dt)
+            v += kap[o] * dt * (th[o] - v)
+            avg += s
+        A = avg / n_steps
+        if o == 0:
+            assert A == pytest.approx(S[0] * np.mean(np.exp((r - th[0] / 2) * dt * np.arange(1, n_steps + 1))),
+                                      rel=1e-14)
+        disc = math.exp(-r * T[o])
+        want = disc * 1.96 * A * math.sqrt((N * c * (c + 1) * (2 * c + 1) / 6 - c * c) / (N * (N - 1))) / math.sqrt(N)
+        assert res[o] == pytest.approx(disc * A, rel=1e-13)
+        assert conf[o] == pytest.approx(want, rel=1e-12)
+
+
+def pin_geometric_asian_discrete():
+    """The discretely monitored geometric-average call (t_k = kT/N, k = 1..N)
+    against an exact-GBM Monte Carlo of 2^20 paths, within 4 SE, for N = 4
+    and 12: log G is normal with mean log S + (r - sigma^2/2) T (N+1)/(2N)
+    and variance sigma^2 T (N+1)(2N+1)/(6N^2).  The continuous-average
+    variance sigma^2 T/3 misses by 25 SE (N = 4), the continuous mean by 60."""
+    S, K, T, r, sig = 100.0, 100.0, 2.0, .05, .4
+    rng = np.random.default_rng(23)
+    for N in (4, 12):
+        dt = T / N
+        logS = np.cumsum((r - .5 * sig * sig) * dt + sig * math.sqrt(dt) * rng.standard_normal((1 << 20, N)),
+                         axis=1)
+        pay = math.exp(-r * T) * np.maximum(S * np.exp(logS.mean(axis=1)) - K, 0)
+        se = pay.std() / math.sqrt(pay.size)
+        assert abs(orc.geometric_asian_call(S, K, T, r, sig, N) - pay.mean()) < 4 * se
+
+
+def test_lsmc_ref_two_dates_closed_form():
+    pin_lsmc_ref_two_dates()
+
+
+def test_lsmc_ref_lag_and_ridge():
+    pin_lsmc_ref_lag_and_ridge()
+
+
+def test_libor_spot_measure_drift():
+    pin_libor_spot_measure_drift()
+
+
+def test_heston_running_val2_confidence():
+    pin_heston_running_val2()
+
+
+def test_geometric_asian_discrete_monitoring():
+    pin_geometric_asian_discrete()
