@@ -33,8 +33,9 @@ _VP = ctypes.c_void_p
 
 
 class FsmcError(RuntimeError):
-    def __init__(self, status, where):
-        detail = _lib.fsmc_last_error().decode() if _lib is not None else ""
+    def __init__(self, status, where, detail=None):
+        if detail is None:
+            detail = _lib.fsmc_last_error().decode() if _lib is not None else ""
         name = _lib.fsmc_status_string(status).decode() if _lib is not None else str(status)
         super().__init__(f"{where}: {name} ({status}) {detail}")
         self.status = status
@@ -375,6 +376,7 @@ class MT19937:
         return out
 
     def gaussian_async(self, n, out, stream=None, mean=0.0, stddev=1.0):
+        _need(out, "float64", n, "out", "fsmc_mt19937_gaussian_async")
         _check(lib().fsmc_mt19937_gaussian_async(self._g, n, mean, stddev, out.data_ptr(),
                                                  stream if stream is not None else _stream()),
                "fsmc_mt19937_gaussian_async")
@@ -500,6 +502,14 @@ def mixed_requests(n, seed=4096):
 
 
 # ---- asynchronous entry points on torch device tensors ------------------------
+# The C entry points take raw device addresses and cannot see a tensor's
+# dtype, layout or size, so every tensor is checked here before any launch:
+# on the current CUDA device, of the stated dtype, contiguous, and holding at
+# least the elements the kernels read or write.  A failed check raises
+# FsmcError(FSMC_ERR_INVALID_ARG) and nothing is enqueued.
+ERR_INVALID_ARG = -1
+
+
 def _ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
@@ -507,6 +517,35 @@ def _ptr(t):
 def _stream():
     import torch
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _need(t, dtype, n, name, where):
+    """Validate device tensor t (dtype given as a torch dtype name) for n
+    elements; None passes (optional arguments are checked by the C side)."""
+    if t is None:
+        return
+    import torch
+    want = getattr(torch, dtype)
+
+    def bad(msg):
+        raise FsmcError(ERR_INVALID_ARG, where, f"{name} {msg}")
+    if not isinstance(t, torch.Tensor):
+        bad(f"is {type(t).__name__}, not a torch.Tensor")
+    if t.dtype != want:
+        bad(f"has dtype {t.dtype}, expected {want}")
+    if not t.is_contiguous():
+        bad("is not contiguous")
+    if t.numel() < n:
+        bad(f"has {t.numel()} elements, needs {n}")
+    if not t.is_cuda:
+        bad("is not a CUDA (HIP) device tensor")
+    if t.device.index != torch.cuda.current_device():
+        bad(f"is on cuda:{t.device.index}, the current device is cuda:{torch.cuda.current_device()}")
+
+
+def _workspace(ws, need, where):
+    _need(ws, "uint8", need, "workspace", where)
+    return ws.numel() * ws.element_size()
 
 
 def gbm_european_workspace(n_options, n_paths, r, sigma, m, control_variate=False):
@@ -517,17 +556,30 @@ def gbm_european_workspace(n_options, n_paths, r, sigma, m, control_variate=Fals
 def sfmt19937_gaussian_async(seed0, n_streams, n_per_stream, out, seg=1, mean=None, stddev=None):
     """Streams seed0 .. seed0 + n_streams - 1 into the float32 tensor out
     ([n_streams * n_per_stream]); mean / stddev: float64 tensors per segment."""
+    where = "fsmc_sfmt19937_gaussian_async"
+    total = n_streams * n_per_stream
+    _need(out, "float32", total, "out", where)
+    nseg = (total + seg - 1) // seg if seg > 0 else 0
+    _need(mean, "float64", nseg, "mean", where)
+    _need(stddev, "float64", nseg, "stddev", where)
     _check(lib().fsmc_sfmt19937_gaussian_async(seed0, n_streams, n_per_stream, seg, _ptr(mean), _ptr(stddev),
-                                               _ptr(out), _stream()), "fsmc_sfmt19937_gaussian_async")
+                                               _ptr(out), _stream()), where)
 
 
 def gbm_european_async(S, X, T, r, sigma, n_paths, m, out_res, out_conf, workspace, samples=None,
                        out_greeks=None, control_variate=False):
-    p = GbmParams(S.numel(), n_paths, r, sigma, 1 if control_variate else 0)
+    where = "fsmc_gbm_european_async"
+    _need(S, "float64", 1, "S", where)
+    n = S.numel()
+    for name, t in (("X", X), ("T", T), ("out_res", out_res), ("out_conf", out_conf)):
+        _need(t, "float64", n, name, where)
+    _need(out_greeks, "float64", 6 * n, "out_greeks", where)
+    _need(samples, "float32", n * n_paths, "samples", where)
+    p = GbmParams(n, n_paths, r, sigma, 1 if control_variate else 0)
+    ws_bytes = _workspace(workspace, lib().fsmc_gbm_european_workspace(ctypes.byref(p), ctypes.byref(m)), where)
     _check(lib().fsmc_gbm_european_async(ctypes.byref(p), ctypes.byref(m), _ptr(S), _ptr(X), _ptr(T),
                                          _ptr(samples), _ptr(out_res), _ptr(out_conf), _ptr(out_greeks),
-                                         _ptr(workspace), workspace.numel() * workspace.element_size(),
-                                         _stream()), "fsmc_gbm_european_async")
+                                         _ptr(workspace), ws_bytes, _stream()), where)
 
 
 def heston_asian_workspace(n_options, n_paths, n_steps, r, rho, m):
@@ -535,14 +587,31 @@ def heston_asian_workspace(n_options, n_paths, n_steps, r, rho, m):
     return lib().fsmc_heston_asian_workspace(ctypes.byref(p), ctypes.byref(m))
 
 
+def _heston_check(params, n_paths, n_steps, out_res, out_conf, z1, z2, out_greeks, where):
+    if len(params) != 7:
+        raise FsmcError(ERR_INVALID_ARG, where, "params must be the 7 tensors S, X, T, v0, kappa, theta, xi")
+    _need(params[0], "float64", 1, "S", where)
+    n = params[0].numel()
+    for name, t in zip(("S", "X", "T", "v0", "kappa", "theta", "xi"), params):
+        _need(t, "float64", n, name, where)
+    _need(out_res, "float64", n, "out_res", where)
+    _need(out_conf, "float64", n, "out_conf", where)
+    _need(out_greeks, "float64", 6 * n, "out_greeks", where)
+    _need(z1, "float64", n * n_paths * n_steps, "z1", where)
+    _need(z2, "float64", n * n_paths * n_steps, "z2", where)
+    return n
+
+
 def heston_asian_async(params, n_paths, n_steps, r, rho, m, out_res, out_conf, workspace, z1=None, z2=None,
                        out_greeks=None):
     """params: sequence of 7 device tensors (S, X, T, v0, kappa, theta, xi)."""
-    p = HestonParams(params[0].numel(), n_paths, n_steps, r, rho)
+    where = "fsmc_heston_asian_async"
+    n = _heston_check(params, n_paths, n_steps, out_res, out_conf, z1, z2, out_greeks, where)
+    p = HestonParams(n, n_paths, n_steps, r, rho)
+    ws_bytes = _workspace(workspace, lib().fsmc_heston_asian_workspace(ctypes.byref(p), ctypes.byref(m)), where)
     _check(lib().fsmc_heston_asian_async(ctypes.byref(p), ctypes.byref(m), *[_ptr(t) for t in params],
                                          _ptr(z1), _ptr(z2), _ptr(out_res), _ptr(out_conf), _ptr(out_greeks),
-                                         _ptr(workspace), workspace.numel() * workspace.element_size(),
-                                         _stream()), "fsmc_heston_asian_async")
+                                         _ptr(workspace), ws_bytes, _stream()), where)
 
 
 def libor_workspace(n_paths, m, L0=0.05, lam=0.2):
@@ -552,8 +621,11 @@ def libor_workspace(n_paths, m, L0=0.05, lam=0.2):
 
 def libor_async(n_paths, m, out_value, out_deltas, workspace, L0=0.05, lam=0.2, z=None):
     """out_value: 2 doubles, out_deltas: 160 doubles (device tensors)."""
+    where = "fsmc_libor_swaption_greeks_async"
+    _need(out_value, "float64", 2, "out_value", where)
+    _need(out_deltas, "float64", 160, "out_deltas", where)
+    _need(z, "float64", n_paths * 40, "z", where)
     p = LiborParams(n_paths, L0, lam)
+    ws_bytes = _workspace(workspace, lib().fsmc_libor_workspace(ctypes.byref(p), ctypes.byref(m)), where)
     _check(lib().fsmc_libor_swaption_greeks_async(ctypes.byref(p), ctypes.byref(m), _ptr(z), _ptr(out_value),
-                                                  _ptr(out_deltas), _ptr(workspace),
-                                                  workspace.numel() * workspace.element_size(), _stream()),
-           "fsmc_libor_swaption_greeks_async")
+                                                  _ptr(out_deltas), _ptr(workspace), ws_bytes, _stream()), where)

@@ -113,3 +113,46 @@ def test_batch_tuning_validated_without_device(lib):
     t = mcengine.BatchTuning(4096, 0, 1, 2)
     assert ctypes.sizeof(t) == 16 and (t.chunk_paths, t.requests_per_launch, t.replay_graph,
                                         t.workgroups_per_cu) == (4096, 0, 1, 2)
+
+
+# ---- argument checks of the torch-tensor wrappers (before any launch) --------
+def _cpu(n, dtype="float64"):
+    import torch
+    return torch.zeros(n, dtype=getattr(torch, dtype))
+
+
+def test_async_wrappers_check_tensors_before_launch(lib):
+    import torch
+    m = mcengine.mode()
+    prm = [_cpu(4) for _ in range(7)]
+    cases = [
+        # wrong dtype of an output
+        lambda: mcengine.heston_asian_async(prm, 1024, 8, .03, -.7, m, _cpu(4, "float32"), _cpu(4), _cpu(1 << 20,
+                                                                                                        "uint8")),
+        # Greeks output too small ([n][6] needed)
+        lambda: mcengine.heston_asian_async(prm, 1024, 8, .03, -.7, m, _cpu(4), _cpu(4), _cpu(1 << 20, "uint8"),
+                                            out_greeks=_cpu(23)),
+        # a non-contiguous input
+        lambda: mcengine.heston_asian_async([torch.zeros(8, dtype=torch.float64)[::2]] + prm[1:], 1024, 8, .03, -.7,
+                                            m, _cpu(4), _cpu(4), _cpu(1 << 20, "uint8")),
+        # six parameter tensors instead of seven
+        lambda: mcengine.heston_asian_async(prm[:6], 1024, 8, .03, -.7, m, _cpu(4), _cpu(4), _cpu(1 << 20, "uint8")),
+        # buffer normals shorter than options x paths x steps
+        lambda: mcengine.heston_asian_async(prm, 64, 8, .03, -.7, mcengine.mode(rng=mcengine.RNG_BUFFER),
+                                            _cpu(4), _cpu(4), _cpu(1 << 20, "uint8"), z1=_cpu(4 * 64 * 8 - 1),
+                                            z2=_cpu(4 * 64 * 8)),
+        # a host tensor that passes every other check
+        lambda: mcengine.heston_asian_async(prm, 1024, 8, .03, -.7, m, _cpu(4), _cpu(4), _cpu(1 << 20, "uint8")),
+        lambda: mcengine.gbm_european_async(prm[0], prm[1], prm[2], .02, .3, 1024, m, _cpu(4), _cpu(3),
+                                            _cpu(1 << 20, "uint8")),
+        lambda: mcengine.gbm_european_async(prm[0], prm[1], prm[2], .02, .3, 1024, m, _cpu(4), _cpu(4),
+                                            _cpu(1 << 20, "uint8"), out_greeks=_cpu(24, "float32")),
+        lambda: mcengine.libor_async(1024, m, _cpu(2), _cpu(159), _cpu(1 << 20, "uint8")),
+        lambda: mcengine.libor_async(1024, m, _cpu(2), _cpu(160), _cpu(16, "uint8")),
+        lambda: mcengine.sfmt19937_gaussian_async(5, 3, 100, _cpu(299, "float32")),
+        lambda: mcengine.sfmt19937_gaussian_async(5, 3, 100, _cpu(300, "float32"), 100, _cpu(3), _cpu(2)),
+    ]
+    for i, call in enumerate(cases):
+        with pytest.raises(mcengine.FsmcError) as e:
+            call()
+        assert e.value.status == -1, (i, str(e.value))
