@@ -226,24 +226,35 @@ size_t fsmc_heston_asian_workspace(const fsmc_heston_params* p, const fsmc_mode*
   return greeks_possible && h.gws > h.ws ? h.gws : h.ws;
 }
 
-int fsmc_heston_asian_async(const fsmc_heston_params* p, const fsmc_mode* m, const double* d_S,
-                            const double* d_X, const double* d_T, const double* d_v0, const double* d_kappa,
-                            const double* d_theta, const double* d_xi, const double* d_z1, const double* d_z2,
-                            double* d_result, double* d_conf, double* d_greeks, void* d_ws, size_t ws_bytes,
-                            void* stream) {
+}  // extern "C"
+
+namespace fsmc {
+namespace {
+
+int heston_check_args(const fsmc_heston_params* p, const fsmc_mode* m, const double* d_S, const double* d_X,
+                      const double* d_T, const double* d_v0, const double* d_kappa, const double* d_theta,
+                      const double* d_xi, const double* d_z1, const double* d_z2, const double* d_result,
+                      const double* d_conf, const double* d_greeks) {
   FSMC_TRY(heston_validate(p, m));
   FSMC_REQUIRE(d_S && d_X && d_T && d_v0 && d_kappa && d_theta && d_xi && d_result && d_conf,
                "NULL device pointer");
   const bool buffer = m->rng == FSMC_RNG_BUFFER;
   FSMC_REQUIRE(!buffer || (d_z1 && d_z2), "buffer mode needs z1 and z2");
   FSMC_REQUIRE(!d_greeks || (!buffer && m->quirks == 0), "Greeks need Philox mode without quirks");
+  return FSMC_OK;
+}
+
+// Enqueues the path (or adjoint) kernel and its finalize on stream s with the
+// workspace d_ws; the arguments have passed heston_check_args and d_ws holds
+// at least the plan's forward (ws) or adjoint (gws) bytes.  Every workspace
+// word the kernels read is written earlier in the same launch sequence, so
+// the workspace needs no clearing.
+int heston_enqueue(const fsmc_heston_params* p, const fsmc_mode* m, const double* d_S, const double* d_X,
+                   const double* d_T, const double* d_v0, const double* d_kappa, const double* d_theta,
+                   const double* d_xi, const double* d_z1, const double* d_z2, double* d_result, double* d_conf,
+                   double* d_greeks, void* d_ws, hipStream_t s) {
+  const bool buffer = m->rng == FSMC_RNG_BUFFER;
   const HestonPlan h = heston_plan(p, m);
-  const size_t need = fsmc_heston_asian_workspace(p, m);
-  if (d_ws == nullptr || ws_bytes < need) {
-    set_error("workspace %zu bytes < required %zu", ws_bytes, need);
-    return FSMC_ERR_WORKSPACE;
-  }
-  hipStream_t s = (hipStream_t)stream;
   HestonArgs a;
   a.n_paths = p->n_paths;
   a.n_steps = p->n_steps;
@@ -292,6 +303,82 @@ int fsmc_heston_asian_async(const fsmc_heston_params* p, const fsmc_mode* m, con
                      d_T, a.partials, a.chunks, ch ? 1 : 0, d_result, d_conf);
   FSMC_HIP(hipGetLastError());
   return FSMC_OK;
+}
+
+// Workspace of fsmc_heston_asian_noalloc: a fixed device array in this
+// library's code object.  Each GPU holds its own copy until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress
+// returns on that device (the array's name is a host-side stand-in).
+constexpr size_t kNoallocScratchBytes = size_t(16) << 20;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+
+}  // namespace
+}  // namespace fsmc
+
+static __device__ __align__(256) unsigned char scratch[fsmc::kNoallocScratchBytes];
+
+namespace fsmc {
+namespace {
+
+// The scratch array's address on the current device, looked up once per
+// device (with that device current) and kept.
+int scratch_on_current_device(void** out) {
+  int dev = -1;
+  FSMC_HIP(hipGetDevice(&dev));
+  FSMC_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d outside [0, %d)", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    FSMC_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    FSMC_REQUIRE(ptr != nullptr, "hipGetSymbolAddress returned NULL for the scratch array on device %d", dev);
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return FSMC_OK;
+}
+
+}  // namespace
+}  // namespace fsmc
+
+extern "C" {
+
+int fsmc_heston_asian_async(const fsmc_heston_params* p, const fsmc_mode* m, const double* d_S,
+                            const double* d_X, const double* d_T, const double* d_v0, const double* d_kappa,
+                            const double* d_theta, const double* d_xi, const double* d_z1, const double* d_z2,
+                            double* d_result, double* d_conf, double* d_greeks, void* d_ws, size_t ws_bytes,
+                            void* stream) {
+  FSMC_TRY(heston_check_args(p, m, d_S, d_X, d_T, d_v0, d_kappa, d_theta, d_xi, d_z1, d_z2, d_result, d_conf,
+                             d_greeks));
+  const size_t need = fsmc_heston_asian_workspace(p, m);
+  if (d_ws == nullptr || ws_bytes < need) {
+    set_error("workspace %zu bytes < required %zu", ws_bytes, need);
+    return FSMC_ERR_WORKSPACE;
+  }
+  return heston_enqueue(p, m, d_S, d_X, d_T, d_v0, d_kappa, d_theta, d_xi, d_z1, d_z2, d_result, d_conf, d_greeks,
+                        d_ws, (hipStream_t)stream);
+}
+
+size_t fsmc_heston_asian_noalloc_limit(void) { return kNoallocScratchBytes; }
+
+int fsmc_heston_asian_noalloc(const fsmc_heston_params* p, const fsmc_mode* m, const double* d_S,
+                              const double* d_X, const double* d_T, const double* d_v0, const double* d_kappa,
+                              const double* d_theta, const double* d_xi, const double* d_z1, const double* d_z2,
+                              double* d_result, double* d_conf, double* d_greeks, void* stream) {
+  FSMC_TRY(heston_check_args(p, m, d_S, d_X, d_T, d_v0, d_kappa, d_theta, d_xi, d_z1, d_z2, d_result, d_conf,
+                             d_greeks));
+  const HestonPlan h = heston_plan(p, m);
+  const size_t need = d_greeks ? h.gws : h.ws;
+  if (need > kNoallocScratchBytes) {
+    set_error("this pricing needs %zu bytes of workspace; the allocation-free entry point has %zu", need,
+              kNoallocScratchBytes);
+    return FSMC_ERR_WORKSPACE;
+  }
+  void* ws = nullptr;
+  FSMC_TRY(scratch_on_current_device(&ws));
+  return heston_enqueue(p, m, d_S, d_X, d_T, d_v0, d_kappa, d_theta, d_xi, d_z1, d_z2, d_result, d_conf, d_greeks,
+                        ws, (hipStream_t)stream);
 }
 
 int fsmc_heston_asian(fsmc_ctx* ctx, const fsmc_heston_params* p, const fsmc_mode* m, const double* S,

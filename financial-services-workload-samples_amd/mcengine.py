@@ -119,6 +119,9 @@ def lib():
         L.fsmc_heston_asian_workspace.argtypes = [ctypes.POINTER(HestonParams), PM]
         L.fsmc_heston_asian_async.argtypes = [ctypes.POINTER(HestonParams), PM] + [_VP] * 12 + [
             _VP, ctypes.c_size_t, _VP]
+        L.fsmc_heston_asian_noalloc_limit.restype = ctypes.c_size_t
+        L.fsmc_heston_asian_noalloc_limit.argtypes = []
+        L.fsmc_heston_asian_noalloc.argtypes = [ctypes.POINTER(HestonParams), PM] + [_VP] * 12 + [_VP]
         PL = ctypes.POINTER(LiborParams)
         L.fsmc_libor_swaption_greeks.argtypes = [_VP, PL, PM, _D, _D, _D]
         L.fsmc_libor_workspace.restype = ctypes.c_size_t
@@ -612,6 +615,23 @@ def heston_asian_async(params, n_paths, n_steps, r, rho, m, out_res, out_conf, w
     _check(lib().fsmc_heston_asian_async(ctypes.byref(p), ctypes.byref(m), *[_ptr(t) for t in params],
                                          _ptr(z1), _ptr(z2), _ptr(out_res), _ptr(out_conf), _ptr(out_greeks),
                                          _ptr(workspace), ws_bytes, _stream()), where)
+
+
+def heston_asian_noalloc_limit():
+    """Bytes of the fixed device workspace of the allocation-free entry point."""
+    return lib().fsmc_heston_asian_noalloc_limit()
+
+
+def heston_asian_noalloc(params, n_paths, n_steps, r, rho, m, out_res, out_conf, z1=None, z2=None,
+                         out_greeks=None):
+    """fsmc_heston_asian_noalloc: as heston_asian_async, with the library's
+    fixed device workspace instead of a caller one (nothing is allocated)."""
+    where = "fsmc_heston_asian_noalloc"
+    n = _heston_check(params, n_paths, n_steps, out_res, out_conf, z1, z2, out_greeks, where)
+    p = HestonParams(n, n_paths, n_steps, r, rho)
+    _check(lib().fsmc_heston_asian_noalloc(ctypes.byref(p), ctypes.byref(m), *[_ptr(t) for t in params],
+                                           _ptr(z1), _ptr(z2), _ptr(out_res), _ptr(out_conf), _ptr(out_greeks),
+                                           _stream()), where)
 
 
 def libor_workspace(n_paths, m, L0=0.05, lam=0.2):

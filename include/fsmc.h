@@ -142,6 +142,23 @@ int fsmc_heston_asian_async(const fsmc_heston_params* p, const fsmc_mode* m, con
                             const double* d_z1, const double* d_z2, double* d_call_result,
                             double* d_call_confidence, double* d_greeks, void* d_workspace,
                             size_t workspace_bytes, void* stream);
+/* Allocation-free variant of fsmc_heston_asian_async.  It has no counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels run with their workspace in a fixed device
+ * array of fsmc_heston_asian_noalloc_limit() bytes (16 MiB) that the library
+ * holds on each GPU it runs on until the process ends; the call allocates
+ * nothing and rejects, with FSMC_ERR_WORKSPACE, a pricing whose workspace
+ * (fsmc_heston_asian_workspace without Greeks; with Greeks the adjoint's
+ * checkpoints, 16 B per path and 16 steps per option) exceeds it.  The
+ * stream must belong to the current device.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (two streams, two threads) race
+ * and must be serialised by the caller. */
+size_t fsmc_heston_asian_noalloc_limit(void);
+int fsmc_heston_asian_noalloc(const fsmc_heston_params* p, const fsmc_mode* m, const double* d_S,
+                              const double* d_X, const double* d_T, const double* d_v0, const double* d_kappa,
+                              const double* d_theta, const double* d_xi, const double* d_z1, const double* d_z2,
+                              double* d_call_result, double* d_call_confidence, double* d_greeks, void* stream);
 
 /* ---- LIBOR market model swaption portfolio + pathwise adjoint deltas -------
  * Replaces the per-path body of liborSwaptionGreeks, liborSwaptionGreeks.cpp:

@@ -136,13 +136,12 @@ def test_async_wrappers_check_tensors_before_launch(lib):
         lambda: mcengine.heston_asian_async([torch.zeros(8, dtype=torch.float64)[::2]] + prm[1:], 1024, 8, .03, -.7,
                                             m, _cpu(4), _cpu(4), _cpu(1 << 20, "uint8")),
         # six parameter tensors instead of seven
-        lambda: mcengine.heston_asian_async(prm[:6], 1024, 8, .03, -.7, m, _cpu(4), _cpu(4), _cpu(1 << 20, "uint8")),
+        lambda: mcengine.heston_asian_noalloc(prm[:6], 1024, 8, .03, -.7, m, _cpu(4), _cpu(4)),
         # buffer normals shorter than options x paths x steps
-        lambda: mcengine.heston_asian_async(prm, 64, 8, .03, -.7, mcengine.mode(rng=mcengine.RNG_BUFFER),
-                                            _cpu(4), _cpu(4), _cpu(1 << 20, "uint8"), z1=_cpu(4 * 64 * 8 - 1),
-                                            z2=_cpu(4 * 64 * 8)),
+        lambda: mcengine.heston_asian_noalloc(prm, 64, 8, .03, -.7, mcengine.mode(rng=mcengine.RNG_BUFFER),
+                                              _cpu(4), _cpu(4), z1=_cpu(4 * 64 * 8 - 1), z2=_cpu(4 * 64 * 8)),
         # a host tensor that passes every other check
-        lambda: mcengine.heston_asian_async(prm, 1024, 8, .03, -.7, m, _cpu(4), _cpu(4), _cpu(1 << 20, "uint8")),
+        lambda: mcengine.heston_asian_noalloc(prm, 1024, 8, .03, -.7, m, _cpu(4), _cpu(4)),
         lambda: mcengine.gbm_european_async(prm[0], prm[1], prm[2], .02, .3, 1024, m, _cpu(4), _cpu(3),
                                             _cpu(1 << 20, "uint8")),
         lambda: mcengine.gbm_european_async(prm[0], prm[1], prm[2], .02, .3, 1024, m, _cpu(4), _cpu(4),
@@ -156,3 +155,20 @@ def test_async_wrappers_check_tensors_before_launch(lib):
         with pytest.raises(mcengine.FsmcError) as e:
             call()
         assert e.value.status == -1, (i, str(e.value))
+
+
+def test_noalloc_limit_and_size_rejection(lib):
+    # 16 MiB; the rejection of a larger pricing happens before any device call
+    assert lib.fsmc_heston_asian_noalloc_limit() == 16 << 20
+    h = mcengine.HestonParams(64, 131072, 365, .03, -.7)
+    m = mcengine.mode(mcengine.FP32)
+    fake = ctypes.c_void_p(16)  # never dereferenced: the size check comes first
+    args = [fake] * 7 + [None, None, fake, fake]
+    # the forward pricing of 64 x 131072 fits (1.5 MB of partials and chunk sums)
+    assert lib.fsmc_heston_asian_workspace(ctypes.byref(h), ctypes.byref(m)) > (16 << 20)  # sized for Greeks
+    # with Greeks its checkpoints (1.5 GB) do not: rejected with FSMC_ERR_WORKSPACE
+    st = lib.fsmc_heston_asian_noalloc(ctypes.byref(h), ctypes.byref(m), *args, fake, None)
+    assert st == -4 and b"16777216" in lib.fsmc_last_error()
+    # invalid parameters are rejected as such
+    bad = mcengine.HestonParams(0, 131072, 365, .03, -.7)
+    assert lib.fsmc_heston_asian_noalloc(ctypes.byref(bad), ctypes.byref(m), *args, None, None) == -1
