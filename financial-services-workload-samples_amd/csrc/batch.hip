@@ -203,6 +203,7 @@ struct BatchImpl {
   std::vector<hipEvent_t> done;  // one per micro-batch (stream mode)
   hipEvent_t start = nullptr;
   hipEvent_t t0 = nullptr, t1 = nullptr;  // autotuner's timing around a run
+  hipEvent_t ran = nullptr;  // recorded by fsmc_batch_run after the last launch
   double wall_khz = 0.0;
   hipStream_t capture = nullptr;  // private stream the graph is captured on
   hipGraph_t graph = nullptr;
@@ -504,6 +505,7 @@ int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_
                  hipDeviceGetAttribute(&bi.cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess &&
                  bi.cus > 0 && hipEventCreate(&bi.start) == hipSuccess &&
                  hipEventCreate(&bi.t0) == hipSuccess && hipEventCreate(&bi.t1) == hipSuccess &&
+                 hipEventCreateWithFlags(&bi.ran, hipEventDisableTiming) == hipSuccess &&
                  hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) == hipSuccess && khz > 0
              ? FSMC_OK
              : FSMC_ERR_DEVICE;
@@ -525,10 +527,13 @@ int fsmc_batch_run(fsmc_batch* b, void* stream) {
   hipStream_t s = stream ? (hipStream_t)stream : b->ctx->stream;
   if (b->impl.exec) {
     FSMC_HIP(hipGraphLaunch(b->impl.exec, s));
-    return FSMC_OK;
+  } else {
+    FSMC_TRY(b->impl.mode.precision == FSMC_FP64 ? batch_enqueue<double>(b->impl, s, false)
+                                                 : batch_enqueue<float>(b->impl, s, false));
   }
-  return b->impl.mode.precision == FSMC_FP64 ? batch_enqueue<double>(b->impl, s, false)
-                                             : batch_enqueue<float>(b->impl, s, false);
+  // fsmc_batch_results may be called with another stream: it waits on this.
+  FSMC_HIP(hipEventRecord(b->impl.ran, s));
+  return FSMC_OK;
 }
 
 int fsmc_batch_results(fsmc_batch* b, void* stream, fsmc_result* out) {
@@ -537,6 +542,7 @@ int fsmc_batch_results(fsmc_batch* b, void* stream, fsmc_result* out) {
   hipStream_t s = stream ? (hipStream_t)stream : b->ctx->stream;
   std::vector<double> res((size_t)bi.n * 8);
   FSMC_HIP(hipSetDevice(b->ctx->device));
+  FSMC_HIP(hipStreamWaitEvent(s, bi.ran, 0));  // the last run, whatever stream it was on
   FSMC_HIP(hipMemcpyAsync(res.data(), (char*)bi.ws.ptr + bi.off_res, res.size() * sizeof(double),
                           hipMemcpyDeviceToHost, s));
   std::vector<unsigned long long> stamp(bi.exec ? bi.waves.size() + 1 : 0);
@@ -663,6 +669,7 @@ int fsmc_batch_destroy(fsmc_batch* b) {
   if (b->impl.start) (void)hipEventDestroy(b->impl.start);
   if (b->impl.t0) (void)hipEventDestroy(b->impl.t0);
   if (b->impl.t1) (void)hipEventDestroy(b->impl.t1);
+  if (b->impl.ran) (void)hipEventDestroy(b->impl.ran);
   if (b->impl.capture) (void)hipStreamDestroy(b->impl.capture);
   b->impl.ws.release();
   delete b;

@@ -15,8 +15,11 @@ void set_error(const char* fmt, ...) {
 int DeviceBuffer::reserve(size_t want) {
   if (want <= bytes) return FSMC_OK;
   if (ptr) {
-    // The previous contents are dead; every user of this slot synchronises
-    // its stream before returning, so the old block is idle.
+    // The previous contents are dead, but work enqueued earlier on any
+    // stream may still use the block (the stream-ordered generator entry
+    // points grow their slots without synchronising): wait for the device
+    // before freeing it.
+    (void)hipDeviceSynchronize();
     (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;

@@ -375,6 +375,8 @@ typedef struct fsmc_batch fsmc_batch;
 int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_mode* m,
                       const fsmc_batch_tuning* tuning, fsmc_batch** out);
 int fsmc_batch_run(fsmc_batch* b, void* stream);
+/* Makes its stream wait for the last fsmc_batch_run (whichever stream that
+ * run was enqueued on), copies the results and synchronises its stream. */
 int fsmc_batch_results(fsmc_batch* b, void* stream, fsmc_result* out);
 /* Times the batch over chunk sizes x micro-batch sizes (grouped grids), then
  * the winner as a persistent grid of 1 to 4 workgroups per CU, then the

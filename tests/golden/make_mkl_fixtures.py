@@ -10,7 +10,11 @@ ctypes).  Run by hand where that library exists; the tests read only the
   MT19937 seeds 42 and 43 (AsianOptionsPricing/mkl_rng.h:19,60): raw 32-bit
   outputs and BoxMuller2 normals;
   SFMT19937 seed 777 (MonteCarloInsideBlockingDP.cpp:229,249): raw 32-bit
-  outputs and single-precision ICDF normals.
+  outputs and single-precision ICDF normals, and three consecutive blocks of
+  2048 drawn as the sample draws them, vsRngGaussian(ICDF, ..., MuByT,
+  VBySqrtT) with its per-option log2-domain mean and deviation (sample
+  constants r = 0.06, sigma = 0.10, :71-72,160-163,238-240, at
+  T = 1, 3.7, 5).
 """
 import ctypes
 import os
@@ -57,6 +61,22 @@ def main():
         L.vslDeleteStream(ctypes.byref(s))
         return out
 
+    def sample_blocks(seed, mean, sd, n=N):
+        """Consecutive blocks of one stream, block k with (mean[k], sd[k])
+        passed as the sample passes them (double arguments of a float call)."""
+        s = stream(SFMT19937, seed)
+        out = np.zeros((len(mean), n), dtype=np.float32)
+        for k in range(len(mean)):
+            assert L.vsRngGaussian(ICDF, s, n, out[k].ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                   ctypes.c_float(mean[k]), ctypes.c_float(sd[k])) == 0
+        L.vslDeleteStream(ctypes.byref(s))
+        return out
+
+    log2e, r, sig = 1.4426950408889634074, 0.06, 0.10
+    T = np.array([1.0, 3.7, 5.0])
+    mu_by_t = log2e * (r - 0.5 * sig * sig) * T
+    v_by_sqrt_t = log2e * sig * np.sqrt(T)
+
     np.savez_compressed(
         os.path.join(HERE, "mkl_streams.npz"),
         mkl_version=np.frombuffer(ver.value, dtype=np.uint8),
@@ -69,6 +89,9 @@ def main():
         mt43_boxmuller2=draw(MT19937, 43, "boxmuller2"),
         sfmt777_bits=draw(SFMT19937, 777, "bits"),
         sfmt777_icdf_float=draw(SFMT19937, 777, "icdf_float"),
+        sfmt777_sample_mean=mu_by_t,
+        sfmt777_sample_sd=v_by_sqrt_t,
+        sfmt777_sample_blocks=sample_blocks(777, mu_by_t, v_by_sqrt_t),
     )
 
 

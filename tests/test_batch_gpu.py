@@ -5,6 +5,8 @@ the standalone entry point with the request index as option index, so FP64
 batch results equal the standalone GPU results (and therefore the oracle) to
 summation order; reruns, with eager work in between, reproduce the first run
 bitwise."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -89,6 +91,23 @@ def test_mixed_batch_reruns_and_tuning(engine):
     tuned = b.results()
     b.close()
     assert np.allclose(tuned["price"], first["price"], rtol=1e-9, atol=1e-12)
+
+
+def test_results_wait_for_a_run_on_another_stream(engine):
+    # run on a torch side stream, read results on the context's own stream:
+    # fsmc_batch_results waits for the run's completion event first
+    import torch
+    reqs = mc.mixed_requests(64, seed=8)
+    b = mc.Batch(engine, reqs, mc.mode(mc.FP32, seed=4), tuning=(4096, 0))
+    b.run()
+    ref = b.results()
+    side = torch.cuda.Stream()
+    for _ in range(3):
+        b.run(ctypes.c_void_p(side.cuda_stream))
+        got = b.results()  # NULL stream = the context's stream
+        for k in ("price", "delta", "vega", "rho"):
+            assert np.array_equal(ref[k], got[k])
+    b.close()
 
 
 @pytest.mark.parametrize("prec,tuning", [(mc.FP32, (2048, 16, 1, 0)), (mc.FP64, (2048, 16, 1, 0)),

@@ -207,6 +207,29 @@ def test_sfmt19937_generator(engine):
         assert np.max(np.abs(got[t] - want) / np.maximum(np.abs(want), 1e-3)) < 2e-7
 
 
+def test_sfmt19937_sample_scaled_draws_against_mkl(engine):
+    # the European sample's draws, vsRngGaussian(ICDF, ..., MuByT, VBySqrtT)
+    # on stream 777 in three consecutive blocks, against MKL's own output
+    # (tests/golden/mkl_streams.npz): MKL applies mean and deviation in float
+    # (pinned on the CPU), so the gap is MKL's float inverse CDF, up to
+    # 6e-5 |z| of the deviation in the tails
+    import torch
+    from scipy.special import ndtri
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mkl_streams.npz"))
+    blocks, mean, sd = d["sfmt777_sample_blocks"], d["sfmt777_sample_mean"], d["sfmt777_sample_sd"]
+    dev = torch.device("cuda:0")
+    out = torch.zeros(blocks.size, dtype=torch.float32, device=dev)
+    mc.sfmt19937_gaussian_async(777, 1, blocks.size, out, blocks.shape[1], torch.tensor(mean, device=dev),
+                                torch.tensor(sd, device=dev))
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().reshape(blocks.shape)
+    z = ndtri(orc.sfmt19937_icdf_uniforms(777, blocks.size)).reshape(blocks.shape)
+    assert np.all(np.abs(got - blocks) <= sd[:, None] * 6e-5 * np.maximum(1.0, np.abs(z)) + 2e-7)
+    # and the engine's value is the exact mean + sd z rounded once to float
+    exact = mean[:, None] + sd[:, None] * z
+    assert np.all(np.abs(got - exact) <= 1.5 * np.spacing(np.abs(exact).astype(np.float32)))
+
+
 @pytest.mark.parametrize("real", [np.float64, np.float32])
 def test_price_european_dropin(engine, real):
     cfg = _european_cfg()

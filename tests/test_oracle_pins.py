@@ -742,3 +742,21 @@ def test_heston_running_val2_confidence():
 
 def test_geometric_asian_discrete_monitoring():
     pin_geometric_asian_discrete()
+
+
+def test_sfmt19937_sample_scaled_draws_against_mkl_fixture():
+    # The European sample draws vsRngGaussian(ICDF, ..., MuByT, VBySqrtT)
+    # (MonteCarloInsideBlockingDP.cpp:249): MKL applies the (float) mean and
+    # deviation in float arithmetic -- its scaled block equals fl(m) + fl(s) z
+    # of its own unscaled normals to one ulp -- so what separates the engine's
+    # FP64 inverse CDF from MKL is MKL's float ICDF alone: within 6e-5 |z|
+    # (its error grows in the tails, 2e-4 at z = -4.18).
+    from scipy.special import ndtri
+    d = _mkl()
+    blocks, mean, sd = d["sfmt777_sample_blocks"], d["sfmt777_sample_mean"], d["sfmt777_sample_sd"]
+    m0, s0 = np.float32(mean[0]), np.float32(sd[0])
+    own = (m0 + s0 * d["sfmt777_icdf_float"][:2048]).astype(np.float32)
+    assert np.max(np.abs(blocks[0] - own)) <= 6e-8
+    z = ndtri(orc.sfmt19937_icdf_uniforms(777, 3 * 2048)).reshape(3, 2048)
+    exact = mean[:, None] + sd[:, None] * z
+    assert np.all(np.abs(blocks - exact) <= sd[:, None] * 6e-5 * np.maximum(1.0, np.abs(z)) + 1e-7)
