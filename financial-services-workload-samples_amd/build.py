@@ -2,7 +2,9 @@
 hipcc for gfx950) and the C++ drop-in entry points under host/dropin_*.cpp
 (g++), linked into one shared library in lib/; then the sample-compatible
 command lines under host/cli_*.cpp into bin/, each in a double (_dp, the
-samples' -DUSE_DOUBLE) and a float (_sp) build.
+samples' -DUSE_DOUBLE) and a float (_sp) build; and lib/libfsmc_probe.so,
+the instruction issue-rate probe bench.py measures the GPU with (not linked
+into the pricing library).
 
 Run directly (``python build.py``) or through ``__graft_entry__.build()``.
 """
@@ -18,6 +20,8 @@ HOST = os.path.join(HERE, "host")
 OBJ = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libfsmc.so")
+PROBE_SRC = os.path.join(HERE, "probe", "issue_rates.hip")
+PROBE_LIB = os.path.join(LIB_DIR, "libfsmc_probe.so")
 BIN_DIR = os.path.join(HERE, "bin")
 GXX = os.environ.get("CXX", "g++")
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{os.path.join(os.path.dirname(HERE), 'include')}"]
@@ -77,6 +81,11 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, LIB)
+    tmp = PROBE_LIB + ".tmp"
+    r = subprocess.run([HIPCC] + COMMON + ["-shared", PROBE_SRC, "-o", tmp], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("probe build failed:\n" + r.stderr)
+    os.replace(tmp, PROBE_LIB)
     os.makedirs(BIN_DIR, exist_ok=True)
     clis = []
     for src in sorted(glob.glob(os.path.join(HOST, "cli_*.cpp"))):
