@@ -36,14 +36,22 @@ METRIC = "MC path-steps/s and adjoint Greeks/s at 3-SE accuracy, FP32 vs FP64 GP
 N_OPT, N_PATHS, N_STEPS = 64, 131072, 365
 R, RHO = 0.03, -0.7
 # Algorithmic FP32 work per path-step of the Heston kernel (add/sub/mul/max
-# = 1, FMA = 2, each transcendental = 1): Euler step 25 FLOP (sqrt, exp2),
-# one Box-Muller pair 9 FLOP (log2, sqrt, sin, cos).  Philox integer work
-# is not counted.
-FLOP_PER_PATH_STEP = 34
-FP32_VECTOR_PEAK_TFLOPS = 157.3
+# = 1, FMA = 2, each transcendental = 1 and also a T): the full-truncation
+# Euler step 17 FLOP (2 T: sqrt, exp) and its two normals 6 FLOP each by
+# Box-Muller (2 T + 4 FLOP per normal).  Philox integer work is not counted.
+FLOP_PER_PATH_STEP = 17 + 2 * 6
+FP32_VECTOR_PEAK_TFLOPS = 157.3      # datasheet, packed v_pk_fma_f32
 FP64_VECTOR_PEAK_TFLOPS = 78.6
-# 256 CUs x 4 SIMDs, each issuing one wave64 VALU instruction per 4 cycles at 2.4 GHz
-VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4
+# Files whose text fixes the Heston path kernel's code object; the committed
+# counter profile records their SHA-256, so a line computed from counters of
+# another build says so.
+KERNEL_SOURCES = ["financial-services-workload-samples_amd/csrc/heston.hip",
+                  "financial-services-workload-samples_amd/csrc/path_models.hpp",
+                  "financial-services-workload-samples_amd/csrc/fsmc_device.hpp",
+                  "financial-services-workload-samples_amd/build.py"]
+COUNTER_PROFILE = "profiles/heston_asian_fp32_64x131072x365_counters.txt"
+PROBE_KINDS = ["fma_f32", "exp_f32", "log_f32", "sqrt_f32", "rcp_f32", "sin_f32", "mad_u64_u32", "fma_f64",
+               "mix_7fma_1exp", "fma_f32_vop3", "add_f32", "cvt_f32_u32"]
 # LIBOR per path (same convention, division = reciprocal + multiply): each of
 # the 2380 forward (step, rate) updates 10 FLOP (2 T: rcp, exp2), each of the
 # 2380 adjoint updates 14 FLOP (2 T: two rcp), portfolio and its reverse
@@ -58,12 +66,22 @@ def _params(np):
                 xi=np.full(N_OPT, 0.3))
 
 
+def kernel_source_sha256():
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
 def _profile_figures():
-    """HBM bytes, VALU wavefront instructions and counted FP32 FLOP per
-    launch of the path kernel, from the committed rocprofv3 summary of this
-    workload (FETCH_SIZE + WRITE_SIZE scaled by a plain copy; SQ_INSTS_VALU;
-    64 lanes x (ADD + MUL + 2 FMA + TRANS) of SQ_INSTS_VALU_*_F32)."""
-    path = os.path.join(ROOT, "profiles", "heston_asian_fp32_64x131072x365_traffic.txt")
+    """Per-launch counter figures of the Heston path kernel from the committed
+    rocprofv3 summary (profiles/heston_asian_fp32_64x131072x365_counters.txt:
+    HBM bytes = FETCH_SIZE + WRITE_SIZE scaled by a plain copy; VALU
+    wavefront instructions by class), with the commit and kernel-source hash
+    they were taken at."""
+    path = os.path.join(ROOT, COUNTER_PROFILE)
     vals = {}
     if os.path.exists(path):
         for line in open(path):
@@ -76,7 +94,52 @@ def _profile_figures():
             return float(vals[k])
         except (KeyError, ValueError):
             return None
-    return num("bytes_per_launch"), num("valu_insts_per_launch"), num("fp32_flop_per_launch")
+    return {"bytes": num("bytes_per_launch"), "valu": num("valu_insts_per_launch"),
+            "trans_f32": num("trans_f32_insts_per_launch"), "int64": num("int64_insts_per_launch"),
+            "cvt": num("cvt_insts_per_launch"), "f64": num("f64_insts_per_launch"),
+            "fp32_flop": num("fp32_flop_per_launch"), "commit": vals.get("commit"),
+            "source_sha256": vals.get("source_sha256")}
+
+
+def _issue_rates():
+    """Sustained whole-GPU issue rate of each probe kind (wave64 instructions
+    per second), measured now on this GPU by lib/libfsmc_probe.so."""
+    import ctypes
+    path = os.path.join(ROOT, "financial-services-workload-samples_amd", "lib", "libfsmc_probe.so")
+    L = ctypes.CDLL(path)
+    n = L.fsmc_probe_kinds()
+    rates = (ctypes.c_double * n)()
+    st = L.fsmc_probe_issue_rates(4096, rates, n)
+    if st != 0:
+        raise RuntimeError(f"issue-rate probe failed ({st})")
+    return dict(zip(PROBE_KINDS, list(rates)))
+
+
+def _instruction_mix(prof, rates, kernel_ms):
+    """Instruction-mix ceiling of one launch: each VALU class of the kernel
+    (transcendental FP32, 64-bit integer multiply-add = Philox's
+    v_mad_u64_u32, conversions, FP64, and all other VALU instructions at the
+    rate of a three-operand v_fma_f32) at its rate measured by the probe
+    just now, summed.  The probe's 7 FMA + 1 exp kernel tests the no-overlap
+    assumption: its time over the sum of its parts is reported (1 = the
+    classes do not overlap)."""
+    keys = ("valu", "trans_f32", "int64", "cvt", "f64")
+    if any(prof.get(k) is None for k in keys):
+        return None
+    trans_rate = sum(rates[k] for k in ("exp_f32", "log_f32", "sqrt_f32", "rcp_f32", "sin_f32")) / 5
+    other = prof["valu"] - prof["trans_f32"] - prof["int64"] - prof["cvt"] - prof["f64"]
+    parts = {"trans_f32": prof["trans_f32"] / trans_rate, "int64": prof["int64"] / rates["mad_u64_u32"],
+             "cvt": prof["cvt"] / rates["cvt_f32_u32"], "f64": prof["f64"] / rates["fma_f64"],
+             "other": other / rates["fma_f32_vop3"]}
+    t = sum(parts.values())
+    mix_sum = 7 / 8 / rates["fma_f32"] + 1 / 8 / rates["exp_f32"]
+    return {"ceiling_ms": t * 1e3, "frac": t * 1e3 / kernel_ms,
+            "ms_by_class": {k: v * 1e3 for k, v in parts.items()},
+            "insts_per_launch": {"trans_f32": prof["trans_f32"], "int64": prof["int64"], "cvt": prof["cvt"],
+                                 "f64": prof["f64"], "other": other},
+            "rate_ginst_per_s": {k: v / 1e9 for k, v in rates.items()},
+            "mix_probe_time_over_sum_of_parts": (1 / rates["mix_7fma_1exp"]) / mix_sum,
+            "model": "sum over classes at the rates measured in this run (no overlap)"}
 
 
 def _cpu_baseline(np, P, target_s):
@@ -206,11 +269,25 @@ def _libor_config(np, torch, mc, eng, precisions, steps):
     for name in res:
         ms = res[name][0]
         out["ms_" + name] = ms
-        out["greeks_per_s_" + name] = 80 * lp / (ms * 1e-3)
+        # 80 deltas of every path; the portfolio's own 80 deltas per run
+        out["path_greeks_per_s_" + name] = 80 * lp / (ms * 1e-3)
+        out["portfolio_greeks_per_s_" + name] = 80 / (ms * 1e-3)
         out["path_steps_per_s_" + name] = 40 * lp / (ms * 1e-3)
     if "fp32" in res:
         achieved = LIBOR_FLOP_PER_PATH * lp / (res["fp32"][0] * 1e-3) / 1e12
         out.update({"achieved_tflops_fp32": achieved, "frac_fp32_peak": achieved / FP32_VECTOR_PEAK_TFLOPS})
+        # The same algorithm as the FP64 kernel (trajectory stored in HBM):
+        # the FP32 kernel stores it above lambda = 0.3 instead of rebuilding
+        # rows, and lambda does not change its instruction count.
+        m = dict(precisions)["fp32"]
+        ws = torch.empty(mc.libor_workspace(lp, m, lam=0.31), dtype=torch.uint8, device=dev)
+        v = torch.zeros(2, dtype=torch.float64, device=dev)
+        d = torch.zeros(160, dtype=torch.float64, device=dev)
+        out["ms_fp32_stored_trajectory"] = _timed(torch, np, lambda: mc.libor_async(lp, m, v, d, ws, lam=0.31),
+                                                  steps, 2)
+        del ws
+        if "fp64" in res:
+            out["fp32_stored_trajectory_speedup_over_fp64"] = res["fp64"][0] / out["ms_fp32_stored_trajectory"]
     if "fp64" in res and "fp32" in res:
         se_d = res["fp64"][2][80:]
         err_d = np.abs(res["fp32"][2][:80] - res["fp64"][2][:80])
@@ -261,6 +338,7 @@ def _batch_config(np, torch, mc, eng, precisions, steps):
         tuning = b.autotune()
         tune_s = time.perf_counter() - t0
         ms = _timed(torch, np, lambda: b.run(mc._stream()), steps, 1)
+        tlat = b.results(mc._stream())["latency_us"] / 1e3
         b.close()
         lb = mc.Batch(eng, reqs, m, tuning=(tuning[0], 256, tuning[2], tuning[3]))
         lms = _timed(torch, np, lambda: lb.run(mc._stream()), steps, 1)
@@ -272,9 +350,14 @@ def _batch_config(np, torch, mc, eng, precisions, steps):
                     "tuning_" + name: {"chunk_paths": tuning[0], "requests_per_launch": tuning[1],
                                        "replay_graph": tuning[2], "workgroups_per_cu": tuning[3]},
                     "autotune_s_" + name: tune_s,
+                    # request completion latency of the throughput configuration above
+                    "latency_p50_ms_" + name: float(np.percentile(tlat, 50)),
+                    "latency_p99_ms_" + name: float(np.percentile(tlat, 99)),
+                    # the same requests served in micro-batches of 256 (in arrival order)
                     "microbatch256_ms_" + name: lms,
-                    "latency_p50_ms_" + name: float(np.percentile(lat, 50)),
-                    "latency_p99_ms_" + name: float(np.percentile(lat, 99))})
+                    "microbatch256_requests_per_s_" + name: len(reqs) / (lms * 1e-3),
+                    "microbatch256_latency_p50_ms_" + name: float(np.percentile(lat, 50)),
+                    "microbatch256_latency_p99_ms_" + name: float(np.percentile(lat, 99))})
     return out
 
 
@@ -317,7 +400,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=14.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
-    ap.add_argument("--workload", choices=["heston", "heston_adjoint", "copy"] + list(CONFIGS),
+    ap.add_argument("--workload", choices=["heston", "heston_adjoint", "copy", "probe"] + list(CONFIGS),
                     help="profiling aid: run only this workload in FP32 for --steps launches and print its timing")
     ap.add_argument("--fp64", action="store_true", help="with --workload: run the plain FP64 path as well")
     args = ap.parse_args()
@@ -337,6 +420,11 @@ def main():
         return
     if args.workload == "copy":
         print(json.dumps(_copy_probe(torch, np, args.steps)), flush=True)
+        return
+    if args.workload == "probe":
+        torch.zeros(1, device="cuda:0")
+        rates = _issue_rates()
+        print(json.dumps({k: v / 1e9 for k, v in rates.items()}), flush=True)
         return
     dev = torch.device("cuda:0")
     P = _params(np)
@@ -369,8 +457,17 @@ def main():
         msg, kg, _, _ = run(m32, args.steps, args.warmup, with_greeks=True)
         print(json.dumps({"workload": "heston_adjoint", "ms": msg, "kernel_ms": kg}), flush=True)
         return
+    # The issue-rate probe runs first (about 0.2 s of GPU work): its rates
+    # feed the instruction-mix ceiling, and the GPU clock, which ramps up
+    # over the first ~30 ms of load (a cold first launch of the path kernel
+    # takes 3.7 ms, a warm one 3.16 ms), has settled by the timed steps.
+    rates, rate_err = None, None
+    try:
+        rates = _issue_rates()
+    except (OSError, RuntimeError) as e:
+        rate_err = str(e)
     ms, kern_ms, p32, c32 = run(m32, args.steps, args.warmup)
-    traffic, valu, counted_flop = _profile_figures()
+    prof = _profile_figures()
     if args.workload == "heston":
         rec = {"workload": "heston", "ms": ms, "kernel_ms": kern_ms}
         if args.fp64:
@@ -380,6 +477,8 @@ def main():
     path_steps = N_OPT * N_PATHS * N_STEPS
     value = path_steps / (ms * 1e-3)
     achieved = FLOP_PER_PATH_STEP * path_steps / (kern_ms * 1e-3) / 1e12
+    sha = kernel_source_sha256()
+    mix = _instruction_mix(prof, rates, kern_ms) if rates else {"error": rate_err}
     rec = {
         "metric": METRIC, "value": value, "unit": "path-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -387,12 +486,17 @@ def main():
         "config": {"workload": "heston_asian_arithmetic_call", "options": N_OPT, "paths": N_PATHS,
                    "steps": N_STEPS, "rng": "philox4x32-10+box-muller", "parallelism": "single-gpu"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "traffic": traffic,
+                     "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "traffic": prof["bytes"],
                      "kernel": "heston_paths_kernel<float> (+finalize)", "kernel_ms": kern_ms,
                      "flop_per_path_step": FLOP_PER_PATH_STEP,
-                     # issue ceiling: one wave64 VALU instruction per SIMD per 4 cycles
-                     "valu_issue_frac": (valu / (kern_ms * 1e-3) / VALU_ISSUE_PEAK) if valu else None,
-                     "counted_tflops": (counted_flop / (kern_ms * 1e-3) / 1e12) if counted_flop else None},
+                     # the FP32 FLOP rate of back-to-back three-operand v_fma_f32, measured now
+                     "measured_fma_tflops": (mix["rate_ginst_per_s"]["fma_f32_vop3"] * 128 / 1e3
+                                             if mix and "rate_ginst_per_s" in mix else None),
+                     "counted_tflops": (prof["fp32_flop"] / (kern_ms * 1e-3) / 1e12) if prof["fp32_flop"] else None,
+                     "instruction_mix": mix,
+                     "counters": {"profile": COUNTER_PROFILE, "commit": prof["commit"],
+                                  "source_sha256": prof["source_sha256"], "current_source_sha256": sha,
+                                  "match": prof["source_sha256"] == sha}},
     }
     if not args.no_secondary:
         ms64, k64, p64, c64 = run(mc.mode(mc.FP64, seed=1), 3, 1)
@@ -404,7 +508,10 @@ def main():
             "fp32_vs_fp64_max_rel_price_err": rel,
             "fp32_vs_fp64_max_err_in_se": float(np.max(np.abs(p32 - p64) / (c64 / 1.96))),
             "adjoint_fp32_ms": msg,
-            "adjoint_greeks_per_s": N_OPT * N_PATHS * 3 / (msg * 1e-3),
+            # delta, vega and rho of every path (the per-path estimator terms)
+            "adjoint_path_greeks_per_s": N_OPT * N_PATHS * 3 / (msg * 1e-3),
+            # the 3 Greeks of each of the 64 options
+            "adjoint_option_greeks_per_s": N_OPT * 3 / (msg * 1e-3),
             "adjoint_path_steps_per_s": path_steps / (msg * 1e-3),
         }
     if not args.no_secondary:
