@@ -58,51 +58,80 @@ __device__ __forceinline__ Real normal_at(const LiborArgs& a, int p, int n, Real
   return j == 0 ? zq[0] : j == 1 ? zq[1] : j == 2 ? zq[2] : zq[3];
 }
 
-// portfolio_b (liborSwaptionGreeks.cpp:120-211) on the final rates L (row
-// 40): returns the discounted portfolio value and sets Lb = dV/dL(40, .).
-template <typename Real, bool kInplace>
-__device__ __forceinline__ Real portfolio_adjoint(const Real (&L)[NN], Real (&Lb)[NN]) {
+// portfolio_b (liborSwaptionGreeks.cpp:120-211) on the final rates: returns
+// the discounted portfolio value and sets Lb = dV/dL(40, .).  The sample
+// keeps the bond prices B[m] and the adjoints B_b[m], S_b[m] in arrays of
+// 40; here the adjoints are running scalars (S_b and B_b of date m only feed
+// date m - 1), the swaption payoffs' contributions to them are five pairs of
+// scalars (one per maturity), and the reverse sweep recomputes B[m] forward
+// from a checkpoint every 8 dates -- the same divisions in the same order as
+// the forward sweep, so every value equals the array formulation's bit for
+// bit while the section needs ~20 registers besides the rates and adjoints.
+// lo(n) returns rate n < 40 (registers or LDS, as the caller keeps them).
+constexpr int kSeg = 8;
+constexpr int kMats = 5;  // distinct maturities of params.h:38
+__host__ __device__ constexpr int mat_slot(int m) {  // m = maturity - 1 -> its slot, or -1
+  return m == 3 ? 0 : m == 7 ? 1 : m == 19 ? 2 : m == 27 ? 3 : m == 39 ? 4 : -1;
+}
+template <typename Real, bool kInplace, typename Lo>
+__device__ __forceinline__ Real portfolio_adjoint(const Real (&L)[NN], Lo lo, Real (&Lb)[NN]) {
   using O = MathOps<Real>;
   const Real delta = Real(0.25);
-  Real B[NMAT], Bb[NMAT], Sb[NMAT];
+  Real bck[NMAT / kSeg];  // B at the start of each segment (B[-1] = 1 for the first)
+  Real injS[kMats], injB[kMats];
   Real b = 1, s = 0, val = 0;
 #pragma unroll
   for (int m = 0; m < NMAT; ++m) {
+    if (m % kSeg == 0) bck[m / kSeg] = b;
     b = O::div(b, Real(1) + delta * L[m + NMAT]);
     s = s + delta * b;
-    B[m] = b;
-    Bb[m] = 0;
-    Sb[m] = 0;
+    if (mat_slot(m) >= 0) {
+      Real sb = 0, bb = 0;
 #pragma unroll
-    for (int j = 0; j < NOPT; ++j) {
-      if (mat_of(j) - 1 == m) {
-        const Real K = (Real)rate_of(j);
-        const Real sw = b + K * s - Real(1);
-        if (sw < Real(0)) {
-          val += Real(-100) * sw;
-          Sb[m] += Real(-100) * K;
-          Bb[m] += Real(-100);
+      for (int j = 0; j < NOPT; ++j) {
+        if (mat_of(j) - 1 == m) {
+          const Real K = (Real)rate_of(j);
+          const Real sw = b + K * s - Real(1);
+          if (sw < Real(0)) {
+            val += Real(-100) * sw;
+            sb += Real(-100) * K;
+            bb += Real(-100);
+          }
         }
       }
+      injS[mat_slot(m)] = sb;
+      injB[mat_slot(m)] = bb;
     }
   }
+  Real Sb = 0, Bb = 0;  // S_b[m], B_b[m] of the date being processed
 #pragma unroll
-  for (int m = NMAT - 1; m >= 0; --m) {
-    const int n = m + NMAT;
-    Bb[m] += delta * Sb[m];
-    const Real dn = Real(1) + delta * L[n];
-    Lb[n] = O::div(-Bb[m] * B[m] * delta, dn);
-    if (m > 0) {
-      Sb[m - 1] += Sb[m];
-      Bb[m - 1] += O::div(Bb[m], kInplace ? Real(1) + delta * Lb[n] : dn);
+  for (int g = NMAT / kSeg - 1; g >= 0; --g) {
+    Real B[kSeg];  // B[g kSeg .. g kSeg + kSeg - 1], recomputed as in the forward sweep
+    Real bb0 = bck[g];
+#pragma unroll
+    for (int q = 0; q < kSeg; ++q) {
+      bb0 = O::div(bb0, Real(1) + delta * L[g * kSeg + q + NMAT]);
+      B[q] = bb0;
+    }
+#pragma unroll
+    for (int q = kSeg - 1; q >= 0; --q) {
+      const int m = g * kSeg + q, n = m + NMAT;
+      if (mat_slot(m) >= 0) {
+        Sb += injS[mat_slot(m)];
+        Bb += injB[mat_slot(m)];
+      }
+      Bb += delta * Sb;
+      const Real dn = Real(1) + delta * L[n];
+      Lb[n] = O::div(-Bb * B[q] * delta, dn);
+      if (m > 0) Bb = O::div(Bb, kInplace ? Real(1) + delta * Lb[n] : dn);
     }
   }
   b = 1;
 #pragma unroll
-  for (int n = 0; n < NMAT; ++n) b = O::div(b, Real(1) + delta * L[n]);
+  for (int n = 0; n < NMAT; ++n) b = O::div(b, Real(1) + delta * lo(n));
   val = b * val;
 #pragma unroll
-  for (int n = 0; n < NMAT; ++n) Lb[n] = O::div(-val * delta, Real(1) + delta * L[n]);
+  for (int n = 0; n < NMAT; ++n) Lb[n] = O::div(-val * delta, Real(1) + delta * lo(n));
 #pragma unroll
   for (int n = NMAT; n < NN; ++n) Lb[n] = b * Lb[n];
 
@@ -146,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void libor_path_kernel(LiborArgs a) {
 
   // ---- portfolio_b ---------------------------------------------------------------
   Real Lb[NN];
-  const Real val = portfolio_adjoint<Real, kInplace>(L, Lb);
+  const Real val = portfolio_adjoint<Real, kInplace>(L, [&](int n) { return L[n]; }, Lb);
 
   // ---- adjoint: path_calc_b2 (L holds row n+1, Lb the adjoints) ---------------------
   for (int n = NMAT - 1; n >= 0; --n) {
@@ -240,16 +269,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   // becomes M(n,i) = M(n+1,i) + L(n,i) lambda v1 faci^2, with no division, and
   // the row total S = sum_{k>n} L(n+1,k) Lb(k) is the plain sum of M.  Every
   // rate starts at L0, so Lb(0,i) = M(0,i) / L0.
-  float M[NN];
-  const float val = portfolio_adjoint<float, kInplace>(L, M);
-#pragma unroll
-  for (int i = 1; i < NN; ++i) M[i] *= L[i];
-  // Rates 1..39 move to LDS for the reverse sweep (each thread its own
-  // column, no barrier needed): with only rates 40..79 and the adjoints in
-  // registers the kernel fits three waves per SIMD.
+  // Rates 1..39 move to LDS for the portfolio and the reverse sweep (each
+  // thread its own column, no barrier needed): with only rates 0 and
+  // 40..79 and the adjoints in registers the kernel fits three waves per
+  // SIMD without spilling.
   __shared__ float Llo[NMAT][kBlock];
 #pragma unroll
   for (int i = 1; i < NMAT; ++i) Llo[i][threadIdx.x] = L[i];
+  // compiler barrier (no instruction): the rates are read back from LDS
+  // below instead of being kept live in registers
+  asm volatile("" ::: "memory");
+  float M[NN];
+  const float L0r = L[0];
+  const float val = portfolio_adjoint<float, kInplace>(
+      L, [&](int n) { return n == 0 ? L0r : Llo[n][threadIdx.x]; }, M);
+#pragma unroll
+  for (int i = 1; i < NN; ++i) M[i] *= i < NMAT ? Llo[i][threadIdx.x] : L[i];
 
   // ---- adjoint: path_calc_b2, rows rebuilt in registers ------------------------
   // v1(i) lambda faci^2 = con1^2 (S - P) / (1 + delta L(n,i))^2, with P the
