@@ -340,7 +340,7 @@ def _batch_config(np, torch, mc, eng, precisions, steps):
         ms = _timed(torch, np, lambda: b.run(mc._stream()), steps, 1)
         tlat = b.results(mc._stream())["latency_us"] / 1e3
         b.close()
-        lb = mc.Batch(eng, reqs, m, tuning=(tuning[0], 256, tuning[2], tuning[3]))
+        lb = mc.Batch(eng, reqs, m, tuning=(tuning[0], 256, tuning[2], tuning[3], tuning[4]))
         lms = _timed(torch, np, lambda: lb.run(mc._stream()), steps, 1)
         lat = lb.results(mc._stream())["latency_us"] / 1e3
         lb.close()
@@ -348,7 +348,8 @@ def _batch_config(np, torch, mc, eng, precisions, steps):
                     "path_steps_per_s_" + name: ps / (ms * 1e-3),
                     "path_greeks_per_s_" + name: 3 * npaths / (ms * 1e-3),
                     "tuning_" + name: {"chunk_paths": tuning[0], "requests_per_launch": tuning[1],
-                                       "replay_graph": tuning[2], "workgroups_per_cu": tuning[3]},
+                                       "replay_graph": tuning[2], "workgroups_per_cu": tuning[3],
+                                       "work_queue": tuning[4]},
                     "autotune_s_" + name: tune_s,
                     # request completion latency of the throughput configuration above
                     "latency_p50_ms_" + name: float(np.percentile(tlat, 50)),

@@ -154,6 +154,50 @@ __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(BatchArgs a, c
   }
 }
 
+// Work-queue variant: gridDim.x = CUs x workgroups per CU; each workgroup
+// takes the next item of this micro-batch's longest-first order from the
+// shared counter *head (one relaxed device-scope increment per item, by
+// thread 0, broadcast through LDS; the increment for the following item is
+// issued when an item starts, so its latency hides behind the item) and
+// runs it, until the index it takes is past the end.  Every index below
+// n_items is taken exactly once, so a workgroup takes at most n_items + 1
+// indices: the loop ends for every workgroup whatever the others do.  Nothing waits on another workgroup and
+// nothing needs a fence: the items' inputs are written before the launch
+// and their partials are read by the next kernel.  *head is zeroed by
+// batch_reset_kernel earlier on the same stream.
+template <typename Real>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 3 : 1))) void
+batch_queue_kernel(BatchArgs a, const int* order, int n_items, int* head) {  // FP32: <= 168 registers, three
+                                                                             // waves per SIMD as the grouped
+                                                                             // Heston grid gets
+  __shared__ double lds[kNV * (kBlock / 64)];
+  __shared__ int next;
+  Real* ck = (Real*)a.ckpt + (size_t)blockIdx.x * kBlock + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  if (threadIdx.x == 0) next = atomicAdd(head, 1);
+  __syncthreads();
+  int j = next;
+  __syncthreads();  // every thread has read next before thread 0 replaces it
+  while (j < n_items) {  // uniform per workgroup
+    // thread 0 takes the following index now; its latency overlaps the item
+    const int ahead = threadIdx.x == 0 ? atomicAdd(head, 1) : 0;
+    const int item = order[j];
+    if (a.req[a.items[item].x].kind == FSMC_REQ_GBM_EUROPEAN)  // uniform per workgroup
+      gbm_item<Real>(a, item, lds);
+    else
+      heston_item<Real>(a, item, ck, stride, lds);
+    if (threadIdx.x == 0) next = ahead;
+    __syncthreads();  // next is written, and thread 0's reads of lds in block_sum precede the next item's writes
+    j = next;
+    __syncthreads();  // every thread has read next before thread 0 replaces it
+  }
+}
+
+__global__ void batch_reset_kernel(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 // Wall-clock stamp i (graph mode: i = 0 before the first micro-batch, w + 1
 // after micro-batch w).
 __global__ void batch_stamp_kernel(unsigned long long* stamps, int i) {
@@ -186,6 +230,7 @@ struct MicroBatch {
   int gbm_first, gbm_count;  // items
   int hes_first, hes_count;
   int off_first;             // persistent mode: this micro-batch's G + 1 schedule offsets
+  int q_first, q_count;      // work-queue mode: its items' longest-first order in qorder
 };
 
 struct BatchImpl {
@@ -193,13 +238,13 @@ struct BatchImpl {
   fsmc_mode mode;
   std::vector<fsmc_request> req;
   int max_seg = 1;
-  fsmc_batch_tuning tuning = {4096, 0, 0, 0};
+  fsmc_batch_tuning tuning = {4096, 0, 0, 0, 0};
   std::vector<MicroBatch> waves;
   int max_hes_grid = 0;
   int cus = 0;  // compute units (persistent grid = cus x workgroups_per_cu)
   DeviceBuffer ws;
   size_t off_req = 0, off_items = 0, off_first = 0, off_nitems = 0, off_part = 0, off_res = 0, off_ckpt = 0,
-         off_stamp = 0, off_sched = 0, off_soff = 0;
+         off_stamp = 0, off_sched = 0, off_soff = 0, off_qorder = 0, off_heads = 0;
   std::vector<hipEvent_t> done;  // one per micro-batch (stream mode)
   hipEvent_t start = nullptr;
   hipEvent_t t0 = nullptr, t1 = nullptr;  // autotuner's timing around a run
@@ -273,6 +318,8 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
   FSMC_REQUIRE(t.replay_graph == 0 || t.replay_graph == 1, "replay_graph %d is not 0 or 1", t.replay_graph);
   FSMC_REQUIRE(t.workgroups_per_cu >= 0 && t.workgroups_per_cu <= 16, "workgroups_per_cu %d outside [0, 16]",
                t.workgroups_per_cu);
+  FSMC_REQUIRE(t.work_queue == 0 || (t.work_queue == 1 && t.workgroups_per_cu >= 1),
+               "work_queue %d must be 0, or 1 with workgroups_per_cu >= 1", t.work_queue);
   FSMC_HIP(hipSetDevice(ctx->device));
   FSMC_HIP(hipStreamSynchronize(ctx->stream));
   batch_release_events(b);
@@ -319,7 +366,7 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
   // workgroup lists; the lists are concatenated into sched, with G + 1
   // offsets per micro-batch in soff.
   const int G = t.workgroups_per_cu > 0 ? b.cus * t.workgroups_per_cu : 0;
-  std::vector<int> sched, soff;
+  std::vector<int> sched, soff, qorder;
   if (G > 0) {
     for (MicroBatch& mb : b.waves) {
       std::vector<std::pair<double, int>> cost;
@@ -330,6 +377,20 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
         cost.push_back({(double)paths * (r.kind == FSMC_REQ_GBM_EUROPEAN ? 1.0 : 3.0 * r.n_steps), it});
       }
       std::stable_sort(cost.begin(), cost.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+      // the work queue hands the items out in this order
+      mb.q_first = (int)qorder.size();
+      mb.q_count = (int)cost.size();
+      for (const auto& c : cost) qorder.push_back(c.second);
+      {
+        // the order is a permutation of the micro-batch's items (checked
+        // here, so the kernel's reads order[j], j < q_count, stay in bounds)
+        std::vector<char> seen(cost.size(), 0);
+        for (int q = 0; q < mb.q_count; ++q) {
+          const int it = qorder[mb.q_first + q] - mb.gbm_first;
+          FSMC_REQUIRE(it >= 0 && it < mb.q_count && !seen[it], "internal: work-queue order is not a permutation");
+          seen[it] = 1;
+        }
+      }
       std::vector<std::vector<int>> lists(G);
       std::vector<std::pair<double, int>> load;  // min-heap of (load, workgroup)
       for (int g = 0; g < G; ++g) load.push_back({0.0, g});
@@ -386,6 +447,8 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
   b.off_stamp = take(sizeof(unsigned long long) * (b.waves.size() + 1));
   b.off_sched = take(sizeof(int) * std::max<size_t>(1, sched.size()));
   b.off_soff = take(sizeof(int) * std::max<size_t>(1, soff.size()));
+  b.off_qorder = take(sizeof(int) * std::max<size_t>(1, qorder.size()));
+  b.off_heads = take(sizeof(int) * b.waves.size());
   FSMC_TRY(b.ws.reserve(off));
   char* ws = (char*)b.ws.ptr;
   hipStream_t s = ctx->stream;
@@ -396,6 +459,8 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
   if (G > 0) {
     FSMC_HIP(hipMemcpyAsync(ws + b.off_sched, sched.data(), sizeof(int) * sched.size(), hipMemcpyHostToDevice, s));
     FSMC_HIP(hipMemcpyAsync(ws + b.off_soff, soff.data(), sizeof(int) * soff.size(), hipMemcpyHostToDevice, s));
+    FSMC_HIP(hipMemcpyAsync(ws + b.off_qorder, qorder.data(), sizeof(int) * qorder.size(), hipMemcpyHostToDevice,
+                            s));
   }
   FSMC_HIP(hipStreamSynchronize(s));
   if (t.replay_graph) {
@@ -421,13 +486,22 @@ static int batch_enqueue(BatchImpl& b, hipStream_t s, bool stamps) {
   a.w_gbm = counter_w(kTagGbm, b.mode.stream);
   a.w_heston = counter_w(kTagHeston, b.mode.stream);
   unsigned long long* stamp = (unsigned long long*)(ws + b.off_stamp);
+  int* heads = (int*)(ws + b.off_heads);
+  if (b.tuning.work_queue) {  // one counter per micro-batch, zeroed by a kernel (no memset node in a graph)
+    const int nw = (int)b.waves.size();
+    hipLaunchKernelGGL(batch_reset_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, heads, nw);
+  }
   if (stamps)
     hipLaunchKernelGGL(batch_stamp_kernel, dim3(1), dim3(64), 0, s, stamp, 0);
   else
     FSMC_HIP(hipEventRecord(b.start, s));
   for (size_t w = 0; w < b.waves.size(); ++w) {
     const MicroBatch& mb = b.waves[w];
-    if (b.tuning.workgroups_per_cu > 0) {
+    if (b.tuning.workgroups_per_cu > 0 && b.tuning.work_queue) {
+      const int G = b.cus * b.tuning.workgroups_per_cu;
+      hipLaunchKernelGGL(batch_queue_kernel<Real>, dim3(G), dim3(kBlock), 0, s, a,
+                         (const int*)(ws + b.off_qorder) + mb.q_first, mb.q_count, heads + w);
+    } else if (b.tuning.workgroups_per_cu > 0) {
       const int G = b.cus * b.tuning.workgroups_per_cu;
       hipLaunchKernelGGL(batch_persistent_kernel<Real>, dim3(G), dim3(kBlock), 0, s, a,
                          (const int*)(ws + b.off_sched), (const int*)(ws + b.off_soff) + mb.off_first);
@@ -489,7 +563,7 @@ int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_
   bi.mode = *m;
   bi.req.assign(req, req + n);
   bi.max_seg = max_seg;
-  fsmc_batch_tuning t = {4096, 0, 0, 0};
+  fsmc_batch_tuning t = {4096, 0, 0, 0, 0};
   if (tuning) {
     t = *tuning;
   } else {
@@ -616,28 +690,32 @@ int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen) {
     for (int per : pers) {
       if (per >= bi.n) continue;
       float ms = 0.0f;
-      st = batch_time(b, fsmc_batch_tuning{c, per, 0, 0}, &ms);
+      st = batch_time(b, fsmc_batch_tuning{c, per, 0, 0, 0}, &ms);
       if (st != FSMC_OK) break;
       if (ms < best_ms) {
         best_ms = ms;
-        best = fsmc_batch_tuning{c, per, 0, 0};
+        best = fsmc_batch_tuning{c, per, 0, 0, 0};
       }
     }
     if (st != FSMC_OK) break;
   }
   if (st != FSMC_OK) return st;
-  for (int wpc : {1, 2, 3, 4}) {  // the persistent kernel at this chunk and micro-batch size
-    float ms = 0.0f;
-    const fsmc_batch_tuning p = {best.chunk_paths, best.requests_per_launch, 0, wpc};
-    FSMC_TRY(batch_time(b, p, &ms));
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = p;
+  const int chunk = best.chunk_paths, per = best.requests_per_launch;
+  for (int queue : {0, 1}) {
+    for (int wpc : {1, 2, 3, 4}) {  // the persistent kernels at this chunk and micro-batch size
+      float ms = 0.0f;
+      const fsmc_batch_tuning p = {chunk, per, 0, wpc, queue};
+      FSMC_TRY(batch_time(b, p, &ms));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = p;
+      }
     }
   }
   {
     float ms = 0.0f;
-    const fsmc_batch_tuning g = {best.chunk_paths, best.requests_per_launch, 1, best.workgroups_per_cu};
+    const fsmc_batch_tuning g = {best.chunk_paths, best.requests_per_launch, 1, best.workgroups_per_cu,
+                                 best.work_queue};
     FSMC_TRY(batch_time(b, g, &ms));
     if (ms < best_ms) best = g;
   }

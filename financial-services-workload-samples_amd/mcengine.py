@@ -77,7 +77,7 @@ class Result(ctypes.Structure):
 
 class BatchTuning(ctypes.Structure):
     _fields_ = [("chunk_paths", ctypes.c_int), ("requests_per_launch", ctypes.c_int), ("replay_graph", ctypes.c_int),
-                ("workgroups_per_cu", ctypes.c_int)]
+                ("workgroups_per_cu", ctypes.c_int), ("work_queue", ctypes.c_int)]
 
 
 class HestonParams(ctypes.Structure):
@@ -459,12 +459,12 @@ class Batch:
     def autotune(self):
         t = BatchTuning()
         _check(lib().fsmc_batch_autotune(self._b, ctypes.byref(t)), "fsmc_batch_autotune")
-        return t.chunk_paths, t.requests_per_launch, t.replay_graph, t.workgroups_per_cu
+        return t.chunk_paths, t.requests_per_launch, t.replay_graph, t.workgroups_per_cu, t.work_queue
 
     def tuning(self):
         t = BatchTuning()
         _check(lib().fsmc_batch_tuning_get(self._b, ctypes.byref(t)), "fsmc_batch_tuning_get")
-        return t.chunk_paths, t.requests_per_launch, t.replay_graph, t.workgroups_per_cu
+        return t.chunk_paths, t.requests_per_launch, t.replay_graph, t.workgroups_per_cu, t.work_queue
 
     def close(self):
         if self._b:

@@ -366,12 +366,17 @@ typedef struct fsmc_batch_tuning {
                               CUs x this many workgroups per micro-batch,
                               each running a static longest-first list of
                               items of both kinds                         */
+  int work_queue;          /* with workgroups_per_cu >= 1: 0 the static
+                              lists above; 1 the persistent workgroups take
+                              items from a shared counter, longest first
+                              (one relaxed atomic increment per item; no
+                              workgroup waits on another)                 */
 } fsmc_batch_tuning;
 
 typedef struct fsmc_batch fsmc_batch;
 
 /* tuning NULL: the autotuner's cached choice for a batch of this shape, or
- * {4096, 0, 0, 0}. */
+ * {4096, 0, 0, 0, 0}. */
 int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_mode* m,
                       const fsmc_batch_tuning* tuning, fsmc_batch** out);
 int fsmc_batch_run(fsmc_batch* b, void* stream);
@@ -379,9 +384,9 @@ int fsmc_batch_run(fsmc_batch* b, void* stream);
  * run was enqueued on), copies the results and synchronises its stream. */
 int fsmc_batch_results(fsmc_batch* b, void* stream, fsmc_result* out);
 /* Times the batch over chunk sizes x micro-batch sizes (grouped grids), then
- * the winner as a persistent grid of 1 to 4 workgroups per CU, then the
- * best with graph replay; keeps the fastest and caches it for batches of the
- * same shape in this process. */
+ * the winner as a persistent grid of 1 to 4 workgroups per CU with static
+ * lists and with the work queue, then the best with graph replay; keeps the
+ * fastest and caches it for batches of the same shape in this process. */
 int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen);
 int fsmc_batch_tuning_get(const fsmc_batch* b, fsmc_batch_tuning* out);
 int fsmc_batch_destroy(fsmc_batch* b);

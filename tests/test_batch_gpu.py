@@ -86,6 +86,7 @@ def test_mixed_batch_reruns_and_tuning(engine):
     assert np.allclose(other["price"], first["price"], rtol=1e-9, atol=1e-12)
     chosen = b.autotune()
     assert chosen[0] % 1024 == 0 and chosen[1] >= 0 and chosen[2] in (0, 1) and chosen[3] in (0, 1, 2, 3, 4)
+    assert chosen[4] in (0, 1) and (chosen[4] == 0 or chosen[3] >= 1)
     assert b.tuning() == chosen
     b.run()
     tuned = b.results()
@@ -110,18 +111,21 @@ def test_results_wait_for_a_run_on_another_stream(engine):
     b.close()
 
 
-@pytest.mark.parametrize("prec,tuning", [(mc.FP32, (2048, 16, 1, 0)), (mc.FP64, (2048, 16, 1, 0)),
-                                         (mc.FP32, (2048, 16, 0, 2)), (mc.FP32, (2048, 16, 1, 1)),
-                                         (mc.FP64, (2048, 16, 0, 1))])
+@pytest.mark.parametrize("prec,tuning", [(mc.FP32, (2048, 16, 1, 0, 0)), (mc.FP64, (2048, 16, 1, 0, 0)),
+                                         (mc.FP32, (2048, 16, 0, 2, 0)), (mc.FP32, (2048, 16, 1, 1, 0)),
+                                         (mc.FP64, (2048, 16, 0, 1, 0)),
+                                         (mc.FP32, (2048, 16, 0, 2, 1)), (mc.FP64, (2048, 16, 0, 1, 1)),
+                                         (mc.FP32, (2048, 16, 1, 3, 1)), (mc.FP64, (1024, 0, 1, 1, 1))])
 def test_graph_and_persistent_match_grouped_launches(engine, prec, tuning):
-    # graph replay (with eager work and synchronisations between replays) and
-    # the persistent static-schedule kernel give the grouped stream launches'
-    # results bit for bit on every run (an item's sums do not depend on which
-    # workgroup runs it), and their completion times order the micro-batches
+    # graph replay (with eager work and synchronisations between replays),
+    # the persistent static-schedule kernel and the persistent work-queue
+    # kernel give the grouped stream launches' results bit for bit on every
+    # run (an item's sums do not depend on which workgroup runs it), and
+    # their completion times order the micro-batches
     import torch
     reqs = mc.mixed_requests(64, seed=8)
     m = mc.mode(prec, seed=6)
-    ref = mc.Batch(engine, reqs, m, tuning=(2048, 16, 0, 0))
+    ref = mc.Batch(engine, reqs, m, tuning=(2048, 16, 0, 0) if tuning[1] else (1024, 0, 0, 0))
     ref.run()
     want = ref.results()
     ref.close()
@@ -167,6 +171,8 @@ def test_batch_errors(engine):
         mc.Batch(engine, [mc.gbm_request(100, 10, 10, 1, .02, .3)], mc.mode(), tuning=(1024, 0, 2))
     with pytest.raises(mc.FsmcError):
         mc.Batch(engine, [mc.gbm_request(100, 10, 10, 1, .02, .3)], mc.mode(), tuning=(1024, 0, 0, 17))
+    with pytest.raises(mc.FsmcError):  # the work queue runs on the persistent grid only
+        mc.Batch(engine, [mc.gbm_request(100, 10, 10, 1, .02, .3)], mc.mode(), tuning=(1024, 0, 0, 0, 1))
 
 
 def test_serving_batch_4096_requests(engine):
