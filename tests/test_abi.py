@@ -110,9 +110,13 @@ def test_bench_refuses_more_than_one_gpu():
 def test_batch_tuning_validated_without_device(lib):
     # fsmc_batch_create rejects bad tunings before any device work; with no
     # GPU here the context cannot be created, so check the declared layout
-    t = mcengine.BatchTuning(4096, 0, 1, 2)
-    assert ctypes.sizeof(t) == 16 and (t.chunk_paths, t.requests_per_launch, t.replay_graph,
-                                        t.workgroups_per_cu) == (4096, 0, 1, 2)
+    t = mcengine.BatchTuning(4096, 0, 1, 2, 1)
+    assert ctypes.sizeof(t) == 20 and (t.chunk_paths, t.requests_per_launch, t.replay_graph,
+                                        t.workgroups_per_cu, t.work_queue) == (4096, 0, 1, 2, 1)
+    # the C struct has the same five ints
+    hdr = open(os.path.join(ROOT, "include", "fsmc.h")).read()
+    body = hdr[hdr.index("typedef struct fsmc_batch_tuning {"):hdr.index("} fsmc_batch_tuning;")]
+    assert [w for w in re.findall(r"int (\w+);", body)] == [f for f, _ in mcengine.BatchTuning._fields_]
 
 
 # ---- argument checks of the torch-tensor wrappers (before any launch) --------
