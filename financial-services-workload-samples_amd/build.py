@@ -40,7 +40,7 @@ def _compile(src):
     if src.endswith(".cpp"):
         # a drop-in that manages device buffers itself includes the HIP runtime
         hip = ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"] if "hip/hip_runtime.h" in open(src).read() else []
-        cmd = [GXX] + HOST_FLAGS + hip + ["-c", src, "-o", obj]
+        cmd = [GXX] + HOST_FLAGS + (["-O3"] if "gf2_" in src else []) + hip + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     # hipcc's host pass warns that it does not know the packed-fp32 feature.
     noise = "packed-fp32-ops' is not a recognized feature"
@@ -53,7 +53,8 @@ def _compile(src):
 
 
 def sources():
-    return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(HOST, "dropin_*.cpp")))
+    return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(HOST, "dropin_*.cpp")) +
+                  glob.glob(os.path.join(HOST, "gf2_*.cpp")))
 
 
 def _cli(src, suffix, defines):

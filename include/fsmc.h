@@ -227,6 +227,18 @@ int fsmc_mt19937_gaussian_async(fsmc_mt19937* g, long long n, double mean, doubl
                                 void* stream);
 int fsmc_mt19937_destroy(fsmc_mt19937* g);
 
+/* Jump-ahead polynomial of the MT19937 (generator 0: one step = one 32-bit
+ * word) or SFMT19937 (generator 1: one step = one 128-bit word, four
+ * outputs) stream: coeffs[0..311] receives x^steps mod phi, phi the minimal
+ * polynomial of the step (found by Berlekamp-Massey on the generator's own
+ * output bits), coefficient i in bit i % 64 of word i / 64.  The state
+ * `steps` words ahead is the XOR, over the set coefficients i, of the state
+ * window i words ahead (Haramoto et al., INFORMS J. Computing 20(3), 2008);
+ * the generators' skip-ahead uses this.  Host only; returns the degree of
+ * phi (19937 or 19968) or a negative status.  No counterpart in the
+ * samples, which cannot skip these streams. */
+int fsmc_jump_poly(int generator, unsigned long long steps, uint64_t* coeffs, int n_words);
+
 /* The European sample's streams (MonteCarloInsideBlockingDP.cpp:229,249:
  * vslNewStream(VSL_BRNG_SFMT19937, 777 + thread) and vsRngGaussian with
  * VSL_RNG_METHOD_GAUSSIAN_ICDF): SFMT19937 seeded by init_by_array({seed})
