@@ -25,8 +25,10 @@
 // barrier.  The Box-Muller transform then runs over the bits in a grid-wide
 // kernel.
 #include <algorithm>
+#include <map>
 #include <vector>
 
+#include "../host/gf2_jump.hpp"
 #include "fsmc_device.hpp"
 #include "fsmc_host.hpp"
 
@@ -142,6 +144,113 @@ __global__ __launch_bounds__(kBitsThreads) void mt_bits_kernel(MtState* st, long
   if (t == 0) st->pos = (uint32_t)pos;
 }
 
+// ---- skip-ahead: one stream over many workgroups ----------------------------
+// The raw words X_s of the stream obey X_{s+624} = twist(X_s, X_{s+1},
+// X_{s+397}); output s is temper(X_{s+624}).  Let U be the window of the 624
+// raw words of the next 624 outputs (U[i] belongs to output P + i, P the
+// stream position).  The window J words later is, word by word,
+// W[t] = XOR over the set coefficients i of x^J mod phi of X'_{i + t}, with
+// X' the sequence continued from U (tests/test_jump_poly.py checks the
+// identity on the oracle's stream).  Substream k of a call starts
+// J = k L words after U, so workgroup k regenerates X' into an LDS ring,
+// correlates it with the coefficient list of x^(kL) mod phi (host-computed,
+// sorted, read through the scalar cache: every thread takes the same i),
+// and then emits its L outputs from W.
+constexpr int kJumpThreads = 640;  // thread t < 624 owns window word t
+constexpr int kRing = 4096;        // LDS ring of raw words (16 KB)
+constexpr int kRingMask = kRing - 1;
+constexpr int kLag = kN - kM;      // 227 words can be generated at once
+
+// X[s & mask] = twist of its three predecessors for s in [g, g + 227)
+__device__ __forceinline__ void mt_ring_chunk(uint32_t* X, long long g) {
+  const int t = threadIdx.x;
+  if (t < kLag) {
+    const long long s = g + t;
+    X[s & kRingMask] =
+        twist_word(X[(s - kN) & kRingMask], X[(s - kN + 1) & kRingMask], X[(s - kLag) & kRingMask]);
+  }
+  __syncthreads();
+}
+
+// The window starting at the next output's raw word, from the state (mt, pos):
+// mt[pos..623] followed by the first pos words of the next block.
+__global__ __launch_bounds__(kBitsThreads) void mt_align_kernel(const MtState* st, uint32_t* U) {
+  __shared__ uint32_t A[kN], B[kN];
+  const int t = threadIdx.x;
+  for (int i = t; i < kN; i += kBitsThreads) A[i] = st->mt[i];
+  const int pos = (int)st->pos;
+  __syncthreads();
+  // the next block, in three dependent phases (as mt_bits_kernel)
+  if (t < kLag) B[t] = twist_word(A[t], A[t + 1], A[t + kM]);
+  __syncthreads();
+  if (t < kLag) B[t + kLag] = twist_word(A[t + kLag], A[t + kLag + 1], B[t]);
+  __syncthreads();
+  if (t + 2 * kLag < kN) {
+    const int i = t + 2 * kLag;
+    B[i] = twist_word(A[i], i + 1 < kN ? A[i + 1] : B[0], B[i - kLag]);
+  }
+  __syncthreads();
+  for (int i = t; i < kN; i += kBitsThreads) U[i] = pos + i < kN ? A[pos + i] : B[pos + i - kN];
+}
+
+// Workgroup k emits outputs [k L, min(n, (k + 1) L)) of the call into out;
+// the last one also leaves the stream's state after output n - 1 in *st.
+// poly_off[k - 1] .. poly_off[k] - 1 index the sorted coefficients of
+// x^(k L) mod phi in poly (values < 19937).
+__global__ __launch_bounds__(kJumpThreads) void mt_jump_bits_kernel(const uint32_t* U, const uint16_t* poly,
+                                                                     const int* poly_off, long long L, long long n,
+                                                                     uint32_t* out, MtState* st) {
+  __shared__ uint32_t X[kRing];
+  const int t = threadIdx.x;
+  const int k = blockIdx.x;
+  if (t < kN) X[t] = U[t];
+  __syncthreads();
+  if (k > 0) {
+    // W[t] = XOR of X'_{i + t} over the coefficients i, taken in ascending
+    // order in pieces of at most kRing - 624 so that X'_{i .. i + 623} is still
+    // in the ring when i is used
+    uint32_t acc = 0;
+    long long G = kN;  // X'_s is in the ring for s < G
+    const int q1 = poly_off[k];
+    int q = poly_off[k - 1];
+    while (q < q1) {
+      const int a = poly[q];
+      const int lim = a + (kRing - kN - kLag);  // the coefficients of this piece are < lim
+      while (G < (long long)lim + kN) {
+        mt_ring_chunk(X, G);
+        G += kLag;
+      }
+      if (t < kN) {
+        int qq = q;
+        for (; qq < q1 && (int)poly[qq] < lim; ++qq) acc ^= X[(poly[qq] + t) & kRingMask];
+      }
+      // every thread advances q the same way (uniform)
+      while (q < q1 && (int)poly[q] < lim) ++q;
+      __syncthreads();  // the reads of this piece before the ring moves on
+    }
+    if (t < kN) X[t] = acc;
+    __syncthreads();
+  }
+  const long long first = (long long)k * L;
+  const long long m = first + L < n ? L : n - first;  // outputs of this substream
+  long long G = kN;
+  for (long long base = 0; base < m; base += kLag) {
+    while (G < base + kLag) {
+      mt_ring_chunk(X, G);
+      G += kLag;
+    }
+    if (t < kLag && base + t < m) out[first + base + t] = temper(X[(base + t) & kRingMask]);
+  }
+  if (k == (int)gridDim.x - 1) {
+    // the state after output n - 1: mt = X'_{m - 624 .. m - 1} with pos = 624
+    // (twist first), or the start window with pos = m for a short substream
+    __syncthreads();
+    const long long s0 = m >= kN ? m - kN : 0;
+    for (int i = t; i < kN; i += kJumpThreads) st->mt[i] = X[(s0 + i) & kRingMask];
+    if (t == 0) st->pos = (uint32_t)(m >= kN ? kN : m);
+  }
+}
+
 // Box-Muller pairs over consecutive outputs (even count).
 __global__ __launch_bounds__(256) void mt_boxmuller_kernel(const uint32_t* bits, long long pairs, double mean,
                                                            double stddev, double* out) {
@@ -166,9 +275,84 @@ struct fsmc_mt19937 {
   fsmc_ctx* ctx;
   MtState* state;         // device
   DeviceBuffer bits;      // scratch of the normals calls
+  DeviceBuffer jump;      // skip-ahead: start window U, coefficient lists, offsets
+  long long jump_L = -1;  // substream length and count the lists in `jump` are for
+  int jump_K = 0;
 };
 
 namespace {
+
+// A call of at least kJumpMin outputs is split into K substreams of at least
+// kPerSub outputs (K <= kMaxSub), each started by a skip-ahead.
+constexpr long long kPerSub = 624LL * 64;
+constexpr long long kJumpMin = 2 * kPerSub;
+constexpr int kMaxSub = 256;
+
+// Sorted coefficient lists of x^(k L) mod phi, k = 1 .. K - 1, with their
+// offsets, cached per (L, K) for the process (the streams of every seed
+// share them).
+struct JumpLists {
+  std::vector<uint16_t> coef;
+  std::vector<int> off;  // K entries: list k - 1 is coef[off[k - 1] .. off[k])
+};
+std::mutex g_jump_mu;
+std::map<std::pair<long long, int>, JumpLists>& jump_cache() {
+  static std::map<std::pair<long long, int>, JumpLists> c;
+  return c;
+}
+
+int jump_lists(long long L, int K, const JumpLists** out) {
+  std::lock_guard<std::mutex> lock(g_jump_mu);
+  auto key = std::make_pair(L, K);
+  auto it = jump_cache().find(key);
+  if (it == jump_cache().end()) {
+    JumpLists jl;
+    jl.off.push_back(0);
+    const std::vector<uint64_t> step = gf2::xpow(gf2::kMT19937, (unsigned long long)L);
+    FSMC_REQUIRE(step.size() == (size_t)gf2::kWords, "MT19937 jump polynomial unavailable");
+    std::vector<uint64_t> p = step;
+    for (int k = 1; k < K; ++k) {
+      if (k > 1) p = gf2::mulmod(gf2::kMT19937, p, step);
+      for (int w = 0; w < gf2::kWords; ++w)
+        for (uint64_t v = p[w]; v; v &= v - 1) jl.coef.push_back((uint16_t)(64 * w + __builtin_ctzll(v)));
+      jl.off.push_back((int)jl.coef.size());
+    }
+    it = jump_cache().emplace(key, std::move(jl)).first;
+  }
+  *out = &it->second;
+  return FSMC_OK;
+}
+
+// n outputs by K substreams: the start window, then one workgroup per
+// substream (skip-ahead, then its outputs); the last leaves the state.
+int mt_jump_enqueue(fsmc_mt19937* g, long long n, uint32_t* d_out, hipStream_t s) {
+  const int K = (int)std::min<long long>(kMaxSub, n / kPerSub);
+  const long long L = (n + K - 1) / K;
+  const JumpLists* jl = nullptr;
+  FSMC_TRY(jump_lists(L, K, &jl));
+  const size_t u_bytes = align_up(kN * sizeof(uint32_t), 256);
+  const size_t off_bytes = align_up(jl->off.size() * sizeof(int), 256);
+  const size_t coef_bytes = jl->coef.size() * sizeof(uint16_t);
+  char* base = nullptr;
+  if (g->jump_L != L || g->jump_K != K || !g->jump.ptr) {
+    FSMC_TRY(g->jump.reserve(u_bytes + off_bytes + coef_bytes + 256));
+    base = (char*)g->jump.ptr;
+    // the lists are the same for every call of this (L, K): uploaded once, in order on s
+    FSMC_HIP(hipMemcpyAsync(base + u_bytes, jl->off.data(), jl->off.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    FSMC_HIP(hipMemcpyAsync(base + u_bytes + off_bytes, jl->coef.data(), coef_bytes, hipMemcpyHostToDevice, s));
+    FSMC_HIP(hipStreamSynchronize(s));  // the host vectors are cached, but keep the copy off the pageable path
+    g->jump_L = L;
+    g->jump_K = K;
+  }
+  base = (char*)g->jump.ptr;
+  uint32_t* U = (uint32_t*)base;
+  hipLaunchKernelGGL(mt_align_kernel, dim3(1), dim3(kBitsThreads), 0, s, g->state, U);
+  hipLaunchKernelGGL(mt_jump_bits_kernel, dim3(K), dim3(kJumpThreads), 0, s, U,
+                     (const uint16_t*)(base + u_bytes + off_bytes), (const int*)(base + u_bytes), L, n, d_out,
+                     g->state);
+  FSMC_HIP(hipGetLastError());
+  return FSMC_OK;
+}
 
 int mt_validate(long long n, const void* out) {
   FSMC_REQUIRE(n >= 0 && n <= (1LL << 40), "n %lld outside [0, 2^40]", n);
@@ -178,6 +362,7 @@ int mt_validate(long long n, const void* out) {
 
 int mt_bits_enqueue(fsmc_mt19937* g, long long n, uint32_t* d_out, hipStream_t s) {
   if (n == 0) return FSMC_OK;
+  if (n >= kJumpMin) return mt_jump_enqueue(g, n, d_out, s);
   hipLaunchKernelGGL(mt_bits_kernel, dim3(1), dim3(kBitsThreads), 0, s, g->state, n, d_out);
   FSMC_HIP(hipGetLastError());
   return FSMC_OK;
@@ -270,6 +455,7 @@ int fsmc_mt19937_destroy(fsmc_mt19937* g) {
   (void)hipDeviceSynchronize();
   if (g->state) (void)hipFree(g->state);
   g->bits.release();
+  g->jump.release();
   delete g;
   return FSMC_OK;
 }

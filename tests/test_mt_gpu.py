@@ -30,6 +30,27 @@ def test_bits_chain_across_calls(engine):
     g.close()
 
 
+def test_skip_ahead_calls_equal_the_sequential_stream(engine):
+    # calls of at least 2 x 624 x 64 outputs are split over substreams that
+    # each start by a skip-ahead (x^(kL) mod phi): the concatenated outputs
+    # of a chain of small and large calls equal the oracle's sequential
+    # stream bit for bit, and every call continues where the last one ended
+    for seed in (42, 7):
+        g = engine.mt19937(seed)
+        sizes = (5, 79872, 1000, 300001, 624, 2000000, 3)
+        print(f"mt19937 skip-ahead chain seed {seed}: {sizes}", flush=True)
+        got = np.concatenate([g.bits(n) for n in sizes])
+        g.close()
+        want = orc.mt19937_bits(seed, 0, got.size)
+        assert np.array_equal(got, want)
+    # normals of a large call = the oracle's Box-Muller on the same stream
+    g = engine.mt19937(43)
+    z = g.gaussian(600000)
+    g.close()
+    oz = orc.mt19937_gaussian(43, 0, 600000)
+    assert np.max(np.abs(z - oz) / np.maximum(np.abs(oz), 1.0)) < 1e-13
+
+
 def test_gaussian_chain_and_async(engine):
     import torch
     g = engine.mt19937(42)
