@@ -26,11 +26,12 @@ struct EuropeanBuffers {
   double *in = nullptr, *out = nullptr;
   float* samples = nullptr;
   void* ws = nullptr;
+  void* skip = nullptr;  // SFMT19937 skip-ahead workspace
   hipStream_t s = nullptr;
   ~EuropeanBuffers() {
     if (s) (void)hipStreamSynchronize(s);
     if (s) (void)hipStreamDestroy(s);
-    for (void* p : {(void*)in, (void*)out, (void*)samples, ws})
+    for (void* p : {(void*)in, (void*)out, (void*)samples, ws, skip})
       if (p) (void)hipFree(p);
   }
 };
@@ -82,9 +83,12 @@ bool price_sample_stream(const fsmc_gbm_params& p, int precision, unsigned long 
               "hipMemcpyAsync"))
     return false;
   const double* d = b.in;
-  if (!fsmc_ok(fsmc_sfmt19937_gaussian_async((uint32_t)seed, threads, (long long)(z / threads), p.n_paths,
-                                             d + 3 * n, d + 4 * n, b.samples, b.s),
-               "fsmc_sfmt19937_gaussian_async") ||
+  // each sample thread's stream spread over substreams started by skip-ahead
+  const size_t skip_bytes = fsmc_sfmt19937_workspace(threads, (long long)(z / threads));
+  if (skip_bytes && !hip_ok(hipMalloc(&b.skip, skip_bytes), "hipMalloc")) return false;
+  if (!fsmc_ok(fsmc_sfmt19937_gaussian_skip_async((uint32_t)seed, threads, (long long)(z / threads), p.n_paths,
+                                                  d + 3 * n, d + 4 * n, b.samples, b.skip, skip_bytes, b.s),
+               "fsmc_sfmt19937_gaussian_skip_async") ||
       !fsmc_ok(fsmc_gbm_european_async(&p, &m, d, d + n, d + 2 * n, b.samples, b.out, b.out + n, nullptr, b.ws,
                                        ws_bytes, b.s),
                "fsmc_gbm_european_async") ||

@@ -148,6 +148,11 @@ def lib():
                                               ctypes.c_double, _F]
         L.fsmc_sfmt19937_gaussian_async.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_longlong,
                                                     ctypes.c_longlong, _VP, _VP, _VP, _VP]
+        L.fsmc_sfmt19937_workspace.restype = ctypes.c_size_t
+        L.fsmc_sfmt19937_workspace.argtypes = [ctypes.c_int, ctypes.c_longlong]
+        L.fsmc_sfmt19937_gaussian_skip_async.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_longlong,
+                                                         ctypes.c_longlong, _VP, _VP, _VP, _VP, ctypes.c_size_t,
+                                                         _VP]
         _I = ctypes.POINTER(ctypes.c_int)
         L.fsmc_lsmc_plan_create.argtypes = [_VP, ctypes.c_int, PM] + [_D] * 5 + [_I, _I, ctypes.POINTER(_VP)]
         L.fsmc_lsmc_plan_workspace.restype = ctypes.c_size_t
@@ -567,6 +572,24 @@ def sfmt19937_gaussian_async(seed0, n_streams, n_per_stream, out, seg=1, mean=No
     _need(stddev, "float64", nseg, "stddev", where)
     _check(lib().fsmc_sfmt19937_gaussian_async(seed0, n_streams, n_per_stream, seg, _ptr(mean), _ptr(stddev),
                                                _ptr(out), _stream()), where)
+
+
+def sfmt19937_workspace(n_streams, n_per_stream):
+    return lib().fsmc_sfmt19937_workspace(n_streams, n_per_stream)
+
+
+def sfmt19937_gaussian_skip_async(seed0, n_streams, n_per_stream, out, workspace, seg=1, mean=None, stddev=None):
+    """As sfmt19937_gaussian_async, each stream spread over substreams started
+    by skip-ahead; workspace: uint8 tensor of sfmt19937_workspace(...) bytes."""
+    where = "fsmc_sfmt19937_gaussian_skip_async"
+    total = n_streams * n_per_stream
+    _need(out, "float32", total, "out", where)
+    nseg = (total + seg - 1) // seg if seg > 0 else 0
+    _need(mean, "float64", nseg, "mean", where)
+    _need(stddev, "float64", nseg, "stddev", where)
+    ws_bytes = _workspace(workspace, sfmt19937_workspace(n_streams, n_per_stream), where)
+    _check(lib().fsmc_sfmt19937_gaussian_skip_async(seed0, n_streams, n_per_stream, seg, _ptr(mean), _ptr(stddev),
+                                                    _ptr(out), _ptr(workspace), ws_bytes, _stream()), where)
 
 
 def gbm_european_async(S, X, T, r, sigma, n_paths, m, out_res, out_conf, workspace, samples=None,

@@ -253,6 +253,14 @@ int fsmc_jump_poly(int generator, unsigned long long steps, uint64_t* coeffs, in
  * seeded in place); nothing is allocated. */
 int fsmc_sfmt19937_gaussian_async(uint32_t seed0, int n_streams, long long n_per_stream, long long seg,
                                   const double* d_mean, const double* d_stddev, float* d_out, void* stream);
+/* The same outputs with each stream split into up to 1024 substreams that
+ * start by a skip-ahead (fsmc_jump_poly), so that one stream spreads over
+ * many workgroups; d_workspace holds fsmc_sfmt19937_workspace(n_streams,
+ * n_per_stream) bytes (0: the stream is short and runs sequentially). */
+size_t fsmc_sfmt19937_workspace(int n_streams, long long n_per_stream);
+int fsmc_sfmt19937_gaussian_skip_async(uint32_t seed0, int n_streams, long long n_per_stream, long long seg,
+                                       const double* d_mean, const double* d_stddev, float* d_out, void* d_workspace,
+                                       size_t workspace_bytes, void* stream);
 /* One stream's first n values, mean + stddev z, to host memory. */
 int fsmc_sfmt19937_gaussian(fsmc_ctx* ctx, uint32_t seed, long long n, double mean, double stddev, float* out);
 

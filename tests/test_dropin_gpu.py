@@ -207,6 +207,34 @@ def test_sfmt19937_generator(engine):
         assert np.max(np.abs(got[t] - want) / np.maximum(np.abs(want), 1e-3)) < 2e-7
 
 
+def test_sfmt19937_skip_ahead_equals_sequential(engine):
+    # the skip-ahead path splits each stream into substreams whose start
+    # states come from jumps (x^(2^r L / 4) mod phi, doubling): its float
+    # normals are bit for bit those of the sequential kernel, and the first
+    # ones those of the oracle's SFMT19937 stream
+    import torch
+    from scipy.special import ndtri
+    dev = torch.device("cuda:0")
+    for n_streams, per, seg in ((1, 2_000_003, 2_000_003), (3, 700_000, 350_000)):
+        print(f"sfmt19937 skip-ahead: {n_streams} streams x {per}", flush=True)
+        nseg = n_streams * per // seg
+        mean = torch.linspace(-0.5, 0.5, nseg, dtype=torch.float64, device=dev)
+        sd = torch.linspace(1.0, 2.0, nseg, dtype=torch.float64, device=dev)
+        seq = torch.zeros(n_streams * per, dtype=torch.float32, device=dev)
+        mc.sfmt19937_gaussian_async(777, n_streams, per, seq, seg, mean, sd)
+        ws_bytes = mc.sfmt19937_workspace(n_streams, per)
+        assert ws_bytes > 0
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        skip = torch.full((n_streams * per,), float("nan"), dtype=torch.float32, device=dev)
+        mc.sfmt19937_gaussian_skip_async(777, n_streams, per, skip, ws, seg, mean, sd)
+        torch.cuda.synchronize()
+        a, b = seq.cpu().numpy(), skip.cpu().numpy()
+        assert np.array_equal(a, b)
+        e = ndtri(orc.sfmt19937_icdf_uniforms(777, 20000))
+        want = mean.cpu().numpy()[0] + sd.cpu().numpy()[0] * e
+        assert np.max(np.abs(b[:20000] - want) / np.maximum(np.abs(want), 1e-3)) < 2e-7
+
+
 def test_sfmt19937_sample_scaled_draws_against_mkl(engine):
     # the European sample's draws, vsRngGaussian(ICDF, ..., MuByT, VBySqrtT)
     # on stream 777 in three consecutive blocks, against MKL's own output
