@@ -42,13 +42,14 @@ R, RHO = 0.03, -0.7
 FLOP_PER_PATH_STEP = 17 + 2 * 6
 FP32_VECTOR_PEAK_TFLOPS = 157.3      # datasheet, packed v_pk_fma_f32
 FP64_VECTOR_PEAK_TFLOPS = 78.6
-# Files whose text fixes the Heston path kernel's code object; the committed
+# Device sources of the Heston path kernel; with the device compile flags
+# (the COMMON list of build.py) they fix its code object.  The committed
 # counter profile records their SHA-256, so a line computed from counters of
 # another build says so.
 KERNEL_SOURCES = ["financial-services-workload-samples_amd/csrc/heston.hip",
                   "financial-services-workload-samples_amd/csrc/path_models.hpp",
-                  "financial-services-workload-samples_amd/csrc/fsmc_device.hpp",
-                  "financial-services-workload-samples_amd/build.py"]
+                  "financial-services-workload-samples_amd/csrc/fsmc_device.hpp"]
+BUILD_SCRIPT = "financial-services-workload-samples_amd/build.py"
 COUNTER_PROFILE = "profiles/heston_asian_fp32_64x131072x365_counters.txt"
 PROBE_KINDS = ["fma_f32", "exp_f32", "log_f32", "sqrt_f32", "rcp_f32", "sin_f32", "mad_u64_u32", "fma_f64",
                "mix_7fma_1exp", "fma_f32_vop3", "add_f32", "cvt_f32_u32"]
@@ -67,11 +68,21 @@ def _params(np):
 
 
 def kernel_source_sha256():
+    """SHA-256 over the kernel's device sources and the text of the
+    assignment of build.py's device compile flags (COMMON)."""
+    import ast
     import hashlib
     h = hashlib.sha256()
     for rel in KERNEL_SOURCES:
         with open(os.path.join(ROOT, rel), "rb") as f:
             h.update(f.read())
+    with open(os.path.join(ROOT, BUILD_SCRIPT)) as f:
+        text = f.read()
+    flags = [ast.get_source_segment(text, node) for node in ast.parse(text).body
+             if isinstance(node, ast.Assign) and any(getattr(t, "id", None) == "COMMON" for t in node.targets)]
+    if len(flags) != 1:
+        raise RuntimeError("build.py: no single COMMON flag list")
+    h.update(flags[0].encode())
     return h.hexdigest()
 
 
