@@ -152,14 +152,21 @@ __global__ __launch_bounds__(kBitsThreads) void mt_bits_kernel(MtState* st, long
 // W[t] = XOR over the set coefficients i of x^J mod phi of X'_{i + t}, with
 // X' the sequence continued from U (tests/test_jump_poly.py checks the
 // identity on the oracle's stream).  Substream k of a call starts
-// J = k L words after U, so workgroup k regenerates X' into an LDS ring,
+// J = k L words after U, so workgroup k regenerates X' into an LDS ring and
 // correlates it with the coefficient list of x^(kL) mod phi (host-computed,
-// sorted, read through the scalar cache: every thread takes the same i),
-// and then emits its L outputs from W.
+// sorted), piece by piece: piece p holds the coefficients in
+// [p kPiece, (p + 1) kPiece), short enough that X'_{i .. i + 623} of all of
+// them is in the ring at once.  Each piece's coefficients are first copied
+// to LDS (coalesced), then every thread reads them eight at a time (one
+// broadcast 16-byte read) and XORs the eight ring words they select, so the
+// loop has no global-memory wait per coefficient.  The workgroup then emits
+// its L outputs from W.
 constexpr int kJumpThreads = 640;  // thread t < 624 owns window word t
 constexpr int kRing = 4096;        // LDS ring of raw words (16 KB)
 constexpr int kRingMask = kRing - 1;
 constexpr int kLag = kN - kM;      // 227 words can be generated at once
+constexpr int kPiece = kRing - kN - kLag;                // 3245 coefficient values per piece
+constexpr int kPieces = (19937 + kPiece - 1) / kPiece;  // 7 pieces cover x^0 .. x^19936
 
 // X[s & mask] = twist of its three predecessors for s in [g, g + 227)
 __device__ __forceinline__ void mt_ring_chunk(uint32_t* X, long long g) {
@@ -170,6 +177,22 @@ __device__ __forceinline__ void mt_ring_chunk(uint32_t* X, long long g) {
         twist_word(X[(s - kN) & kRingMask], X[(s - kN + 1) & kRingMask], X[(s - kLag) & kRingMask]);
   }
   __syncthreads();
+}
+
+// XOR of X[(c + t) & mask] over the n coefficients c in C (16-byte aligned).
+__device__ __forceinline__ uint32_t ring_correlate(const uint32_t* X, const uint16_t* C, int n, int t) {
+  uint32_t a0 = 0, a1 = 0;
+  const uint4* C8 = reinterpret_cast<const uint4*>(C);
+  int j = 0;
+  for (; j + 8 <= n; j += 8) {
+    const uint4 v = C8[j >> 3];
+    a0 ^= X[((v.x & 0xffffu) + t) & kRingMask] ^ X[((v.x >> 16) + t) & kRingMask];
+    a1 ^= X[((v.y & 0xffffu) + t) & kRingMask] ^ X[((v.y >> 16) + t) & kRingMask];
+    a0 ^= X[((v.z & 0xffffu) + t) & kRingMask] ^ X[((v.z >> 16) + t) & kRingMask];
+    a1 ^= X[((v.w & 0xffffu) + t) & kRingMask] ^ X[((v.w >> 16) + t) & kRingMask];
+  }
+  for (; j < n; ++j) a0 ^= X[(C[j] + t) & kRingMask];
+  return a0 ^ a1;
 }
 
 // The window starting at the next output's raw word, from the state (mt, pos):
@@ -195,38 +218,35 @@ __global__ __launch_bounds__(kBitsThreads) void mt_align_kernel(const MtState* s
 
 // Workgroup k emits outputs [k L, min(n, (k + 1) L)) of the call into out;
 // the last one also leaves the stream's state after output n - 1 in *st.
-// poly_off[k - 1] .. poly_off[k] - 1 index the sorted coefficients of
-// x^(k L) mod phi in poly (values < 19937).
+// The coefficients of x^(k L) mod phi (values < 19937, ascending) are
+// poly[poff[(k - 1) (kPieces + 1)] ..): piece p of them is
+// poly[poff[(k - 1) (kPieces + 1) + p] .. poff[(k - 1) (kPieces + 1) + p + 1]).
 __global__ __launch_bounds__(kJumpThreads) void mt_jump_bits_kernel(const uint32_t* U, const uint16_t* poly,
-                                                                     const int* poly_off, long long L, long long n,
+                                                                     const int* poff, long long L, long long n,
                                                                      uint32_t* out, MtState* st) {
   __shared__ uint32_t X[kRing];
+  __shared__ __align__(16) uint16_t C[kPiece + 8];
   const int t = threadIdx.x;
   const int k = blockIdx.x;
   if (t < kN) X[t] = U[t];
   __syncthreads();
   if (k > 0) {
-    // W[t] = XOR of X'_{i + t} over the coefficients i, taken in ascending
-    // order in pieces of at most kRing - 624 so that X'_{i .. i + 623} is still
-    // in the ring when i is used
+    // W[t] = XOR of X'_{i + t} over the coefficients i
     uint32_t acc = 0;
     long long G = kN;  // X'_s is in the ring for s < G
-    const int q1 = poly_off[k];
-    int q = poly_off[k - 1];
-    while (q < q1) {
-      const int a = poly[q];
-      const int lim = a + (kRing - kN - kLag);  // the coefficients of this piece are < lim
-      while (G < (long long)lim + kN) {
+    const int* po = poff + (size_t)(k - 1) * (kPieces + 1);
+    for (int p = 0; p < kPieces; ++p) {
+      const int q0 = po[p], nq = po[p + 1] - q0;  // uniform
+      if (nq == 0) continue;
+      for (int i = t; i < nq; i += kJumpThreads) C[i] = poly[q0 + i];
+      // X'_s for s < (p + 1) kPiece + 623 is needed; the oldest word still read is p kPiece
+      while (G < (long long)(p + 1) * kPiece + kN) {
         mt_ring_chunk(X, G);
         G += kLag;
       }
-      if (t < kN) {
-        int qq = q;
-        for (; qq < q1 && (int)poly[qq] < lim; ++qq) acc ^= X[(poly[qq] + t) & kRingMask];
-      }
-      // every thread advances q the same way (uniform)
-      while (q < q1 && (int)poly[q] < lim) ++q;
-      __syncthreads();  // the reads of this piece before the ring moves on
+      __syncthreads();  // C staged (and the last chunk written)
+      if (t < kN) acc ^= ring_correlate(X, C, nq, t);
+      __syncthreads();  // this piece's reads of X and C before either moves on
     }
     if (t < kN) X[t] = acc;
     __syncthreads();
@@ -288,12 +308,12 @@ constexpr long long kPerSub = 624LL * 64;
 constexpr long long kJumpMin = 2 * kPerSub;
 constexpr int kMaxSub = 256;
 
-// Sorted coefficient lists of x^(k L) mod phi, k = 1 .. K - 1, with their
-// offsets, cached per (L, K) for the process (the streams of every seed
-// share them).
+// Sorted coefficient lists of x^(k L) mod phi, k = 1 .. K - 1, with the
+// offsets of their pieces, cached per (L, K) for the process (the streams of
+// every seed share them).
 struct JumpLists {
   std::vector<uint16_t> coef;
-  std::vector<int> off;  // K entries: list k - 1 is coef[off[k - 1] .. off[k])
+  std::vector<int> poff;  // (K - 1) (kPieces + 1) entries: piece p of list k - 1 starts at poff[(k - 1) (kPieces + 1) + p]
 };
 std::mutex g_jump_mu;
 std::map<std::pair<long long, int>, JumpLists>& jump_cache() {
@@ -307,15 +327,25 @@ int jump_lists(long long L, int K, const JumpLists** out) {
   auto it = jump_cache().find(key);
   if (it == jump_cache().end()) {
     JumpLists jl;
-    jl.off.push_back(0);
     const std::vector<uint64_t> step = gf2::xpow(gf2::kMT19937, (unsigned long long)L);
     FSMC_REQUIRE(step.size() == (size_t)gf2::kWords, "MT19937 jump polynomial unavailable");
     std::vector<uint64_t> p = step;
     for (int k = 1; k < K; ++k) {
       if (k > 1) p = gf2::mulmod(gf2::kMT19937, p, step);
+      int piece = 0;
       for (int w = 0; w < gf2::kWords; ++w)
-        for (uint64_t v = p[w]; v; v &= v - 1) jl.coef.push_back((uint16_t)(64 * w + __builtin_ctzll(v)));
-      jl.off.push_back((int)jl.coef.size());
+        for (uint64_t v = p[w]; v; v &= v - 1) {
+          const int c = 64 * w + __builtin_ctzll(v);
+          while (piece <= c / kPiece) {
+            jl.poff.push_back((int)jl.coef.size());
+            ++piece;
+          }
+          jl.coef.push_back((uint16_t)c);
+        }
+      while (piece <= kPieces) {
+        jl.poff.push_back((int)jl.coef.size());
+        ++piece;
+      }
     }
     it = jump_cache().emplace(key, std::move(jl)).first;
   }
@@ -331,14 +361,15 @@ int mt_jump_enqueue(fsmc_mt19937* g, long long n, uint32_t* d_out, hipStream_t s
   const JumpLists* jl = nullptr;
   FSMC_TRY(jump_lists(L, K, &jl));
   const size_t u_bytes = align_up(kN * sizeof(uint32_t), 256);
-  const size_t off_bytes = align_up(jl->off.size() * sizeof(int), 256);
+  const size_t off_bytes = align_up(jl->poff.size() * sizeof(int), 256);
   const size_t coef_bytes = jl->coef.size() * sizeof(uint16_t);
   char* base = nullptr;
   if (g->jump_L != L || g->jump_K != K || !g->jump.ptr) {
     FSMC_TRY(g->jump.reserve(u_bytes + off_bytes + coef_bytes + 256));
     base = (char*)g->jump.ptr;
     // the lists are the same for every call of this (L, K): uploaded once, in order on s
-    FSMC_HIP(hipMemcpyAsync(base + u_bytes, jl->off.data(), jl->off.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    FSMC_HIP(hipMemcpyAsync(base + u_bytes, jl->poff.data(), jl->poff.size() * sizeof(int), hipMemcpyHostToDevice,
+                            s));
     FSMC_HIP(hipMemcpyAsync(base + u_bytes + off_bytes, jl->coef.data(), coef_bytes, hipMemcpyHostToDevice, s));
     FSMC_HIP(hipStreamSynchronize(s));  // the host vectors are cached, but keep the copy off the pageable path
     g->jump_L = L;

@@ -208,14 +208,18 @@ def test_sfmt19937_generator(engine):
 
 
 def test_sfmt19937_skip_ahead_equals_sequential(engine):
-    # the skip-ahead path splits each stream into substreams whose start
-    # states come from jumps (x^(2^r L / 4) mod phi, doubling): its float
-    # normals are bit for bit those of the sequential kernel, and the first
-    # ones those of the oracle's SFMT19937 stream
+    # the skip-ahead path splits each stream into K = B0 B1 substreams whose
+    # start states come from jumps (round 0: x^(m B1 L / 4) mod phi from the
+    # seeded state; round 1: x^(j L / 4) from those): its float normals are
+    # bit for bit those of the sequential kernel, and the first ones those
+    # of the oracle's SFMT19937 stream.  Cases: K = 32 (one round), 16 on
+    # three streams, 128 = 16 x 8 with segments shorter than a block,
+    # 512 = 32 x 16 on two streams, 1024 = 32 x 32
     import torch
     from scipy.special import ndtri
     dev = torch.device("cuda:0")
-    for n_streams, per, seg in ((1, 2_000_003, 2_000_003), (3, 700_000, 350_000)):
+    for n_streams, per, seg in ((1, 2_000_003, 2_000_003), (3, 700_000, 350_000), (1, 4_999_800, 300),
+                                (2, 21_000_000, 1_000_000), (1, 41_000_000, 41_000_000)):
         print(f"sfmt19937 skip-ahead: {n_streams} streams x {per}", flush=True)
         nseg = n_streams * per // seg
         mean = torch.linspace(-0.5, 0.5, nseg, dtype=torch.float64, device=dev)
@@ -231,7 +235,8 @@ def test_sfmt19937_skip_ahead_equals_sequential(engine):
         a, b = seq.cpu().numpy(), skip.cpu().numpy()
         assert np.array_equal(a, b)
         e = ndtri(orc.sfmt19937_icdf_uniforms(777, 20000))
-        want = mean.cpu().numpy()[0] + sd.cpu().numpy()[0] * e
+        k = np.arange(20000) // seg
+        want = mean.cpu().numpy()[k] + sd.cpu().numpy()[k] * e
         assert np.max(np.abs(b[:20000] - want) / np.maximum(np.abs(want), 1e-3)) < 2e-7
 
 
