@@ -177,6 +177,7 @@ def _cpu_baseline(np, P, target_s):
                       seed=5)
     el1 = time.perf_counter() - t0
     orc.lib().orc_set_threads(prev)
+    mp = _cpu_one_process_per_core(P, p1)
     model = None
     try:
         for line in open("/proc/cpuinfo"):
@@ -187,11 +188,76 @@ def _cpu_baseline(np, P, target_s):
         pass
     return {"value": steps / el, "unit": "path-steps/s", "cores": threads, "kind": "port",
             "host": {"cpu": model, "logical_cpus": os.cpu_count(),
-                     "usable_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None},
+                     "usable_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+                     "threads_per_core": _threads_per_core()},
             "sample": f"FP64 oracle (C, OpenMP over options), {N_OPT} options x {paths} paths x {N_STEPS} steps "
                       f"of the same workload, {el:.1f} s",
             "single_thread": {"value": N_OPT * p1 * N_STEPS / el1, "unit": "path-steps/s", "cores": 1,
-                              "sample": f"{N_OPT} options x {p1} paths x {N_STEPS} steps, {el1:.1f} s"}}
+                              "sample": f"{N_OPT} options x {p1} paths x {N_STEPS} steps, {el1:.1f} s"},
+            "one_process_per_core": mp}
+
+
+def _threads_per_core():
+    """Hardware threads sharing core 0 (from sysfs), or None."""
+    try:
+        with open("/sys/devices/system/cpu/cpu0/topology/thread_siblings_list") as f:
+            n = 0
+            for part in f.read().strip().split(","):
+                a, _, b = part.partition("-")
+                n += int(b) - int(a) + 1 if b else 1
+            return n
+    except (OSError, ValueError):
+        return None
+
+
+_MP_CHILD = r"""
+import json, os, sys, time
+cpu, root, prm = int(sys.argv[1]), sys.argv[2], json.loads(sys.argv[3])
+os.sched_setaffinity(0, {cpu})
+sys.path.insert(0, os.path.join(root, "oracle"))
+import numpy as np
+import oracle as orc
+orc.lib().orc_set_threads(1)
+a = [np.array(prm[k]) for k in ("S", "X", "T", "v0", "kappa", "theta", "xi")]
+print("ready", flush=True)
+sys.stdin.readline()
+t0 = time.perf_counter()
+orc.heston_philox(*a, prm["r"], prm["rho"], prm["paths"], prm["steps"], seed=prm["seed"])
+print(time.perf_counter() - t0, flush=True)
+"""
+
+
+def _cpu_one_process_per_core(P, paths):
+    """The sample's MPTest layout: one single-threaded process pinned to each
+    usable core, all pricing the same-sized sample at once (each on its own
+    seed); aggregate path-steps/s = total work / the slowest process."""
+    import subprocess
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    prm = {k: [float(x) for x in P[k]] for k in ("S", "X", "T", "v0", "kappa", "theta", "xi")}
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    procs = []
+    try:
+        for i, c in enumerate(cpus):
+            arg = json.dumps(dict(prm, r=R, rho=RHO, paths=paths, steps=N_STEPS, seed=100 + i))
+            procs.append(subprocess.Popen([sys.executable, "-c", _MP_CHILD, str(c), ROOT, arg], env=env,
+                                          stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True))
+        for pr in procs:
+            if pr.stdout.readline().strip() != "ready":
+                raise RuntimeError("baseline process failed to start")
+        for pr in procs:
+            pr.stdin.write("go\n")
+            pr.stdin.flush()
+        times = [float(pr.stdout.readline()) for pr in procs]
+        for pr in procs:
+            pr.wait(timeout=60)
+    except (RuntimeError, ValueError, OSError, subprocess.TimeoutExpired) as e:
+        for pr in procs:
+            pr.kill()
+        return {"error": str(e)}
+    work = len(cpus) * N_OPT * paths * N_STEPS
+    return {"value": work / max(times), "unit": "path-steps/s", "cores": len(cpus), "processes": len(cpus),
+            "sample": f"{len(cpus)} processes x ({N_OPT} options x {paths} paths x {N_STEPS} steps), "
+                      f"slowest {max(times):.1f} s, fastest {min(times):.1f} s"}
 
 
 def _timed(torch, np, fn, steps, warmup):
@@ -528,6 +594,11 @@ def main():
         }
     if not args.no_secondary:
         rec["configs"] = _other_configs(np, torch, mc)
+        fma = rec["roofline"]["measured_fma_tflops"]
+        lib = rec["configs"].get("libor_greeks", {})
+        if fma and "achieved_tflops_fp32" in lib:
+            # against the non-packed FMA rate measured in this run (packed FP32 is not used)
+            lib["frac_measured_fma_rate"] = lib["achieved_tflops_fp32"] / fma
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = _cpu_baseline(np, P, args.cpu_seconds)
     print(json.dumps(rec), flush=True)
