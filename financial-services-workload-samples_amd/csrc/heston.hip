@@ -91,19 +91,41 @@ __global__ __launch_bounds__(64) void heston_finalize_kernel(int nbx, int n_path
   const int o = blockIdx.x;
   const double N = (double)n_paths;
   double s[2];
-  if (!running_val2) wave_combine<2>(partials + (size_t)o * nbx * 2, nbx, 2, s);  // uniform branch
+  double val = 0.0, val2 = 0.0;
+  if (running_val2) {  // uniform branch
+    // MCAsianOptions.cpp:184-201 in meaning: val accumulates the chunk
+    // means, val2 the square of the running val after each chunk.  One
+    // wavefront: lane l owns the contiguous chunks [l m, (l + 1) m); the
+    // running val before them is the exclusive prefix of the lanes' sums
+    // (a wave scan), so each lane walks its chunks once more adding the
+    // squares.  Differs from the sequential sum only in rounding order.
+    const int lane = threadIdx.x & 63;
+    const int nchunks = n_paths / kChunk;
+    const int m = (nchunks + 63) / 64;
+    const int c0 = min(nchunks, lane * m), c1 = min(nchunks, c0 + m);
+    const double* cs = chunks + (size_t)o * nchunks;
+    double loc = 0.0;
+    for (int ci = c0; ci < c1; ++ci) loc += cs[ci] * (1.0 / kChunk);
+    double incl = loc;  // inclusive scan over lanes
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const double up = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += up;
+    }
+    double run = __shfl_up(incl, 1, 64), sq = 0.0;  // exclusive prefix
+    if (lane == 0) run = 0.0;
+    for (int ci = c0; ci < c1; ++ci) {
+      run += cs[ci] * (1.0 / kChunk);
+      sq += run * run;
+    }
+    val2 = wave_sum(sq);
+    val = __shfl(incl, 63, 64);
+  } else {
+    wave_combine<2>(partials + (size_t)o * nbx * 2, nbx, 2, s);
+  }
   if (threadIdx.x != 0) return;
   const double exprt = exp(-r * T[o]);
   if (running_val2) {
-    // MCAsianOptions.cpp:184-201 verbatim in meaning: val accumulates chunk
-    // means, val2 accumulates the square of the running val.
-    const int nchunks = n_paths / kChunk;
-    double val = 0.0, val2 = 0.0;
-    const double* cs = chunks + (size_t)o * nchunks;
-    for (int ci = 0; ci < nchunks; ++ci) {
-      val += cs[ci] / kChunk;
-      val2 += val * val;
-    }
     result[o] = exprt * val * (1.0 / N) * kChunk;
     const double stddev = sqrt((N * val2 - val * val) * (1.0 / (N * (N - 1.0))));
     confidence[o] = exprt * 1.96 * stddev * (1.0 / sqrt(N));
