@@ -158,23 +158,47 @@ __global__ __launch_bounds__(kBitsThreads) void mt_bits_kernel(MtState* st, long
 // [p kPiece, (p + 1) kPiece), short enough that X'_{i .. i + 623} of all of
 // them is in the ring at once.  Each piece's coefficients are first copied
 // to LDS (coalesced), then every thread reads them eight at a time (one
-// broadcast 16-byte read) and XORs the eight ring words they select, so the
-// loop has no global-memory wait per coefficient.  The workgroup then emits
-// its L outputs from W.
+// broadcast 16-byte read) and XORs the ring words they select, so the loop
+// has no global-memory wait per coefficient.  The workgroup then emits its
+// L outputs from W.
+//
+// The ring advances 623 words per barrier: X'_s depends on X'_{s-624},
+// X'_{s-623} and X'_{s-227}, so thread t < 227 computes s = G + t, then
+// s + 227 (from its own word s) and, for t < 169, s + 454 -- every other
+// operand is older than G.  (Word G + 623 would need X'_G, another thread's
+// word of the same phase.)
 constexpr int kJumpThreads = 640;  // thread t < 624 owns window word t
 constexpr int kRing = 4096;        // LDS ring of raw words (16 KB)
 constexpr int kRingMask = kRing - 1;
-constexpr int kLag = kN - kM;      // 227 words can be generated at once
-constexpr int kPiece = kRing - kN - kLag;                // 3245 coefficient values per piece
+constexpr int kLag = kN - kM;      // 227 threads generate
+constexpr int kPhase = kN - 1;     // 623 words per phase
+constexpr int kPiece = kRing - kN - kPhase;            // 2849 coefficient values per piece
 constexpr int kPieces = (19937 + kPiece - 1) / kPiece;  // 7 pieces cover x^0 .. x^19936
 
-// X[s & mask] = twist of its three predecessors for s in [g, g + 227)
-__device__ __forceinline__ void mt_ring_chunk(uint32_t* X, long long g) {
+// X[s & mask] for s in [g, g + 623), then a barrier; with out, output s
+// (temper of X'_s) is also stored for s < m.  Every ring operand of the
+// phase is read before any word of it is written (they are all older than
+// g), so a phase waits on one round of LDS reads.
+__device__ __forceinline__ void mt_ring_phase(uint32_t* X, long long g, uint32_t* out, long long m) {
   const int t = threadIdx.x;
   if (t < kLag) {
     const long long s = g + t;
-    X[s & kRingMask] =
-        twist_word(X[(s - kN) & kRingMask], X[(s - kN + 1) & kRingMask], X[(s - kLag) & kRingMask]);
+    const bool third = t < kPhase - 2 * kLag;  // t < 169
+    const long long s3 = third ? s + 2 * kLag : s;  // a valid index either way
+    const uint32_t a1 = X[(s - kN) & kRingMask], b1 = X[(s - kN + 1) & kRingMask], c1 = X[(s - kLag) & kRingMask];
+    const uint32_t a2 = X[(s + kLag - kN) & kRingMask], b2 = X[(s + kLag - kN + 1) & kRingMask];
+    const uint32_t a3 = X[(s3 - kN) & kRingMask], b3 = X[(s3 - kN + 1) & kRingMask];
+    const uint32_t v1 = twist_word(a1, b1, c1);
+    const uint32_t v2 = twist_word(a2, b2, v1);
+    const uint32_t v3 = twist_word(a3, b3, v2);
+    X[s & kRingMask] = v1;
+    X[(s + kLag) & kRingMask] = v2;
+    if (third) X[s3 & kRingMask] = v3;
+    if (out) {
+      if (s < m) out[s] = temper(v1);
+      if (s + kLag < m) out[s + kLag] = temper(v2);
+      if (third && s3 < m) out[s3] = temper(v3);
+    }
   }
   __syncthreads();
 }
@@ -228,7 +252,7 @@ __global__ __launch_bounds__(kJumpThreads) void mt_jump_bits_kernel(const uint32
   __shared__ __align__(16) uint16_t C[kPiece + 8];
   const int t = threadIdx.x;
   const int k = blockIdx.x;
-  if (t < kN) X[t] = U[t];
+  for (int i = t; i < kN; i += kJumpThreads) X[i] = U[i];
   __syncthreads();
   if (k > 0) {
     // W[t] = XOR of X'_{i + t} over the coefficients i
@@ -241,10 +265,10 @@ __global__ __launch_bounds__(kJumpThreads) void mt_jump_bits_kernel(const uint32
       for (int i = t; i < nq; i += kJumpThreads) C[i] = poly[q0 + i];
       // X'_s for s < (p + 1) kPiece + 623 is needed; the oldest word still read is p kPiece
       while (G < (long long)(p + 1) * kPiece + kN) {
-        mt_ring_chunk(X, G);
-        G += kLag;
+        mt_ring_phase(X, G, nullptr, 0);
+        G += kPhase;
       }
-      __syncthreads();  // C staged (and the last chunk written)
+      __syncthreads();  // C staged (and the last phase written)
       if (t < kN) acc ^= ring_correlate(X, C, nq, t);
       __syncthreads();  // this piece's reads of X and C before either moves on
     }
@@ -253,13 +277,12 @@ __global__ __launch_bounds__(kJumpThreads) void mt_jump_bits_kernel(const uint32
   }
   const long long first = (long long)k * L;
   const long long m = first + L < n ? L : n - first;  // outputs of this substream
+  uint32_t* o = out + first;
+  for (int i = t; i < kN && i < m; i += kJumpThreads) o[i] = temper(X[i]);
   long long G = kN;
-  for (long long base = 0; base < m; base += kLag) {
-    while (G < base + kLag) {
-      mt_ring_chunk(X, G);
-      G += kLag;
-    }
-    if (t < kLag && base + t < m) out[first + base + t] = temper(X[(base + t) & kRingMask]);
+  while (G < m) {  // uniform
+    mt_ring_phase(X, G, o, m);
+    G += kPhase;
   }
   if (k == (int)gridDim.x - 1) {
     // the state after output n - 1: mt = X'_{m - 624 .. m - 1} with pos = 624
