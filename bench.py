@@ -579,6 +579,13 @@ def main():
     if not args.no_secondary:
         ms64, k64, p64, c64 = run(mc.mode(mc.FP64, seed=1), 3, 1)
         msg, kg, pg, cg = run(m32, 5, 1, with_greeks=True)
+        # the same Greeks by forward (tangent) propagation
+        tws = torch.empty(mc.heston_asian_tangent_workspace(N_OPT, N_PATHS, N_STEPS, R, RHO, m32), dtype=torch.uint8,
+                          device=dev)
+        tgreeks = torch.zeros_like(greeks)
+        mst = _timed(torch, np, lambda: mc.heston_asian_tangent_async(prm, N_PATHS, N_STEPS, R, RHO, m32, out, conf,
+                                                                      tgreeks, tws), 5, 1)
+        gdiff = float(torch.max(torch.abs(tgreeks[:, :3] - greeks[:, :3]) / greeks[:, 3:]).item())
         rel = float(np.max(np.abs(p32 - p64) / p64))
         rec["secondary"] = {
             "fp64_path_steps_per_s": path_steps / (ms64 * 1e-3),
@@ -591,6 +598,10 @@ def main():
             # the 3 Greeks of each of the 64 options
             "adjoint_option_greeks_per_s": N_OPT * 3 / (msg * 1e-3),
             "adjoint_path_steps_per_s": path_steps / (msg * 1e-3),
+            "tangent_fp32_ms": mst,
+            "tangent_path_greeks_per_s": N_OPT * N_PATHS * 3 / (mst * 1e-3),
+            "tangent_option_greeks_per_s": N_OPT * 3 / (mst * 1e-3),
+            "tangent_vs_adjoint_max_greek_diff_in_se": gdiff,
         }
     if not args.no_secondary:
         rec["configs"] = _other_configs(np, torch, mc)

@@ -4,8 +4,8 @@
 // Requests are served in micro-batches of `requests_per_launch` in arrival
 // order.  Each micro-batch is three launches over a table of (request, chunk)
 // work items: one grid of GBM items (pathwise Greeks), one grid of Heston
-// items (the reverse sweep of heston_adjoint_path, checkpoints in a slot per
-// launched thread), and a finalize grid that combines each request's item
+// items (heston_tangent_path: the three Greeks carried forward with each
+// path), and a finalize grid that combines each request's item
 // partials in item order (bitwise reproducible) into mean and standard error.
 // A HIP event after each micro-batch gives the per-request completion time;
 // an autotuner times chunk size x micro-batch size and keeps the fastest.
@@ -41,7 +41,6 @@ struct BatchArgs {
   const ReqDev* req;
   const int2* items;    // (request, chunk), GBM items then Heston items per micro-batch
   double* partials;     // [item][kNV]
-  void* ckpt;           // Heston checkpoints [segment][S|v][thread of the launch]
   int chunk;
   uint32_t k0, k1, w_gbm, w_heston;
 };
@@ -88,10 +87,10 @@ __device__ __forceinline__ void gbm_item(const BatchArgs& a, int item, double* l
   }
 }
 
-// One Heston item: the adjoint path of each of the chunk's paths, checkpoints
-// in this thread's slot ck (stride = threads of the launch).
+// One Heston item: the tangent path of each of the chunk's paths (price,
+// delta, vega, rho as the standalone tangent entry point computes them).
 template <typename Real>
-__device__ __forceinline__ void heston_item(const BatchArgs& a, int item, Real* ck, size_t stride, double* lds) {
+__device__ __forceinline__ void heston_item(const BatchArgs& a, int item, double* lds) {
   const int2 e = a.items[item];
   const ReqDev q = a.req[e.x];
   const int p0 = e.y * a.chunk;
@@ -101,8 +100,8 @@ __device__ __forceinline__ void heston_item(const BatchArgs& a, int item, Real* 
   double s[kNV] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int p = p0 + (int)threadIdx.x; p < p1; p += kBlock) {
     double g[3];
-    const double x = heston_adjoint_path<Real>(c, q.S0, q.K, q.T, q.v0, q.r, q.n_steps, (uint32_t)p, (uint32_t)e.x,
-                                               a.w_heston, a.k0, a.k1, ck, stride, g);
+    const double x = heston_tangent_path<Real>(c, q.S0, q.K, q.T, q.v0, q.r, q.n_steps, (uint32_t)p, (uint32_t)e.x,
+                                               a.w_heston, a.k0, a.k1, g);
     const double pr = disc * x;
     s[0] += pr;
     s[1] += pr * pr;
@@ -129,8 +128,7 @@ __global__ __launch_bounds__(kBlock) void batch_gbm_kernel(BatchArgs a, int firs
 template <typename Real>
 __global__ __launch_bounds__(kBlock) void batch_heston_kernel(BatchArgs a, int first_item) {
   __shared__ double lds[kNV * (kBlock / 64)];
-  heston_item<Real>(a, first_item + blockIdx.x, (Real*)a.ckpt + (size_t)blockIdx.x * kBlock + threadIdx.x,
-                    (size_t)gridDim.x * kBlock, lds);
+  heston_item<Real>(a, first_item + blockIdx.x, lds);
 }
 
 // Persistent variant: gridDim.x = CUs x workgroups per CU; workgroup g runs
@@ -141,15 +139,13 @@ __global__ __launch_bounds__(kBlock) void batch_heston_kernel(BatchArgs a, int f
 template <typename Real>
 __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(BatchArgs a, const int* sched, const int* off) {
   __shared__ double lds[kNV * (kBlock / 64)];
-  Real* ck = (Real*)a.ckpt + (size_t)blockIdx.x * kBlock + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * kBlock;
   const int j1 = off[blockIdx.x + 1];
   for (int j = off[blockIdx.x]; j < j1; ++j) {
     const int item = sched[j];
     if (a.req[a.items[item].x].kind == FSMC_REQ_GBM_EUROPEAN)  // uniform per workgroup
       gbm_item<Real>(a, item, lds);
     else
-      heston_item<Real>(a, item, ck, stride, lds);
+      heston_item<Real>(a, item, lds);
     __syncthreads();  // thread 0's reads of lds in block_sum before the next item's writes
   }
 }
@@ -172,8 +168,6 @@ batch_queue_kernel(BatchArgs a, const int* order, int n_items, int* head) {  // 
                                                                              // Heston grid gets
   __shared__ double lds[kNV * (kBlock / 64)];
   __shared__ int next;
-  Real* ck = (Real*)a.ckpt + (size_t)blockIdx.x * kBlock + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * kBlock;
   if (threadIdx.x == 0) next = atomicAdd(head, 1);
   __syncthreads();
   int j = next;
@@ -185,7 +179,7 @@ batch_queue_kernel(BatchArgs a, const int* order, int n_items, int* head) {  // 
     if (a.req[a.items[item].x].kind == FSMC_REQ_GBM_EUROPEAN)  // uniform per workgroup
       gbm_item<Real>(a, item, lds);
     else
-      heston_item<Real>(a, item, ck, stride, lds);
+      heston_item<Real>(a, item, lds);
     if (threadIdx.x == 0) next = ahead;
     __syncthreads();  // next is written, and thread 0's reads of lds in block_sum precede the next item's writes
     j = next;
@@ -237,13 +231,11 @@ struct BatchImpl {
   int n = 0;
   fsmc_mode mode;
   std::vector<fsmc_request> req;
-  int max_seg = 1;
   fsmc_batch_tuning tuning = {4096, 0, 0, 0, 0};
   std::vector<MicroBatch> waves;
-  int max_hes_grid = 0;
   int cus = 0;  // compute units (persistent grid = cus x workgroups_per_cu)
   DeviceBuffer ws;
-  size_t off_req = 0, off_items = 0, off_first = 0, off_nitems = 0, off_part = 0, off_res = 0, off_ckpt = 0,
+  size_t off_req = 0, off_items = 0, off_first = 0, off_nitems = 0, off_part = 0, off_res = 0,
          off_stamp = 0, off_sched = 0, off_soff = 0, off_qorder = 0, off_heads = 0;
   std::vector<hipEvent_t> done;  // one per micro-batch (stream mode)
   hipEvent_t start = nullptr;
@@ -329,7 +321,6 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
   std::vector<int2> items;
   std::vector<int> first(b.n), nitems(b.n);
   b.waves.clear();
-  b.max_hes_grid = 0;
   for (int r0 = 0; r0 < b.n; r0 += per) {
     MicroBatch mb;
     mb.r0 = r0;
@@ -356,7 +347,6 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
       } else {
         mb.hes_first = begin;
         mb.hes_count = (int)items.size() - begin;
-        b.max_hes_grid = std::max(b.max_hes_grid, mb.hes_count);
       }
     }
     b.waves.push_back(mb);
@@ -374,7 +364,7 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
         const fsmc_request& r = b.req[items[it].x];
         const int paths = std::min(t.chunk_paths, r.n_paths - items[it].y * t.chunk_paths);
         // one GBM path ~ 35 instructions, one Heston adjoint step ~ 110
-        cost.push_back({(double)paths * (r.kind == FSMC_REQ_GBM_EUROPEAN ? 1.0 : 3.0 * r.n_steps), it});
+        cost.push_back({(double)paths * (r.kind == FSMC_REQ_GBM_EUROPEAN ? 1.0 : 1.7 * r.n_steps), it});
       }
       std::stable_sort(cost.begin(), cost.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
       // the work queue hands the items out in this order
@@ -429,7 +419,6 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
     d.xi = r.xi;
     d.rho = r.rho;
   }
-  const size_t real = b.mode.precision == FSMC_FP64 ? 8 : 4;
   size_t off = 0;
   auto take = [&](size_t bytes) {
     const size_t at = off;
@@ -442,8 +431,6 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
   b.off_nitems = take(sizeof(int) * b.n);
   b.off_part = take(sizeof(double) * kNV * items.size());
   b.off_res = take(sizeof(double) * 8 * b.n);
-  const size_t ck_groups = (size_t)std::max(1, G > 0 ? G : b.max_hes_grid);
-  b.off_ckpt = take(real * 2 * (size_t)b.max_seg * ck_groups * kBlock);
   b.off_stamp = take(sizeof(unsigned long long) * (b.waves.size() + 1));
   b.off_sched = take(sizeof(int) * std::max<size_t>(1, sched.size()));
   b.off_soff = take(sizeof(int) * std::max<size_t>(1, soff.size()));
@@ -479,7 +466,6 @@ static int batch_enqueue(BatchImpl& b, hipStream_t s, bool stamps) {
   a.req = (const ReqDev*)(ws + b.off_req);
   a.items = (const int2*)(ws + b.off_items);
   a.partials = (double*)(ws + b.off_part);
-  a.ckpt = ws + b.off_ckpt;
   a.chunk = b.tuning.chunk_paths;
   a.k0 = (uint32_t)b.mode.seed;
   a.k1 = (uint32_t)(b.mode.seed >> 32);
@@ -540,7 +526,6 @@ int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_
   FSMC_REQUIRE(n >= 1, "empty batch");
   FSMC_TRY(validate_mode(m, false));
   FSMC_REQUIRE(m->quirks == 0, "quirks are not supported in batches");
-  int max_seg = 1;
   for (int i = 0; i < n; ++i) {
     const fsmc_request& r = req[i];
     FSMC_REQUIRE(r.kind == FSMC_REQ_GBM_EUROPEAN || r.kind == FSMC_REQ_HESTON_ASIAN, "request %d: bad kind %d", i,
@@ -553,7 +538,6 @@ int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_
     } else {
       FSMC_REQUIRE(r.n_steps >= 1 && r.v0 > 0 && r.rho >= -1 && r.rho <= 1,
                    "request %d: Heston needs n_steps >= 1, v0 > 0, |rho| <= 1", i);
-      max_seg = std::max(max_seg, (r.n_steps + kHestonSeg - 1) / kHestonSeg);
     }
   }
   fsmc_batch* b = new fsmc_batch();
@@ -562,7 +546,6 @@ int fsmc_batch_create(fsmc_ctx* ctx, const fsmc_request* req, int n, const fsmc_
   bi.n = n;
   bi.mode = *m;
   bi.req.assign(req, req + n);
-  bi.max_seg = max_seg;
   fsmc_batch_tuning t = {4096, 0, 0, 0, 0};
   if (tuning) {
     t = *tuning;

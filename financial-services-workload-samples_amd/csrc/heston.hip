@@ -137,6 +137,36 @@ __global__ __launch_bounds__(64) void heston_finalize_kernel(int nbx, int n_path
   }
 }
 
+// Tangent kernel: one path per lane through heston_tangent_path (the same
+// Greeks carried forward with the path; no checkpoints), partials as the
+// adjoint kernel's.
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void heston_tangent_kernel(HestonArgs a) {
+  __shared__ double lds[8 * (kBlock / 64)];
+  const int o = blockIdx.y;
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  double out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (p < a.n_paths) {
+    const HestonConsts<Real> c(a.T[o], a.n_steps, a.r, a.kappa[o], a.xi[o], a.theta[o], a.rho);
+    double g[3];
+    const double x = heston_tangent_path<Real>(c, a.S[o], a.X[o], a.T[o], a.v0[o], a.r, a.n_steps, (uint32_t)p,
+                                               (uint32_t)o, a.w, a.k0, a.k1, g);
+    out[0] = x;
+    out[1] = x * x;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      out[2 + 2 * k] = g[k];
+      out[3 + 2 * k] = g[k] * g[k];
+    }
+  }
+  block_sum<8>(out, lds);
+  if (threadIdx.x == 0) {
+    double* dst = a.partials + ((size_t)o * gridDim.x + blockIdx.x) * 8;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) dst[k] = out[k];
+  }
+}
+
 // Adjoint kernel: one path per lane through heston_adjoint_path (reverse
 // sweep over kHestonSeg-step checkpoints kept in the workspace as
 // [option][segment][S|v][path]); the forward sweep also yields the price.
@@ -274,7 +304,7 @@ int heston_check_args(const fsmc_heston_params* p, const fsmc_mode* m, const dou
 int heston_enqueue(const fsmc_heston_params* p, const fsmc_mode* m, const double* d_S, const double* d_X,
                    const double* d_T, const double* d_v0, const double* d_kappa, const double* d_theta,
                    const double* d_xi, const double* d_z1, const double* d_z2, double* d_result, double* d_conf,
-                   double* d_greeks, void* d_ws, hipStream_t s) {
+                   double* d_greeks, void* d_ws, hipStream_t s, bool tangent = false) {
   const bool buffer = m->rng == FSMC_RNG_BUFFER;
   const HestonPlan h = heston_plan(p, m);
   HestonArgs a;
@@ -296,6 +326,18 @@ int heston_enqueue(const fsmc_heston_params* p, const fsmc_mode* m, const double
   a.w = counter_w(kTagHeston, m->stream);
   a.partials = (double*)d_ws;
   a.chunks = (double*)((char*)d_ws + h.partial_bytes);
+  if (d_greeks && tangent) {
+    a.partials = (double*)d_ws;
+    if (m->precision == FSMC_FP64)
+      hipLaunchKernelGGL(heston_tangent_kernel<double>, dim3(h.nbx, p->n_options), dim3(kBlock), 0, s, a);
+    else
+      hipLaunchKernelGGL(heston_tangent_kernel<float>, dim3(h.nbx, p->n_options), dim3(kBlock), 0, s, a);
+    FSMC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(heston_greeks_finalize_kernel, dim3(p->n_options), dim3(64), 0, s, h.nbx, p->n_paths,
+                       p->riskfree, d_T, a.partials, d_result, d_conf, d_greeks);
+    FSMC_HIP(hipGetLastError());
+    return FSMC_OK;
+  }
   if (d_greeks) {
     // One launch: the adjoint kernel's forward sweep also yields the price.
     a.partials = (double*)d_ws;
@@ -380,6 +422,27 @@ int fsmc_heston_asian_async(const fsmc_heston_params* p, const fsmc_mode* m, con
   }
   return heston_enqueue(p, m, d_S, d_X, d_T, d_v0, d_kappa, d_theta, d_xi, d_z1, d_z2, d_result, d_conf, d_greeks,
                         d_ws, (hipStream_t)stream);
+}
+
+size_t fsmc_heston_asian_tangent_workspace(const fsmc_heston_params* p, const fsmc_mode* m) {
+  if (heston_validate(p, m) != FSMC_OK) return 0;
+  return heston_plan(p, m).gpartial_bytes;
+}
+
+int fsmc_heston_asian_tangent_async(const fsmc_heston_params* p, const fsmc_mode* m, const double* d_S,
+                                    const double* d_X, const double* d_T, const double* d_v0, const double* d_kappa,
+                                    const double* d_theta, const double* d_xi, double* d_result, double* d_conf,
+                                    double* d_greeks, void* d_ws, size_t ws_bytes, void* stream) {
+  FSMC_REQUIRE(d_greeks != nullptr, "the tangent entry point computes Greeks: d_greeks is NULL");
+  FSMC_TRY(heston_check_args(p, m, d_S, d_X, d_T, d_v0, d_kappa, d_theta, d_xi, nullptr, nullptr, d_result, d_conf,
+                             d_greeks));
+  const size_t need = fsmc_heston_asian_tangent_workspace(p, m);
+  if (d_ws == nullptr || ws_bytes < need) {
+    set_error("workspace %zu bytes < required %zu", ws_bytes, need);
+    return FSMC_ERR_WORKSPACE;
+  }
+  return heston_enqueue(p, m, d_S, d_X, d_T, d_v0, d_kappa, d_theta, d_xi, nullptr, nullptr, d_result, d_conf,
+                        d_greeks, d_ws, (hipStream_t)stream, true);
 }
 
 size_t fsmc_heston_asian_noalloc_limit(void) { return kNoallocScratchBytes; }

@@ -15,9 +15,14 @@ constexpr double kPathLog2e = 1.4426950408889634074;
 template <typename Real>
 struct HestonConsts {
   Real c_rdt, c_hdt, c_sdt, kdt, xsdt, theta, rho, srho, xr, xs;
+  Real dt_n, hdt_n, sdt_n, one_kdt;  // natural-log units, for the tangent sweep
   __device__ HestonConsts(double T, int n_steps, double r, double kappa, double xi, double th, double rh) {
     const double dt = T / n_steps;
     const double sdt = sqrt(dt);
+    dt_n = (Real)dt;
+    hdt_n = (Real)(0.5 * dt);
+    sdt_n = (Real)sdt;
+    one_kdt = (Real)(1.0 - kappa * dt);
     c_rdt = (Real)(r * dt * kPathLog2e);
     c_hdt = (Real)(0.5 * dt * kPathLog2e);
     c_sdt = (Real)(sdt * kPathLog2e);
@@ -221,6 +226,105 @@ __device__ __forceinline__ double heston_adjoint_path(const HestonConsts<Real>& 
   g[0] = (double)U / S0d;
   g[1] = (double)vbar * 2.0 * sqrt(v0d);
   g[2] = -Td * disc * x + dtd * (double)ebar_sum;
+  return x;
+}
+
+// Forward (tangent) Greeks of one Heston-Asian path: the three derivatives
+// the adjoint above computes, carried forward with the path instead of
+// recovered by a reverse sweep.  With vp = max(v, 0) and, on the vp > 0
+// branch, de/dv = sqrt(dt) z1 / (2 sqrt(vp)) - dt/2 and
+// dv'/dv = 1 - kappa dt + xi sqrt(dt) zc / (2 sqrt(vp)) (else 0 and 1):
+//   dS_{k+1} = dS_k e^{e_k} + S_{k+1} de/dv dv_k,  dv_{k+1} = dv_k dv'/dv,
+// seeded dS_0 = 0, dv_0 = 2 sqrt(v0) (the derivative with respect to
+// sqrt(v0)); the spot is linear in S0 (dS_k/dS0 = S_k / S0) and its log is
+// linear in r (dS_k/dr = S_k k dt), so those two need no recursion.  Then
+//   dP/dS0 = e^{-rT} A / S0,  dP/dsqrt(v0) = e^{-rT} (1/N) sum_k dS_k,
+//   dP/dr = e^{-rT} (1/N) sum_k S_k k dt - T P   (A > K; all zero else).
+// The path itself is stepped exactly as heston_philox_pair steps it, so the
+// price is bit for bit the forward kernel's.  Three Greeks cost a reciprocal
+// and ~12 instructions per step here, against a segment recompute and a
+// reverse step for the adjoint.  Returns the undiscounted payoff; g as the
+// adjoint's.
+template <typename Real>
+struct HestonTangent {
+  Real dS = 0, dv = 0, t = 0;  // dS_k/dsqrt(v0), dv_k/dsqrt(v0), k dt
+  PathSum dA, St;                // sum_k dS_k, sum_k S_k k dt
+};
+
+// One step of heston_step_polar (FP32, polar normals) or heston_step (FP64,
+// normals z1 = cs, z2 = sn, r2 = 1) with the tangent; the path arithmetic is
+// the same instructions as those steps'.  Returns the new spot's tangent
+// terms through tg; the caller adds the pair sums.
+template <typename Real>
+__device__ __forceinline__ void heston_step_tangent(const HestonConsts<Real>& c, Real& S, Real& v, Real r2, Real cs,
+                                                    Real sn, HestonTangent<Real>& tg, Real& dS_out, Real& St_out) {
+#pragma clang fp contract(off)
+  const Real vp = positive_part(v);
+  Real q, u;  // q: the step's sqrt(vp) times the radius; u = radius / (2 sqrt(vp))
+  if constexpr (sizeof(Real) == 4) {
+    q = fsqrt(vp * r2);
+    u = vp > Real(0) ? Real(0.5) * q * frcp(vp) : Real(0);
+  } else {
+    q = fsqrt(vp);  // r2 = 1: the normals carry their radius
+    u = vp > Real(0) ? Real(0.5) / q : Real(0);
+  }
+  const Real e = ffma(c.c_sdt * q, cs, ffma(-c.c_hdt, vp, c.c_rdt));
+  const Real zz = ffma(c.xr, cs, c.xs * sn);
+  const Real growth = fexp2(e);
+  const Real Sn = S * growth;
+  const Real dedv = vp > Real(0) ? ffma(c.sdt_n * u, cs, -c.hdt_n) : Real(0);
+  const Real dvdv = vp > Real(0) ? ffma(zz, u, c.one_kdt) : Real(1);
+  tg.dS = ffma(tg.dS, growth, Sn * (dedv * tg.dv));
+  tg.dv = tg.dv * dvdv;
+  tg.t = tg.t + c.dt_n;
+  dS_out = tg.dS;
+  St_out = Sn * tg.t;
+  S = Sn;
+  v = ffma(q, zz, ffma(c.kdt, c.theta - vp, v));
+}
+
+template <typename Real>
+__device__ __forceinline__ double heston_tangent_path(const HestonConsts<Real>& c, double S0d, double Kd, double Td,
+                                                      double v0d, double r, int N, uint32_t p, uint32_t c2,
+                                                      uint32_t w, uint32_t k0, uint32_t k1, double (&g)[3]) {
+#pragma clang fp contract(off)
+  Real S = (Real)S0d, v = (Real)v0d;
+  PathSum avg;
+  HestonTangent<Real> tg;
+  tg.dv = (Real)(2.0 * sqrt(v0d));
+  for (int k = 0; k < N; k += 2) {
+    const bool second = k + 1 < N;
+    Real ra, ca, sa, rb, cb, sb;  // two steps' (squared radius, cos, sin) or (1, z1, z2)
+    if constexpr (sizeof(Real) == 4) {
+      const U4 q = philox4x32_10(U4{p, (uint32_t)(k >> 1), c2, w}, k0, k1);
+      box_muller_polar(q.x, q.y, ra, ca, sa);
+      box_muller_polar(q.z, q.w, rb, cb, sb);
+    } else {
+      normals4<Real>(U4{p, (uint32_t)(k >> 1), c2, w}, k0, k1, ca, sa, cb, sb);
+      ra = rb = Real(1);
+    }
+    Real d0, s0, d1, s1;
+    heston_step_tangent(c, S, v, ra, ca, sa, tg, d0, s0);
+    if (second) {
+      const Real sa0 = S;
+      heston_step_tangent(c, S, v, rb, cb, sb, tg, d1, s1);
+      avg.add(sa0 + S);
+      tg.dA.add(d0 + d1);
+      tg.St.add(s0 + s1);
+    } else {
+      avg.add(S);
+      tg.dA.add(d0);
+      tg.St.add(s0);
+    }
+  }
+  g[0] = g[1] = g[2] = 0.0;
+  const double A = avg.value() / (double)N;
+  const double x = A - Kd;
+  if (!(x > 0.0)) return 0.0;
+  const double disc = exp(-r * Td);
+  g[0] = disc * A / S0d;
+  g[1] = disc * tg.dA.value() / (double)N;
+  g[2] = disc * tg.St.value() / (double)N - Td * disc * x;
   return x;
 }
 

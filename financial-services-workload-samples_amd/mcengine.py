@@ -119,6 +119,10 @@ def lib():
         L.fsmc_heston_asian_workspace.argtypes = [ctypes.POINTER(HestonParams), PM]
         L.fsmc_heston_asian_async.argtypes = [ctypes.POINTER(HestonParams), PM] + [_VP] * 12 + [
             _VP, ctypes.c_size_t, _VP]
+        L.fsmc_heston_asian_tangent_workspace.restype = ctypes.c_size_t
+        L.fsmc_heston_asian_tangent_workspace.argtypes = [ctypes.POINTER(HestonParams), PM]
+        L.fsmc_heston_asian_tangent_async.argtypes = [ctypes.POINTER(HestonParams), PM] + [_VP] * 10 + [
+            _VP, ctypes.c_size_t, _VP]
         L.fsmc_heston_asian_noalloc_limit.restype = ctypes.c_size_t
         L.fsmc_heston_asian_noalloc_limit.argtypes = []
         L.fsmc_heston_asian_noalloc.argtypes = [ctypes.POINTER(HestonParams), PM] + [_VP] * 12 + [_VP]
@@ -638,6 +642,26 @@ def heston_asian_async(params, n_paths, n_steps, r, rho, m, out_res, out_conf, w
     _check(lib().fsmc_heston_asian_async(ctypes.byref(p), ctypes.byref(m), *[_ptr(t) for t in params],
                                          _ptr(z1), _ptr(z2), _ptr(out_res), _ptr(out_conf), _ptr(out_greeks),
                                          _ptr(workspace), ws_bytes, _stream()), where)
+
+
+def heston_asian_tangent_workspace(n_options, n_paths, n_steps, r, rho, m):
+    p = HestonParams(n_options, n_paths, n_steps, r, rho)
+    return lib().fsmc_heston_asian_tangent_workspace(ctypes.byref(p), ctypes.byref(m))
+
+
+def heston_asian_tangent_async(params, n_paths, n_steps, r, rho, m, out_res, out_conf, out_greeks, workspace):
+    """fsmc_heston_asian_tangent_async: price and the three Greeks by forward
+    (tangent) propagation; params as heston_asian_async."""
+    where = "fsmc_heston_asian_tangent_async"
+    if out_greeks is None:
+        raise FsmcError(ERR_INVALID_ARG, where, "out_greeks is required")
+    n = _heston_check(params, n_paths, n_steps, out_res, out_conf, None, None, out_greeks, where)
+    p = HestonParams(n, n_paths, n_steps, r, rho)
+    ws_bytes = _workspace(workspace, lib().fsmc_heston_asian_tangent_workspace(ctypes.byref(p), ctypes.byref(m)),
+                          where)
+    _check(lib().fsmc_heston_asian_tangent_async(ctypes.byref(p), ctypes.byref(m), *[_ptr(t) for t in params],
+                                                 _ptr(out_res), _ptr(out_conf), _ptr(out_greeks), _ptr(workspace),
+                                                 ws_bytes, _stream()), where)
 
 
 def heston_asian_noalloc_limit():

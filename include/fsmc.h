@@ -142,6 +142,21 @@ int fsmc_heston_asian_async(const fsmc_heston_params* p, const fsmc_mode* m, con
                             const double* d_z1, const double* d_z2, double* d_call_result,
                             double* d_call_confidence, double* d_greeks, void* d_workspace,
                             size_t workspace_bytes, void* stream);
+/* The same three Greeks by forward (tangent) propagation: each path carries
+ * dS/dsqrt(v0) and dv/dsqrt(v0) along its steps (dS/dS0 = S/S0 and
+ * dS/dr = S k dt need no recursion), so no checkpoints are stored and no
+ * reverse sweep runs; the price is bit for bit the forward kernel's and the
+ * Greeks equal the adjoint's to rounding.  No counterpart in the samples:
+ * an engine option, cheaper than the adjoint for these three Greeks (the
+ * serving batch prices its Heston requests this way).  Philox mode without
+ * quirks; d_greeks [n_options][6] as above; the workspace holds the
+ * per-workgroup partials (fsmc_heston_asian_tangent_workspace bytes, at
+ * most fsmc_heston_asian_workspace's). */
+size_t fsmc_heston_asian_tangent_workspace(const fsmc_heston_params* p, const fsmc_mode* m);
+int fsmc_heston_asian_tangent_async(const fsmc_heston_params* p, const fsmc_mode* m, const double* d_S,
+                                    const double* d_X, const double* d_T, const double* d_v0, const double* d_kappa,
+                                    const double* d_theta, const double* d_xi, double* d_result, double* d_conf,
+                                    double* d_greeks, void* d_workspace, size_t workspace_bytes, void* stream);
 /* Allocation-free variant of fsmc_heston_asian_async.  It has no counterpart
  * in the reference: it is for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid

@@ -176,3 +176,74 @@ def test_errors(engine):
         _call(engine, P, .03, -.7, 128, 10, mc.mode(quirks=Q4), greeks=True)
     with pytest.raises(mc.FsmcError):
         _call(engine, P, .03, -.7, 128, 0, mc.mode())
+
+
+# ---- forward (tangent) Greeks: fsmc_heston_asian_tangent_async ----
+def _tangent(P, r, rho, n_paths, n_steps, m):
+    import torch
+    dev = torch.device("cuda:0")
+    prm = [torch.tensor(P[k], dtype=torch.float64, device=dev) for k in ("S", "X", "T", "v0", "kappa", "theta",
+                                                                          "xi")]
+    n = len(P["S"])
+    out = torch.zeros(n, dtype=torch.float64, device=dev)
+    oc = torch.zeros_like(out)
+    og = torch.zeros(n, 6, dtype=torch.float64, device=dev)
+    ws = torch.empty(mc.heston_asian_tangent_workspace(n, n_paths, n_steps, r, rho, m), dtype=torch.uint8,
+                     device=dev)
+    mc.heston_asian_tangent_async(prm, n_paths, n_steps, r, rho, m, out, oc, og, ws)
+    torch.cuda.synchronize()
+    return out.cpu().numpy(), oc.cpu().numpy(), og.cpu().numpy()
+
+
+def test_tangent_fp64_matches_oracle_and_adjoint(engine):
+    # the oracle computes these Greeks by the same forward recursion: FP64
+    # agrees to rounding; the adjoint (reverse sweep) to its own rounding
+    P = _params(4, 7)
+    n_paths, n_steps = 2000, 37
+    m = mc.mode(mc.FP64, seed=12)
+    res, conf, g = _tangent(P, .03, -.7, n_paths, n_steps, m)
+    ores, oconf, og = orc.heston_philox(P["S"], P["X"], P["T"], P["v0"], P["kappa"], P["theta"], P["xi"], .03,
+                                        -.7, n_paths, n_steps, seed=12, greeks=True)
+    assert _rel(res, ores) <= 1e-12
+    assert _rel(g[:, :3], og[:, :3]) <= 1e-11
+    assert _rel(g[:, 3:], og[:, 3:]) <= 1e-9
+    _, _, ga = _call(engine, P, .03, -.7, n_paths, n_steps, m, greeks=True)
+    assert _rel(g[:, :3], ga[:, :3]) <= 1e-9
+    # the path is stepped exactly as the forward kernel steps it (the
+    # confidence goes through the Greeks finalize's arithmetic: to rounding)
+    fres, fconf = _call(engine, P, .03, -.7, n_paths, n_steps, m)
+    assert np.array_equal(fres, res) and _rel(conf, fconf) <= 1e-13
+
+
+def test_tangent_fp32(engine):
+    P = _params(8, 9)
+    n_paths, n_steps = 1 << 15, 365
+    r32, c32, g32 = _tangent(P, .03, -.7, n_paths, n_steps, mc.mode(mc.FP32, seed=3))
+    _, _, g64 = _tangent(P, .03, -.7, n_paths, n_steps, mc.mode(mc.FP64, seed=3))
+    # FP32 Greek error far below their standard error, and the FP32 adjoint's
+    assert np.all(np.abs(g32[:, :3] - g64[:, :3]) < 0.02 * g64[:, 3:])
+    _, _, ga32 = _call(engine, P, .03, -.7, n_paths, n_steps, mc.mode(mc.FP32, seed=3), greeks=True)
+    assert np.all(np.abs(g32[:, :3] - ga32[:, :3]) < 0.02 * g64[:, 3:])
+    # FP32 price: bit for bit the forward kernel's
+    f32, fc32 = _call(engine, P, .03, -.7, n_paths, n_steps, mc.mode(mc.FP32, seed=3))
+    assert np.array_equal(f32, r32) and _rel(c32, fc32) <= 1e-13
+    # deterministic
+    assert all(np.array_equal(a, b) for a, b in zip(_tangent(P, .03, -.7, n_paths, n_steps,
+                                                              mc.mode(mc.FP32, seed=3)), (r32, c32, g32)))
+
+
+def test_tangent_errors(engine):
+    import torch
+    P = _params(2, 11)
+    dev = torch.device("cuda:0")
+    prm = [torch.tensor(P[k], dtype=torch.float64, device=dev) for k in ("S", "X", "T", "v0", "kappa", "theta",
+                                                                          "xi")]
+    out, oc = torch.zeros(2, dtype=torch.float64, device=dev), torch.zeros(2, dtype=torch.float64, device=dev)
+    og = torch.zeros(2, 6, dtype=torch.float64, device=dev)
+    ws = torch.empty(1 << 16, dtype=torch.uint8, device=dev)
+    with pytest.raises(mc.FsmcError):  # Greeks need Philox without quirks
+        mc.heston_asian_tangent_async(prm, 128, 10, .03, -.7, mc.mode(quirks=Q4), out, oc, og, ws)
+    with pytest.raises(mc.FsmcError):
+        mc.heston_asian_tangent_async(prm, 128, 10, .03, -.7, mc.mode(), out, oc, None, ws)
+    with pytest.raises(mc.FsmcError):  # workspace too small
+        mc.heston_asian_tangent_async(prm, 1 << 20, 10, .03, -.7, mc.mode(), out, oc, og, ws[:64])
