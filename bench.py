@@ -478,7 +478,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=14.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
-    ap.add_argument("--workload", choices=["heston", "heston_adjoint", "copy", "probe"] + list(CONFIGS),
+    ap.add_argument("--workload", choices=["heston", "heston_adjoint", "heston_tangent", "copy", "probe"] + list(CONFIGS),
                     help="profiling aid: run only this workload in FP32 for --steps launches and print its timing")
     ap.add_argument("--fp64", action="store_true", help="with --workload: run the plain FP64 path as well")
     args = ap.parse_args()
@@ -534,6 +534,13 @@ def main():
     if args.workload == "heston_adjoint":
         msg, kg, _, _ = run(m32, args.steps, args.warmup, with_greeks=True)
         print(json.dumps({"workload": "heston_adjoint", "ms": msg, "kernel_ms": kg}), flush=True)
+        return
+    if args.workload == "heston_tangent":
+        tws = torch.empty(mc.heston_asian_tangent_workspace(N_OPT, N_PATHS, N_STEPS, R, RHO, m32), dtype=torch.uint8,
+                          device=dev)
+        mst = _timed(torch, np, lambda: mc.heston_asian_tangent_async(prm, N_PATHS, N_STEPS, R, RHO, m32, out, conf,
+                                                                      greeks, tws), args.steps, args.warmup)
+        print(json.dumps({"workload": "heston_tangent", "kernel_ms": mst}), flush=True)
         return
     # The issue-rate probe runs first (about 0.2 s of GPU work): its rates
     # feed the instruction-mix ceiling, and the GPU clock, which ramps up
