@@ -36,7 +36,10 @@
 //     at date 1) leaves behind, so the cash flows are not read again.
 // Options are sorted by step count (descending) so the options still active
 // at backward pass j are a prefix of the workgroup table: pass j launches
-// exactly their workgroups.
+// exactly their workgroups.  That is the streamed sweep; the resident sweep
+// (below) runs the same arithmetic on the same normals in one cooperative
+// launch with the paths' state in registers, and is the default when the
+// batch fits the resident grid.
 //
 // Reference algorithm ("refcompat", MonteCarloAmericanOptions/main.cpp:115-279
 // and utilities.cpp:42-123) for one option: path matrix, lagged quadratic
@@ -192,8 +195,12 @@ __device__ __forceinline__ void combine_partials(const double* partials, int fir
 __device__ void lsmc_solve(const double (&s)[kNS], double* beta_out) {
   // Cholesky G = L L^T of G_ij = m_{i+j}; a pivot that is not clearly
   // positive (fewer than 4 ITM paths, or a singular basis) disables exercise
-  // at this date.  Fully unrolled so L stays in registers.
-  double L[4][4] = {{0}};
+  // at this date.  The diagonal enters only through divisions, so its four
+  // reciprocals are formed once and the remaining steps multiply: four
+  // square roots and four divisions on the serial path instead of four and
+  // fourteen (the solve sits on the critical path of every date).  Fully
+  // unrolled so L stays in registers.
+  double L[4][4] = {{0}}, inv[4];
   bool ok = s[0] >= 4.0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -204,9 +211,9 @@ __device__ void lsmc_solve(const double (&s)[kNS], double* beta_out) {
       for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
       if (i == j) {
         ok = ok && v > 1e-13 * s[2 * i];
-        L[i][i] = ok ? sqrt(v) : 1.0;
+        inv[i] = ok ? 1.0 / sqrt(v) : 1.0;
       } else {
-        L[i][j] = v / L[j][j];
+        L[i][j] = v * inv[j];
       }
     }
   }
@@ -216,14 +223,14 @@ __device__ void lsmc_solve(const double (&s)[kNS], double* beta_out) {
     double v = s[7 + i];
 #pragma unroll
     for (int k = 0; k < i; ++k) v -= L[i][k] * y[k];
-    y[i] = v / L[i][i];
+    y[i] = v * inv[i];
   }
 #pragma unroll
   for (int i = 3; i >= 0; --i) {
     double v = y[i];
 #pragma unroll
     for (int k = i + 1; k < 4; ++k) v -= L[k][i] * x[k];
-    x[i] = v / L[i][i];
+    x[i] = v * inv[i];
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) beta_out[i] = ok ? x[i] : 0.0;
@@ -383,6 +390,365 @@ __global__ void zero_f64_kernel(double* p, int n) {
 }
 
 // ---------------------------------------------------------------------------
+// Resident sweep: the whole backward sweep of every option in one cooperative
+// launch, path state held in registers
+// ---------------------------------------------------------------------------
+// An option's paths are cut into parts of up to kThreads * PPT paths; a part
+// is one workgroup that keeps its paths' bridge state W and discounted cash
+// flow in registers from the terminal date down to date 1, so nothing is
+// streamed through HBM between dates.  The workgroup fills a CU (one per CU):
+// an option then meets at most 16 partners per date (262,144 paths in FP32),
+// and no other option's workgroup shares its CU, whose pace would otherwise
+// couple every option on the chip into one lockstep (measured: 256-thread
+// parts four to a CU, 22 ms for the 256-put batch; one 1024-thread part per
+// CU, 13.7 ms).  The parts of one option meet once per date, and nothing else
+// in the grid waits for them:
+//   * each part sums its moments (thread sums, wave trees, FP64 across waves)
+//     and publishes them in its record for the date's parity as 64-bit words
+//     that each carry the date's epoch (pass + 1) in their high half and 32
+//     bits of the sums in their low half (FP32: one word per sum; FP64: two,
+//     the halves of the double).  A 64-bit store is single-copy atomic, so a
+//     word whose epoch matches holds this date's bits and no ordering between
+//     the words is needed; the stores are device-scope (sc1, written through
+//     to the point of coherence);
+//   * wave 0 of every part reads the records of all the option's parts (lane
+//     l takes parts l, l + 64, ...) with device-scope loads, all words of a
+//     record at once, until every word carries the epoch (one round trip when
+//     the part has already published), and the sums are added in part order
+//     (every part derives bit-identical coefficients, and reruns reproduce
+//     them).  Lane 0 solves by the FP64 Cholesky and broadcasts the
+//     coefficients through LDS.
+// Two record buffers (pass parity) suffice: a part can publish pass j + 2
+// (overwriting pass j) only after every part has published pass j + 1, which
+// each does only after it has read all of pass j.
+// No agent-scope fences are used: each one writes back or invalidates the
+// XCD's L2, and a measured 1024-workgroup barrier cost 20 us per date with
+// them against 3 us with device-scope stores and loads of the records.
+// Workgroups run host-built task lists (option, part); all parts of an option
+// appear in the same global order on every list, and the launch is
+// cooperative (all workgroups resident), so every barrier completes.  Each
+// poll is bounded: past spin_limit polls (or once another workgroup gave up)
+// a workgroup raises the abort flag and returns, so every wave exits; the
+// host reports the abort as an error.
+// Record words of one part and date: (epoch << 32) | 32 bits of the sums.
+template <typename Real>
+struct RecWords {
+  static constexpr int kN = sizeof(Real) == 4 ? kNM : 2 * kNM;
+};
+constexpr int kRec = 2 * kNM;  // words per record slot (the FP64 layout; FP32 uses the first kNM)
+
+// Per-option and per-date constants of the resident sweep, computed on the
+// host in FP64 (the kernel rounds them to the working precision as the
+// streamed sweep does): device FP64 exp() would keep its polynomial
+// coefficients in registers beside the paths' state.
+struct ResOpt {
+  double sqT, muT, discT, invK;  // sqrt(T), (r - sigma^2/2) T, e^{-rT}, 1/K
+  int date0, pad;                // first entry of the option's date table (entry d = date d)
+};
+struct DateConst {
+  double mu;    // (r - sigma^2/2) d dt
+  double isw;   // 1 / (sigma sqrt(d dt))
+  double disc;  // e^{-r d dt}
+  double ba;    // bridge d -> d - 1: W_{d-1} = ba W_d + bb Z
+  double bb;
+};
+
+struct ResArgs {
+  const OptMeta* meta;    // first_block = first part, nblk = parts
+  const ResOpt* ropt;
+  const DateConst* dates;
+  const int* slot_begin;  // [grid + 1]: the workgroup's tasks
+  const int2* tasks;      // (option, part) in the global option order
+  unsigned long long* records;  // [part][2][kRec] tagged words
+  double* eur;            // [part] European payoff sums
+  double* cfsum;          // [part][2] final cash-flow sum and sum of squares
+  int* abort_flag;
+  int part_paths;
+  int spin_limit;
+  uint32_t k0, k1, w;
+};
+
+// Wave totals for the resident kernel, in a fixed order (every lane ends with
+// the same bits): four DPP steps inside each row of 16 lanes, a swizzle
+// across the 32-lane halves and the two halves' totals read as scalars.
+// Unlike a ds_bpermute tree they need no per-lane address registers, which
+// the compiler would otherwise hoist out of the sweep's loops and keep live
+// beside the paths' state.
+template <int kCtrl>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, true);
+}
+template <int kCtrl>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = dpp_i32<kCtrl>((int)b), hi = dpp_i32<kCtrl>((int)(b >> 32));
+  return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double swz16_f64(double v) {  // lane ^ 16 within each 32-lane half
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_ds_swizzle((int)b, 0x401F), hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x401F);
+  return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double wave_total(double v) {
+  v += dpp_f64<0xB1>(v);   // lane ^ 1
+  v += dpp_f64<0x4E>(v);   // lane ^ 2
+  v += dpp_f64<0x141>(v);  // row half mirror
+  v += dpp_f64<0x140>(v);  // row mirror
+  v += swz16_f64(v);
+  return readlane_f64(v, 0) + readlane_f64(v, 32);
+}
+__device__ __forceinline__ float wave_total(float v) {
+  v += dpp_f32<0xB1>(v);
+  v += dpp_f32<0x4E>(v);
+  v += dpp_f32<0x141>(v);
+  v += dpp_f32<0x140>(v);
+  v += __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, v), 0x401F));
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 0)) +
+         __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 32));
+}
+
+// Workgroup totals of NV values per thread: wave totals (working precision),
+// combined over the kThreads / 64 waves in FP64 in wave order by lane q of
+// wave 0 for value q.  The total of value q is returned in thread q < NV
+// (other threads: unspecified); the caller separates reuses of lds by a
+// barrier.
+template <int kThreads, typename T, int NV>
+__device__ __forceinline__ double res_block_sum(const T (&v)[NV], double* lds) {
+  constexpr int kWaves = kThreads / 64;
+  const int wave = threadIdx.x >> 6;
+  double w[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) w[q] = (double)wave_total(v[q]);
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) lds[q * kWaves + wave] = w[q];
+  }
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x < NV) {
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) t += lds[threadIdx.x * kWaves + i];
+  }
+  return t;
+}
+
+template <typename Real, int PPT, int kThreads>
+__global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
+  using O = MathOps<Real>;
+  constexpr int NG = PPT / 4;
+  __shared__ double lds[kNS * (kThreads / 64)];
+  __shared__ double stage[64 * kNM];  // records of up to 64 parts, [part][moment]
+  __shared__ double sums[kNS];
+  __shared__ double beta_lds[5];
+  __shared__ int stop_lds;
+  // the option's date constants, copied once per task: a scalar load from
+  // device memory per date would put its latency on every date's critical path
+  constexpr int kDatesLds = 1024;
+  __shared__ DateConst dl[kDatesLds];
+  const int lane = threadIdx.x & 63;
+  const int t_end = a.slot_begin[blockIdx.x + 1];
+  for (int ti = a.slot_begin[blockIdx.x]; ti < t_end; ++ti) {
+    const int2 task = a.tasks[ti];
+    const int o = task.x, part = task.y;
+    const OptMeta m = a.meta[o];
+    const int first = part * a.part_paths;
+    const int np = min(a.part_paths, m.paths - first);
+    const int rec = m.first_block + part;
+    const ResOpt ro = a.ropt[o];
+    const bool dates_in_lds = m.steps < kDatesLds;  // uniform
+    if (dates_in_lds)
+      for (int i = threadIdx.x; i <= m.steps; i += kThreads) dl[i] = a.dates[ro.date0 + i];
+    // (the barrier in the terminal sum below orders the copy before the dates' reads)
+    auto date = [&](int i) -> DateConst { return dates_in_lds ? dl[i] : a.dates[ro.date0 + i]; };
+    const Real K = (Real)m.K, invK = (Real)ro.invK, S0 = (Real)m.S0, sg = (Real)(m.sigma * O::kE);
+    Real W[NG][4], C[NG][4];
+    {
+      const Real sqT = (Real)ro.sqT, muT = (Real)(ro.muT * O::kE), discT = (Real)ro.discT;
+      double eu[1] = {0.0};
+#pragma unroll
+      for (int h = 0; h < NG; ++h) {
+        const int l0 = 4 * (h * kThreads + (int)threadIdx.x);
+        Real z[4];  // computed for padding groups too (never used for them): no branch per group
+        normals4<Real>(U4{(uint32_t)((first + l0) >> 2), (uint32_t)(m.steps - 1), (uint32_t)o, a.w}, a.k0, a.k1,
+                       z[0], z[1], z[2], z[3]);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const bool live = l0 + g < np;
+          W[h][g] = live ? sqT * z[g] : Real(0);
+          const Real ST = S0 * O::ex(muT + sg * W[h][g]);
+          C[h][g] = live ? discT * (K - ST > Real(0) ? K - ST : Real(0)) : Real(0);
+          eu[0] += (double)C[h][g];
+        }
+        __builtin_amdgcn_sched_barrier(0);  // one group of paths at a time: bounds the live registers
+      }
+      const double t = res_block_sum<kThreads, double, 1>(eu, lds);
+      if (threadIdx.x == 0) a.eur[rec] = t;
+      __syncthreads();  // lds is reused by the first pass
+    }
+    for (int pass = 0; pass < m.steps; ++pass) {
+      const int d = m.steps - pass;  // decision date
+      const int k = d - 1;           // regression date
+      if (pass >= 1 && beta_lds[4] != 0.0) {  // exercise decisions of date d (uniform; per path by selects)
+        const Real b0 = (Real)beta_lds[0], b1 = (Real)beta_lds[1], b2 = (Real)beta_lds[2], b3 = (Real)beta_lds[3];
+        const DateConst cd = date(d);
+        const Real mu_d = (Real)(cd.mu * O::kE), isw_d = (Real)cd.isw, disc_d = (Real)cd.disc;
+#pragma unroll
+        for (int h = 0; h < NG; ++h) {
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {
+            const Real Sd = S0 * O::ex(mu_d + sg * W[h][g]);
+            const Real x = Real(1) + (Sd * invK - Real(1)) * isw_d;
+            const Real cont = b0 + x * (b1 + x * (b2 + x * b3));
+            const Real ex = (K - Sd) * disc_d;
+            // padding paths never exercise
+            int l = 4 * (h * kThreads + (int)threadIdx.x) + g;
+            __asm__ volatile("" : "+v"(l));
+            const bool live = l < np;
+            C[h][g] = live && K - Sd > Real(0) && ex > cont ? ex : C[h][g];
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      if (k < 1) {  // decision at date 1 done: the part's final cash-flow sums
+        double fin[2] = {0.0, 0.0};
+#pragma unroll
+        for (int h = 0; h < NG; ++h) {
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {
+            fin[0] += (double)C[h][g];  // padding paths hold 0
+            fin[1] += (double)C[h][g] * (double)C[h][g];
+          }
+        }
+        const double t = res_block_sum<kThreads, double, 2>(fin, lds);
+        if (threadIdx.x < 2) a.cfsum[2 * rec + threadIdx.x] = t;
+        __syncthreads();  // lds is reused by the next task
+        break;
+      }
+      // bridge to date k and the regression moments of date k
+      Real sm[kNM];
+#pragma unroll
+      for (int q = 0; q < kNM; ++q) sm[q] = Real(0);
+      {
+        const DateConst cd = date(d), ck = date(k);
+        const Real mu_k = (Real)(ck.mu * O::kE), isw_k = (Real)ck.isw;
+        const Real ba = (Real)cd.ba, bb = (Real)cd.bb;
+#pragma unroll
+        for (int h = 0; h < NG; ++h) {
+          // l0 is made opaque to the optimiser: otherwise it hoists every
+          // group's path index and first Philox product out of the date loop
+          // and keeps them live (spilled) beside the paths' state
+          int l0 = 4 * (h * kThreads + (int)threadIdx.x);
+          __asm__ volatile("" : "+v"(l0));
+          Real z[4];
+          normals4<Real>(U4{(uint32_t)((first + l0) >> 2), (uint32_t)(k - 1), (uint32_t)o, a.w}, a.k0, a.k1, z[0],
+                         z[1], z[2], z[3]);
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {
+            const bool live = l0 + g < np;
+            const Real Wk = ba * W[h][g] + bb * z[g];
+            W[h][g] = live ? Wk : Real(0);
+            const Real Sk = S0 * O::ex(mu_k + sg * Wk);
+            const bool itm = live && K - Sk > Real(0);
+            const Real x = itm ? Real(1) + (Sk * invK - Real(1)) * isw_k : Real(0);
+            accum_moments(sm, itm, x, C[h][g]);
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      // publish this part's moments, then gather every part's
+      constexpr int NW = RecWords<Real>::kN;
+      const unsigned long long tag = (unsigned long long)(pass + 1) << 32;
+      {
+        const double s = res_block_sum<kThreads, Real, kNM>(sm, lds);
+        const int q = threadIdx.x;  // lane q < kNM publishes moment q
+        if (q < kNM) {
+          unsigned long long* r = a.records + ((size_t)rec * 2 + (pass & 1)) * kRec;
+          if constexpr (sizeof(Real) == 4) {
+            __hip_atomic_store(r + q, tag | __builtin_bit_cast(unsigned, (float)s), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+          } else {
+            const unsigned long long b = __builtin_bit_cast(unsigned long long, s);
+            __hip_atomic_store(r + 2 * q, tag | (b & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(r + 2 * q + 1, tag | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+      }
+      if (threadIdx.x < 64) {
+        // lanes l < parts poll and stage part l's record (rounds of 64 parts);
+        // lanes q < kNM then add moment q over the staged parts in part order
+        double acc = 0.0;
+        int bad = 0;
+        for (int base = 0; base < m.nblk && !bad; base += 64) {
+          const int p = base + lane;
+          if (p < m.nblk) {
+            unsigned long long* r = a.records + ((size_t)(m.first_block + p) * 2 + (pass & 1)) * kRec;
+            unsigned long long w[NW];
+            // read all words of the record, until every one carries the epoch
+            for (int spins = 0;; ++spins) {
+#pragma unroll
+              for (int q = 0; q < NW; ++q) w[q] = __hip_atomic_load(r + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              bool all = true;
+#pragma unroll
+              for (int q = 0; q < NW; ++q) all = all && (w[q] & 0xffffffff00000000ull) == tag;
+              if (all) break;
+              // the abort flag is read once per 64 polls
+              if ((spins & 63) == 63 &&
+                  (spins > a.spin_limit || __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                bad = 1;
+                break;
+              }
+              __builtin_amdgcn_s_sleep(2);
+            }
+            if (!bad) {
+#pragma unroll
+              for (int q = 0; q < kNM; ++q) {
+                if constexpr (sizeof(Real) == 4)
+                  stage[lane * kNM + q] = (double)__builtin_bit_cast(float, (unsigned)w[q]);
+                else
+                  stage[lane * kNM + q] =
+                      __builtin_bit_cast(double, (w[2 * q] & 0xffffffffull) | (w[2 * q + 1] << 32));
+              }
+            }
+          }
+          bad = __any(bad);
+          __builtin_amdgcn_wave_barrier();  // the wave's LDS writes land before its reads
+          if (!bad && lane < kNM) {
+            const int n = min(64, m.nblk - base);
+            for (int i = 0; i < n; ++i) acc += stage[i * kNM + lane];
+          }
+          __builtin_amdgcn_wave_barrier();  // reads done before the next round's writes
+        }
+        if (lane < kNM) sums[lane] = acc;
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) {
+          if (!bad) {
+            double t[kNS];
+#pragma unroll
+            for (int q = 0; q < kNS; ++q) t[q] = q < kNM ? sums[q] : 0.0;
+            lsmc_solve(t, beta_lds);
+          }
+          stop_lds = bad;
+        }
+      }
+      __syncthreads();
+      if (stop_lds) return;  // uniform: every thread read the same flag after the barrier
+    }
+  }
+}
+
+__global__ void lsmc_resident_clear_kernel(unsigned long long* records, size_t n, int* abort_flag) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) records[i] = 0ull;  // epoch 0: no date published
+  if (i == 0) *abort_flag = 0;
+}
+
+// ---------------------------------------------------------------------------
 // Reference algorithm, one option
 // ---------------------------------------------------------------------------
 struct RefArgs {
@@ -528,6 +894,7 @@ struct LsmcPlanImpl {
   int n;
   int precision;
   fsmc_mode mode;
+  int sweep = FSMC_LSMC_SWEEP_STREAMED;
   std::vector<OptMeta> meta;
   std::vector<int4> table;
   std::vector<int> order;         // option indices sorted by steps (descending)
@@ -536,11 +903,116 @@ struct LsmcPlanImpl {
   int max_steps = 0;
   size_t real = 4, cf_elems = 0;
   size_t off_meta, off_table, off_order, off_w, off_cf, off_part, off_beta, off_eur, off_cfsum, off_out, ws = 0;
+  // resident sweep: meta (first_block / nblk = parts), task lists per workgroup
+  int grid = 0, part_paths = 0, parts = 0;
+  std::vector<int> slot_begin;
+  std::vector<int2> tasks;
+  std::vector<ResOpt> ropt;
+  std::vector<DateConst> dates;
+  size_t off_slots = 0, off_tasks = 0, off_rec = 0, off_abort = 0, off_ropt = 0, off_dates = 0;
 };
 
+// Shape of the resident sweep: one workgroup per CU, as wide as the register
+// file allows, so an option meets few partners per date and no other
+// option's workgroup shares (and paces) its CU.  FP32 keeps 16 paths' bridge
+// state and cash flow (32 VGPRs) per thread at 1024 threads (16 waves, four
+// per SIMD: <= 128 VGPRs, no spills); FP64 keeps 16 paths (64 VGPRs) at 512
+// threads (two waves per SIMD).
+constexpr int kResPPT32 = 16, kResThreads32 = 1024;
+constexpr int kResPPT64 = 16, kResThreads64 = 512;
+constexpr int kResSpinLimit = 1 << 22;  // polls of one record (each ~1 us) before a workgroup gives up
+
+template <typename Real>
+const void* resident_kernel() {
+  return sizeof(Real) == 4 ? (const void*)lsmc_resident_kernel<float, kResPPT32, kResThreads32>
+                           : (const void*)lsmc_resident_kernel<double, kResPPT64, kResThreads64>;
+}
+inline int resident_threads(int precision) { return precision == FSMC_FP64 ? kResThreads64 : kResThreads32;
+}
+
+// Workgroups of the resident grid on ctx's device (0 when a cooperative
+// launch is not available).
+int resident_grid(int device, int precision) {
+  int coop = 0, cus = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) != hipSuccess || !coop) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+  const void* f = precision == FSMC_FP64 ? resident_kernel<double>() : resident_kernel<float>();
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, resident_threads(precision), 0) != hipSuccess) return 0;
+  return cus * per_cu;
+}
+
+// Resident sweep plan: parts of part_paths paths; longest-first list
+// schedule of the options over the grid's workgroups (an option takes the
+// nblk workgroups that free up first and holds them for its steps dates).
+// Every list holds its tasks in the global option order.  Returns false if
+// an option has more parts than the grid has workgroups.
+bool lsmc_make_resident(LsmcPlanImpl* pl, int grid) {
+  const int n = pl->n;
+  pl->part_paths = pl->precision == FSMC_FP64 ? kResThreads64 * kResPPT64 : kResThreads32 * kResPPT32;
+  pl->grid = grid;
+  int parts = 0;
+  for (int idx = 0; idx < n; ++idx) {
+    OptMeta& q = pl->meta[pl->order[idx]];
+    q.first_block = parts;
+    q.nblk = (q.paths + pl->part_paths - 1) / pl->part_paths;
+    if (q.nblk > grid) return false;
+    parts += q.nblk;
+  }
+  pl->parts = parts;
+  std::vector<double> avail(grid, 0.0);
+  std::vector<std::vector<int2>> lists(grid);
+  std::vector<int> slot(grid);
+  for (int idx = 0; idx < n; ++idx) {
+    const int o = pl->order[idx];
+    const OptMeta& q = pl->meta[o];
+    std::iota(slot.begin(), slot.end(), 0);
+    std::partial_sort(slot.begin(), slot.begin() + q.nblk, slot.end(),
+                      [&](int x, int y) { return avail[x] < avail[y] || (avail[x] == avail[y] && x < y); });
+    double start = 0.0;
+    for (int j = 0; j < q.nblk; ++j) start = std::max(start, avail[slot[j]]);
+    for (int j = 0; j < q.nblk; ++j) {
+      avail[slot[j]] = start + q.steps;
+      lists[slot[j]].push_back(make_int2(o, j));
+    }
+  }
+  pl->slot_begin.assign(grid + 1, 0);
+  pl->tasks.clear();
+  for (int g = 0; g < grid; ++g) {
+    pl->tasks.insert(pl->tasks.end(), lists[g].begin(), lists[g].end());
+    pl->slot_begin[g + 1] = (int)pl->tasks.size();
+  }
+  // the same FP64 expressions as the streamed sweep's kernels
+  pl->ropt.resize(n);
+  pl->dates.clear();
+  for (int o = 0; o < n; ++o) {
+    const OptMeta& q = pl->meta[o];
+    ResOpt& ro = pl->ropt[o];
+    ro.sqT = sqrt(q.T);
+    ro.muT = (q.r - 0.5 * q.sigma * q.sigma) * q.T;
+    ro.discT = exp(-q.r * q.T);
+    ro.invK = 1.0 / q.K;
+    ro.date0 = (int)pl->dates.size();
+    ro.pad = 0;
+    const double dt = q.T / q.steps, mu = (q.r - 0.5 * q.sigma * q.sigma) * dt;
+    for (int d = 0; d <= q.steps; ++d) {
+      DateConst c;
+      c.mu = mu * d;
+      c.isw = d >= 1 ? 1.0 / (q.sigma * sqrt(d * dt)) : 0.0;
+      c.disc = exp(-q.r * d * dt);
+      c.ba = d >= 1 ? (double)(d - 1) / d : 0.0;
+      c.bb = d >= 1 ? sqrt(dt * (d - 1) / d) : 0.0;
+      pl->dates.push_back(c);
+    }
+  }
+  return true;
+}
+
 int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K, const double* sigma,
-                   const double* r, const double* T, const int* steps, const int* paths, LsmcPlanImpl* pl) {
+                   const double* r, const double* T, const int* steps, const int* paths, int sweep, int grid,
+                   LsmcPlanImpl* pl) {
   FSMC_REQUIRE(n >= 1 && n <= (1 << 20), "n_options %d outside [1, 2^20]", n);
+  FSMC_REQUIRE(sweep == FSMC_LSMC_SWEEP_AUTO || sweep == FSMC_LSMC_SWEEP_RESIDENT || sweep == FSMC_LSMC_SWEEP_STREAMED,
+               "sweep %d is not an FSMC_LSMC_SWEEP_* value", sweep);
   FSMC_TRY(validate_mode(m, false));
   FSMC_REQUIRE(S0 && K && sigma && r && T && steps && paths, "NULL host pointer");
   for (int o = 0; o < n; ++o) {
@@ -598,6 +1070,28 @@ int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K,
     off = align_up(off + bytes, 256);
     return at;
   };
+  pl->sweep = FSMC_LSMC_SWEEP_STREAMED;
+  if (sweep != FSMC_LSMC_SWEEP_STREAMED) {
+    FSMC_REQUIRE(grid > 0 || sweep == FSMC_LSMC_SWEEP_AUTO, "the resident sweep needs a cooperative launch");
+    const std::vector<OptMeta> streamed = pl->meta;
+    if (grid > 0 && lsmc_make_resident(pl, grid)) {
+      pl->sweep = FSMC_LSMC_SWEEP_RESIDENT;
+      pl->off_meta = take(sizeof(OptMeta) * n);
+      pl->off_slots = take(sizeof(int) * (grid + 1));
+      pl->off_tasks = take(sizeof(int2) * pl->tasks.size());
+      pl->off_rec = take(sizeof(unsigned long long) * 2 * kRec * pl->parts);
+      pl->off_eur = take(sizeof(double) * pl->parts);
+      pl->off_cfsum = take(sizeof(double) * 2 * pl->parts);
+      pl->off_abort = take(sizeof(int));
+      pl->off_ropt = take(sizeof(ResOpt) * n);
+      pl->off_dates = take(sizeof(DateConst) * pl->dates.size());
+      pl->off_out = take(sizeof(double) * 3 * n);
+      pl->ws = off;
+      return FSMC_OK;
+    }
+    FSMC_REQUIRE(sweep == FSMC_LSMC_SWEEP_AUTO, "an option needs more than the %d resident workgroups", grid);
+    pl->meta = streamed;
+  }
   pl->off_meta = take(sizeof(OptMeta) * n);
   pl->off_table = take(sizeof(int4) * pl->table.size());
   pl->off_order = take(sizeof(int) * n);
@@ -613,7 +1107,41 @@ int lsmc_make_plan(int n, const fsmc_mode* m, const double* S0, const double* K,
 }
 
 template <typename Real>
+int lsmc_enqueue_resident(const LsmcPlanImpl& pl, char* ws, hipStream_t s) {
+  ResArgs r;
+  r.meta = (const OptMeta*)(ws + pl.off_meta);
+  r.ropt = (const ResOpt*)(ws + pl.off_ropt);
+  r.dates = (const DateConst*)(ws + pl.off_dates);
+  r.slot_begin = (const int*)(ws + pl.off_slots);
+  r.tasks = (const int2*)(ws + pl.off_tasks);
+  r.records = (unsigned long long*)(ws + pl.off_rec);
+  r.eur = (double*)(ws + pl.off_eur);
+  r.cfsum = (double*)(ws + pl.off_cfsum);
+  r.abort_flag = (int*)(ws + pl.off_abort);
+  r.part_paths = pl.part_paths;
+  r.spin_limit = kResSpinLimit;
+  r.k0 = (uint32_t)pl.mode.seed;
+  r.k1 = (uint32_t)(pl.mode.seed >> 32);
+  r.w = counter_w(kTagLsmc, pl.mode.stream);
+  const size_t nrec = (size_t)2 * kRec * pl.parts;
+  hipLaunchKernelGGL(lsmc_resident_clear_kernel, dim3((unsigned)((nrec + 255) / 256)), dim3(256), 0, s, r.records,
+                     nrec, r.abort_flag);
+  void* args[] = {&r};
+  FSMC_HIP(hipLaunchCooperativeKernel(resident_kernel<Real>(), dim3(pl.grid), dim3(resident_threads(pl.precision)),
+                                      args, 0, s));
+  LsmcArgs a{};
+  a.meta = r.meta;
+  a.eur = r.eur;
+  a.cfsum = r.cfsum;
+  double* out = (double*)(ws + pl.off_out);
+  hipLaunchKernelGGL(lsmc_final_kernel<Real>, dim3(pl.n), dim3(kBlock), 0, s, a, out, out + 2 * pl.n);
+  FSMC_HIP(hipGetLastError());
+  return FSMC_OK;
+}
+
+template <typename Real>
 int lsmc_enqueue(const LsmcPlanImpl& pl, char* ws, hipStream_t s) {
+  if (pl.sweep == FSMC_LSMC_SWEEP_RESIDENT) return lsmc_enqueue_resident<Real>(pl, ws, s);
   LsmcArgs a;
   a.meta = (const OptMeta*)(ws + pl.off_meta);
   a.table = (const int4*)(ws + pl.off_table);
@@ -657,26 +1185,41 @@ struct fsmc_lsmc_plan {
 
 extern "C" {
 
-int fsmc_lsmc_plan_create(fsmc_ctx* ctx, int n_options, const fsmc_mode* m, const double* S0, const double* K,
-                          const double* sigma, const double* r, const double* T, const int* steps,
-                          const int* paths, fsmc_lsmc_plan** out) {
+int fsmc_lsmc_plan_create_sweep(fsmc_ctx* ctx, int n_options, const fsmc_mode* m, const double* S0,
+                                const double* K, const double* sigma, const double* r, const double* T,
+                                const int* steps, const int* paths, int sweep, fsmc_lsmc_plan** out) {
   FSMC_REQUIRE(ctx && out, "NULL argument");
   *out = nullptr;
+  FSMC_HIP(hipSetDevice(ctx->device));
+  const int grid = sweep == FSMC_LSMC_SWEEP_STREAMED
+                       ? 0
+                       : resident_grid(ctx->device, m && m->precision == FSMC_FP64 ? FSMC_FP64 : FSMC_FP32);
   fsmc_lsmc_plan* pl = new fsmc_lsmc_plan();
   pl->ctx = ctx;
-  int st = lsmc_make_plan(n_options, m, S0, K, sigma, r, T, steps, paths, &pl->impl);
+  int st = lsmc_make_plan(n_options, m, S0, K, sigma, r, T, steps, paths, sweep, grid, &pl->impl);
   if (st == FSMC_OK) {
     std::lock_guard<std::mutex> lock(ctx->mu);
     st = hipSetDevice(ctx->device) == hipSuccess ? FSMC_OK : FSMC_ERR_DEVICE;
     if (st == FSMC_OK) st = pl->ws.reserve(pl->impl.ws);
     if (st == FSMC_OK) {
       char* ws = (char*)pl->ws.ptr;
-      if (hipMemcpyAsync(ws + pl->impl.off_meta, pl->impl.meta.data(), sizeof(OptMeta) * n_options,
-                         hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
-          hipMemcpyAsync(ws + pl->impl.off_table, pl->impl.table.data(), sizeof(int4) * pl->impl.table.size(),
-                         hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
-          hipMemcpyAsync(ws + pl->impl.off_order, pl->impl.order.data(), sizeof(int) * n_options,
-                         hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+      const LsmcPlanImpl& q = pl->impl;
+      const bool res = q.sweep == FSMC_LSMC_SWEEP_RESIDENT;
+      if (hipMemcpyAsync(ws + q.off_meta, q.meta.data(), sizeof(OptMeta) * n_options, hipMemcpyHostToDevice,
+                         ctx->stream) != hipSuccess ||
+          (!res && hipMemcpyAsync(ws + q.off_table, q.table.data(), sizeof(int4) * q.table.size(),
+                                  hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
+          (!res && hipMemcpyAsync(ws + q.off_order, q.order.data(), sizeof(int) * n_options, hipMemcpyHostToDevice,
+                                  ctx->stream) != hipSuccess) ||
+          (res && hipMemcpyAsync(ws + q.off_slots, q.slot_begin.data(), sizeof(int) * q.slot_begin.size(),
+                                 hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
+          (res && !q.tasks.empty() &&
+           hipMemcpyAsync(ws + q.off_tasks, q.tasks.data(), sizeof(int2) * q.tasks.size(), hipMemcpyHostToDevice,
+                          ctx->stream) != hipSuccess) ||
+          (res && hipMemcpyAsync(ws + q.off_ropt, q.ropt.data(), sizeof(ResOpt) * q.ropt.size(),
+                                 hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
+          (res && hipMemcpyAsync(ws + q.off_dates, q.dates.data(), sizeof(DateConst) * q.dates.size(),
+                                 hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
           hipStreamSynchronize(ctx->stream) != hipSuccess) {
         set_error("upload of the LSMC plan failed");
         st = FSMC_ERR_DEVICE;
@@ -692,7 +1235,15 @@ int fsmc_lsmc_plan_create(fsmc_ctx* ctx, int n_options, const fsmc_mode* m, cons
   return FSMC_OK;
 }
 
+int fsmc_lsmc_plan_create(fsmc_ctx* ctx, int n_options, const fsmc_mode* m, const double* S0, const double* K,
+                          const double* sigma, const double* r, const double* T, const int* steps,
+                          const int* paths, fsmc_lsmc_plan** out) {
+  return fsmc_lsmc_plan_create_sweep(ctx, n_options, m, S0, K, sigma, r, T, steps, paths, FSMC_LSMC_SWEEP_AUTO, out);
+}
+
 size_t fsmc_lsmc_plan_workspace(const fsmc_lsmc_plan* pl) { return pl ? pl->impl.ws : 0; }
+
+int fsmc_lsmc_plan_sweep(const fsmc_lsmc_plan* pl) { return pl ? pl->impl.sweep : 0; }
 
 int fsmc_lsmc_plan_run(fsmc_lsmc_plan* pl, void* stream) {
   FSMC_REQUIRE(pl != nullptr, "plan is NULL");
@@ -711,7 +1262,14 @@ int fsmc_lsmc_plan_results(fsmc_lsmc_plan* pl, void* stream, double* price, doub
   const int n = pl->impl.n;
   FSMC_HIP(hipMemcpyAsync(price, out, 2 * n * sizeof(double), hipMemcpyDeviceToHost, s));
   if (european) FSMC_HIP(hipMemcpyAsync(european, out + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  int aborted = 0;
+  if (pl->impl.sweep == FSMC_LSMC_SWEEP_RESIDENT)
+    FSMC_HIP(hipMemcpyAsync(&aborted, (char*)pl->ws.ptr + pl->impl.off_abort, sizeof(int), hipMemcpyDeviceToHost, s));
   FSMC_HIP(hipStreamSynchronize(s));
+  if (aborted) {
+    set_error("resident LSMC sweep: a workgroup gave up waiting at a date barrier");
+    return FSMC_ERR_DEVICE;
+  }
   return FSMC_OK;
 }
 

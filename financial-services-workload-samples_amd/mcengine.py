@@ -25,6 +25,7 @@ QUIRK_ASIAN_VAR_AS_VOL = 1 << 0
 QUIRK_ASIAN_RUNNING_VAL2 = 1 << 1
 QUIRK_LIBOR_INPLACE_ADJ = 1 << 2
 QUIRK_BS_POLY_CND = 1 << 3
+LSMC_AUTO, LSMC_RESIDENT, LSMC_STREAMED = 0, 1, 2  # FSMC_LSMC_SWEEP_*
 LATTICE_AMERICAN, LATTICE_PUT = 1, 2
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -159,6 +160,9 @@ def lib():
                                                          _VP]
         _I = ctypes.POINTER(ctypes.c_int)
         L.fsmc_lsmc_plan_create.argtypes = [_VP, ctypes.c_int, PM] + [_D] * 5 + [_I, _I, ctypes.POINTER(_VP)]
+        L.fsmc_lsmc_plan_create_sweep.argtypes = [_VP, ctypes.c_int, PM] + [_D] * 5 + [_I, _I, ctypes.c_int,
+                                                                                  ctypes.POINTER(_VP)]
+        L.fsmc_lsmc_plan_sweep.argtypes = [_VP]
         L.fsmc_lsmc_plan_workspace.restype = ctypes.c_size_t
         L.fsmc_lsmc_plan_workspace.argtypes = [_VP]
         L.fsmc_lsmc_plan_run.argtypes = [_VP, _VP]
@@ -286,8 +290,9 @@ class Engine:
                                         pa.ctypes.data_as(_I), ppr, peu), "fsmc_american_lsmc")
         return pr[:, 0].copy(), pr[:, 1].copy(), eu
 
-    def lsmc_plan(self, S0, K, sigma, r, T, steps, paths, m):
-        return LsmcPlan(self, S0, K, sigma, r, T, steps, paths, m)
+    def lsmc_plan(self, S0, K, sigma, r, T, steps, paths, m, sweep=None):
+        """sweep: None (FSMC_LSMC_SWEEP_AUTO), LSMC_RESIDENT or LSMC_STREAMED."""
+        return LsmcPlan(self, S0, K, sigma, r, T, steps, paths, m, sweep)
 
     def american_lsmc_ref(self, n_paths, n_steps, spot, strike, vol, maturity, m, z=None):
         """The sample's own LSMC algorithm; returns (american, european)."""
@@ -408,16 +413,22 @@ class MT19937:
 class LsmcPlan:
     """A batched LSMC plan: option table on the device, workspace owned."""
 
-    def __init__(self, engine, S0, K, sigma, r, T, steps, paths, m):
+    def __init__(self, engine, S0, K, sigma, r, T, steps, paths, m, sweep=None):
         self.n = len(S0)
         self._p = _VP()
         arrs = [_d(a) for a in (S0, K, sigma, r, T)]
         st = np.ascontiguousarray(steps, dtype=np.int32)
         pa = np.ascontiguousarray(paths, dtype=np.int32)
         _I = ctypes.POINTER(ctypes.c_int)
-        _check(lib().fsmc_lsmc_plan_create(engine._ctx, self.n, ctypes.byref(m), *[q for _, q in arrs],
-                                           st.ctypes.data_as(_I), pa.ctypes.data_as(_I), ctypes.byref(self._p)),
-               "fsmc_lsmc_plan_create")
+        _check(lib().fsmc_lsmc_plan_create_sweep(engine._ctx, self.n, ctypes.byref(m), *[q for _, q in arrs],
+                                                 st.ctypes.data_as(_I), pa.ctypes.data_as(_I),
+                                                 LSMC_AUTO if sweep is None else sweep, ctypes.byref(self._p)),
+               "fsmc_lsmc_plan_create_sweep")
+
+    @property
+    def sweep(self):
+        """LSMC_RESIDENT or LSMC_STREAMED: the sweep this plan runs."""
+        return lib().fsmc_lsmc_plan_sweep(self._p)
 
     def workspace_bytes(self):
         return lib().fsmc_lsmc_plan_workspace(self._p)

@@ -300,6 +300,28 @@ typedef struct fsmc_lsmc_plan fsmc_lsmc_plan;
 int fsmc_lsmc_plan_create(fsmc_ctx* ctx, int n_options, const fsmc_mode* m, const double* S0, const double* K,
                           const double* sigma, const double* r, const double* T, const int* steps,
                           const int* paths, fsmc_lsmc_plan** out);
+/* Sweeps of a plan (no counterpart in the sample, whose sweep is one loop over
+ * a stored path matrix, MonteCarloAmericanOptions/main.cpp:131-266):
+ *   FSMC_LSMC_SWEEP_RESIDENT: the whole backward sweep of every option is one
+ *     cooperative launch; an option's paths are cut into parts of 8192 (FP32)
+ *     or 4096 (FP64) paths, one workgroup each, whose bridge state and cash
+ *     flows stay in registers; the parts of an option meet once per date
+ *     through device-memory records (no grid-wide barrier).  Needs every
+ *     option's parts to fit the resident grid at once;
+ *   FSMC_LSMC_SWEEP_STREAMED: one launch per date, state streamed through
+ *     device memory (any sizes);
+ *   FSMC_LSMC_SWEEP_AUTO: resident when it fits, else streamed.
+ * Both draw the same normals, so they price the same paths; the moment sums
+ * are combined in different orders (FP64 results agree to rounding).
+ * fsmc_lsmc_plan_create uses AUTO; fsmc_lsmc_plan_sweep returns the sweep a
+ * plan runs (RESIDENT or STREAMED). */
+#define FSMC_LSMC_SWEEP_AUTO 0
+#define FSMC_LSMC_SWEEP_RESIDENT 1
+#define FSMC_LSMC_SWEEP_STREAMED 2
+int fsmc_lsmc_plan_create_sweep(fsmc_ctx* ctx, int n_options, const fsmc_mode* m, const double* S0,
+                                const double* K, const double* sigma, const double* r, const double* T,
+                                const int* steps, const int* paths, int sweep, fsmc_lsmc_plan** out);
+int fsmc_lsmc_plan_sweep(const fsmc_lsmc_plan* plan);
 size_t fsmc_lsmc_plan_workspace(const fsmc_lsmc_plan* plan);
 int fsmc_lsmc_plan_run(fsmc_lsmc_plan* plan, void* stream);
 int fsmc_lsmc_plan_results(fsmc_lsmc_plan* plan, void* stream, double* price, double* european);

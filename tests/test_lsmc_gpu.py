@@ -163,3 +163,86 @@ def test_reference_algorithm_on_sample_stream(engine, loop):
     oz = _sample_stream_buffer(orc.mrg32k3a_gaussian, n_paths, n_steps, loop)
     oam, oeu = orc.lsmc_ref(n_paths, n_steps, 100.0, 110.0, .25, 2.0, oz)
     assert abs(am - oam) <= 1e-12 * abs(oam) and abs(eu - oeu) <= 1e-12 * abs(oeu)
+
+
+# ---- resident sweep (one cooperative launch, state in registers) ----------
+def _plan_prices(engine, P, m, sweep):
+    plan = engine.lsmc_plan(P["S0"], P["K"], P["sigma"], P["r"], P["T"], P["steps"], P["paths"], m, sweep=sweep)
+    try:
+        got = plan.sweep
+        plan.run()
+        return got, plan.results()
+    finally:
+        plan.close()
+
+
+def test_resident_fp64_matches_oracle(engine):
+    # uneven sizes: parts of 4096 paths, a last part of 1 path, paths not a multiple of 4, one-step and two-step options
+    P = dict(S0=[90.0, 100.0, 110.0, 95.0, 100.0, 85.0], K=[100.0] * 6, sigma=[.2, .3, .4, .15, .25, .35],
+             r=[.03] * 6, T=[1.0, .5, 2.0, .25, 1.0, .75], steps=[20, 9, 13, 2, 1, 17],
+             paths=[9000, 1025, 4097, 700, 3000, 8192])
+    got, (p, se, eu) = _plan_prices(engine, P, mc.mode(mc.FP64, seed=21), mc.LSMC_RESIDENT)
+    assert got == mc.LSMC_RESIDENT
+    op, ose, oeu = orc.lsmc(P["S0"], P["K"], P["sigma"], P["r"], P["T"], P["steps"], P["paths"], seed=21)
+    assert np.allclose(p, op, rtol=1e-12, atol=0)
+    assert np.allclose(se, ose, rtol=1e-9, atol=0)
+    assert np.allclose(eu, oeu, rtol=1e-12, atol=0)
+
+
+def test_resident_and_streamed_sweeps_agree_256_puts(engine):
+    # the 256-put workload: both sweeps price the same paths; FP64 differs only
+    # by the order the moment sums are combined in, FP32 by the rounding of
+    # that order (a handful of exercise decisions at the regression boundary)
+    P = _opts(256, 3)
+    res = {}
+    for prec in (mc.FP32, mc.FP64):
+        for sweep in (mc.LSMC_RESIDENT, mc.LSMC_STREAMED):
+            got, out = _plan_prices(engine, P, mc.mode(prec, seed=3), sweep)
+            assert got == sweep
+            res[prec, sweep] = out
+    p_r, se_r, eu_r = res[mc.FP64, mc.LSMC_RESIDENT]
+    p_s, se_s, eu_s = res[mc.FP64, mc.LSMC_STREAMED]
+    assert np.allclose(p_r, p_s, rtol=1e-11, atol=0)
+    assert np.allclose(eu_r, eu_s, rtol=1e-12, atol=0)
+    q_r, _, qeu_r = res[mc.FP32, mc.LSMC_RESIDENT]
+    q_s, _, qeu_s = res[mc.FP32, mc.LSMC_STREAMED]
+    d = np.abs(q_r - q_s) / se_s
+    assert np.median(d) < 0.1 and np.max(d) < 1.0, (np.median(d), np.max(d))
+    assert np.allclose(qeu_r, qeu_s, rtol=1e-5, atol=0)
+    # and the resident FP32 against FP64 on the same stream, as for the streamed sweep
+    d64 = np.abs(q_r - p_s) / se_s
+    assert np.median(d64) < 0.15 and np.max(d64) < 1.5, (np.median(d64), np.max(d64))
+
+
+def test_resident_reruns_are_identical(engine):
+    P = _opts(40, 6)
+    m = mc.mode(mc.FP32, seed=11)
+    plan = engine.lsmc_plan(P["S0"], P["K"], P["sigma"], P["r"], P["T"], P["steps"], P["paths"], m,
+                            sweep=mc.LSMC_RESIDENT)
+    try:
+        assert plan.sweep == mc.LSMC_RESIDENT
+        plan.run()
+        a = plan.results()
+        plan.run()
+        b = plan.results()
+    finally:
+        plan.close()
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    # the default sweep of a plan that fits is the resident one
+    c = _call(engine, P, m)
+    assert all(np.array_equal(x, y) for x, y in zip(a, c))
+
+
+def test_resident_sweep_needs_every_option_to_fit(engine):
+    # an option of more parts than the resident grid holds: AUTO streams it,
+    # RESIDENT refuses before any launch
+    P = dict(S0=[100.0], K=[100.0], sigma=[.2], r=[.03], T=[1.0], steps=[3], paths=[4096 * 4096])
+    m = mc.mode(mc.FP32, seed=1)
+    plan = engine.lsmc_plan(P["S0"], P["K"], P["sigma"], P["r"], P["T"], P["steps"], P["paths"], m)
+    try:
+        assert plan.sweep == mc.LSMC_STREAMED
+    finally:
+        plan.close()
+    with pytest.raises(mc.FsmcError):
+        engine.lsmc_plan(P["S0"], P["K"], P["sigma"], P["r"], P["T"], P["steps"], P["paths"], m,
+                         sweep=mc.LSMC_RESIDENT)
