@@ -17,7 +17,9 @@ on the launching stream, in FP32 and in plain FP64:
   libor_greeks   262144 paths of the 80-rate / 40-step LMM, 15 swaptions,
                  value + 80 adjoint deltas;
   american_lsmc  256 puts, S0~U[80,120], K=100, sigma~U[.1,.5], r=.03,
-                 T~U[.25,2], steps~U{50..250}, paths~U{16384..262144};
+                 T~U[.25,2], steps~U{50..250}, paths~U{16384..262144}; the
+                 plan's default sweep (one cooperative launch) and the
+                 streamed sweep (one launch per date) beside it;
   serving_batch  4096 mixed GBM / Heston requests with delta, vega, rho;
                  autotuned, p50 / p99 request completion latency.
 """
@@ -383,17 +385,25 @@ def _lsmc_config(np, torch, mc, eng, precisions, steps):
     K, rr = np.full(n, 100.0), np.full(n, 0.03)
     st, pa = rng.integers(50, 251, n), rng.integers(16384, 262145, n)
     ps = int(np.sum(st.astype(np.int64) * pa))
-    res = {}
+    res, streamed, sweep = {}, {}, {}
     for name, m in precisions:
+        # the default plan (the resident sweep: one cooperative launch) and the
+        # streamed sweep (one launch per date) on the same paths
         plan = mc.LsmcPlan(eng, S0, K, sg, rr, T, st, pa, m)
+        sweep[name] = {mc.LSMC_RESIDENT: "resident", mc.LSMC_STREAMED: "streamed"}[plan.sweep]
         ms = _timed(torch, np, lambda: plan.run(mc._stream()), steps, 1)
         pr, se, _ = plan.results(mc._stream())
         plan.close()
         res[name] = (ms, pr, se)
+        plan = mc.LsmcPlan(eng, S0, K, sg, rr, T, st, pa, m, mc.LSMC_STREAMED)
+        streamed[name] = _timed(torch, np, lambda: plan.run(mc._stream()), steps, 1)
+        plan.close()
     out = {"options": n, "path_steps": ps}
     for name in res:
         out["ms_" + name] = res[name][0]
         out["path_steps_per_s_" + name] = ps / (res[name][0] * 1e-3)
+        out["sweep_" + name] = sweep[name]
+        out["ms_streamed_sweep_" + name] = streamed[name]
     if "fp64" in res and "fp32" in res:
         out.update({"fp32_speedup_over_fp64": res["fp64"][0] / res["fp32"][0],
                     "fp32_vs_fp64_max_err_in_se": float(np.max(np.abs(res["fp32"][1] - res["fp64"][1])
