@@ -15,7 +15,8 @@ on the launching stream, in FP32 and in plain FP64:
   gbm_european   1024 calls x 262144 paths (S0~U[5,30], K~U[1,100],
                  T~U[.25,10], r=.02, sigma=.3), error against Black-Scholes;
   libor_greeks   262144 paths of the 80-rate / 40-step LMM, 15 swaptions,
-                 value + 80 adjoint deltas;
+                 value + 80 adjoint deltas; FP32 with the trajectory
+                 checkpointed every 8 steps (and without, beside it);
   american_lsmc  256 puts, S0~U[80,120], K=100, sigma~U[.1,.5], r=.03,
                  T~U[.25,2], steps~U{50..250}, paths~U{16384..262144}; the
                  plan's default sweep (one cooperative launch) and the
@@ -336,14 +337,21 @@ def _libor_config(np, torch, mc, eng, precisions, steps):
     """LIBOR swaption portfolio, 262144 paths, value + 80 adjoint deltas."""
     dev = torch.device("cuda:0")
     lp = 262144
-    res = {}
+    res, nock = {}, None
     for name, m in precisions:
-        ws = torch.empty(mc.libor_workspace(lp, m), dtype=torch.uint8, device=dev)
+        # FP32 as configured: the trajectory checkpointed every 8 steps (the
+        # rebuild restarts from each checkpoint); FP64 stores it whole
+        ck = 8 if name == "fp32" else 0
+        ws = torch.empty(mc.libor_workspace(lp, m, checkpoint_every=ck), dtype=torch.uint8, device=dev)
         v = torch.zeros(2, dtype=torch.float64, device=dev)
         d = torch.zeros(160, dtype=torch.float64, device=dev)
-        ms = _timed(torch, np, lambda: mc.libor_async(lp, m, v, d, ws), steps, 2)
+        ms = _timed(torch, np, lambda: mc.libor_async(lp, m, v, d, ws, checkpoint_every=ck), steps, 2)
         res[name] = (ms, v.cpu().numpy(), d.cpu().numpy())
         del ws
+        if name == "fp32":  # and rebuilt from the final rates alone, without checkpoints
+            ws = torch.empty(mc.libor_workspace(lp, m), dtype=torch.uint8, device=dev)
+            nock = _timed(torch, np, lambda: mc.libor_async(lp, m, v, d, ws), steps, 2)
+            del ws
     out = {"paths": lp, "flop_per_path": LIBOR_FLOP_PER_PATH}
     for name in res:
         ms = res[name][0]
@@ -354,7 +362,8 @@ def _libor_config(np, torch, mc, eng, precisions, steps):
         out["path_steps_per_s_" + name] = 40 * lp / (ms * 1e-3)
     if "fp32" in res:
         achieved = LIBOR_FLOP_PER_PATH * lp / (res["fp32"][0] * 1e-3) / 1e12
-        out.update({"achieved_tflops_fp32": achieved, "frac_fp32_peak": achieved / FP32_VECTOR_PEAK_TFLOPS})
+        out.update({"checkpoint_every_fp32": 8, "ms_fp32_no_checkpoints": nock,
+                    "achieved_tflops_fp32": achieved, "frac_fp32_peak": achieved / FP32_VECTOR_PEAK_TFLOPS})
         # The same algorithm as the FP64 kernel (trajectory stored in HBM):
         # the FP32 kernel stores it above lambda = 0.3 instead of rebuilding
         # rows, and lambda does not change its instruction count.

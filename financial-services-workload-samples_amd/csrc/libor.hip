@@ -214,6 +214,27 @@ __global__ __launch_bounds__(kBlock) void libor_path_kernel(LiborArgs a) {
 // inverse for every rate (2.3e-5 at lambda = 0.75).
 constexpr double kRebuildMaxLambda = 0.3;
 
+// Checkpoints of the FP32 rebuild kernel (kCkpt, fsmc_libor_params::
+// checkpoint_every = 8): the forward sweep stores rows 8, 16, 24 and 32
+// (entries i >= row), and the reverse sweep replaces the rebuilt row by the
+// stored one at the top of each segment, so a segment's rows are recomputed
+// from the exact row above it and the series' truncation never accumulates
+// over more than kCkptEvery steps.  [slot][path] layout; row c starts at slot
+// ckpt_base(c).  Measured on the 262,144-path batch: 0.64 ms without
+// checkpoints, 0.71 ms with them (the 251 MB of stores cost ~20 us, the
+// reloads, which every wave stalls on at nearly the same step, ~45 us; rows
+// staggered over the waves measured 0.74 ms, and reading the checkpoint in
+// place of the step's inversion spilled the kernel).
+constexpr int kCkptEvery = 8;
+__host__ __device__ constexpr bool ckpt_row(int c) { return c < NMAT && c % kCkptEvery == 0 && c > 0; }
+__host__ __device__ constexpr int ckpt_base(int c) {  // rows 8j below c take 80 - 8j slots each
+  return (c / kCkptEvery - 1) * NN - 4 * (c / kCkptEvery - 1) * (c / kCkptEvery) + 0 * c;
+}
+constexpr int kCkptSlots = 240;  // rows 8, 16, 24, 32: 72 + 64 + 56 + 48
+static_assert(ckpt_base(8) == 0 && ckpt_base(16) == 72 && ckpt_base(24) == 136 && ckpt_base(32) == 192,
+              "checkpoint slots");
+static_assert(ckpt_base(32) + NN - 32 == kCkptSlots, "checkpoint slots");
+
 // FP32 variant without the trajectory buffer.  The reverse sweep rebuilds row
 // n from row n+1 instead of reading it back: for i = n+1, n+2, ... the forward
 // map L(n+1,i) = x exp(con1 (v(i-1) + con1 x / (1 + delta x)) + c2(n)) with
@@ -227,9 +248,10 @@ constexpr double kRebuildMaxLambda = 0.3;
 // FP32 rounding for lambda <= kRebuildMaxLambda; larger lambdas take the
 // stored-trajectory kernel.  The adjoint then runs in the same ascending
 // order: its running sum v1(i) = sum_{k >= i} lambda L(n+1,k) Lb(k) is taken
-// as the row total minus the prefix below i.  No HBM traffic beyond the
-// [81][path] outputs.
-template <bool kBuffer, bool kInplace>
+// as the row total minus the prefix below i.  Rows 8, 16, 24 and 32 are
+// checkpointed in the forward sweep (kCkptEvery) and restore the rebuild at
+// the top of each segment; the other HBM traffic is the [81][path] outputs.
+template <bool kBuffer, bool kInplace, bool kCkpt>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void libor_path_rc_kernel(
     LiborArgs a) {  // three waves per SIMD: <= 168 registers per lane; the sweeps do not spill (the portfolio
                     // section around them does, once per path)
@@ -250,6 +272,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 #pragma unroll
   for (int i = 0; i < NN; ++i) L[i] = L0;
   float zq[4] = {0.f, 0.f, 0.f, 0.f};
+  float* ckpt = (float*)a.traj + p;
   for (int n = 0; n < NMAT; ++n) {
     const float sqez = sqrt_delta * normal_at<float, kBuffer>(a, p, n, zq);
     const float c2 = lam * (sqez - 0.5f * con1) * O::kE;
@@ -261,6 +284,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         vs = __builtin_fmaf(Li, __builtin_amdgcn_rcpf(__builtin_fmaf(delta, Li, 1.f)), vs);
         L[i] = Li * O::ex(__builtin_fmaf(kk, vs, c2));
       }
+    }
+    if (kCkpt && ckpt_row(n + 1)) {  // uniform: row n + 1 is a checkpoint, entries i >= n + 1
+      const int cb = ckpt_base(n + 1) - (n + 1);  // slot of entry i is cb + i
+#pragma unroll
+      for (int i = 1; i < NN; ++i)
+        if (i > n) ckpt[(size_t)(cb + i) * P] = L[i];
     }
   }
 
@@ -291,6 +320,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   // prefix of the row total below i.
   const float con1sq = con1 * con1;
   for (int n = NMAT - 1; n >= 0; --n) {
+    if (kCkpt && ckpt_row(n + 1)) {  // uniform: the stored row n + 1 replaces the rebuilt one
+      const int cb = ckpt_base(n + 1) - (n + 1);
+#pragma unroll
+      for (int i = 1; i < NN; ++i) {
+        if (i > n) {
+          const float y = ckpt[(size_t)(cb + i) * P];
+          if (i < NMAT)
+            Llo[i][threadIdx.x] = y;
+          else
+            L[i] = y;
+        }
+      }
+    }
     float z;
     if (kBuffer) {
       z = (float)a.z[(size_t)p * NMAT + n];
@@ -408,8 +450,11 @@ LiborPlan libor_plan(const fsmc_libor_params* p, const fsmc_mode* m) {
   LiborPlan l;
   l.real = m->precision == FSMC_FP64 ? sizeof(double) : sizeof(float);
   const size_t P = (size_t)p->n_paths;
-  // the FP32 kernel rebuilds the trajectory in registers up to kRebuildMaxLambda
-  l.traj_bytes = stores_trajectory(p, m) ? align_up((size_t)kSlots * P * l.real, 256) : 0;
+  // the FP32 kernel rebuilds the trajectory in registers up to kRebuildMaxLambda, from
+  // checkpoint rows every kCkptEvery steps
+  l.traj_bytes = stores_trajectory(p, m) ? align_up((size_t)kSlots * P * l.real, 256)
+                 : p->checkpoint_every ? align_up((size_t)kCkptSlots * P * l.real, 256)
+                                       : 0;
   l.out_bytes = align_up((size_t)(NN + 1) * P * l.real, 256);
   l.part_bytes = align_up((size_t)(NN + 1) * kReduceChunks * 2 * sizeof(double), 256);
   l.ws = l.traj_bytes + l.out_bytes + l.part_bytes;
@@ -422,22 +467,28 @@ int libor_validate(const fsmc_libor_params* p, const fsmc_mode* m, bool mrg_allo
   FSMC_REQUIRE(p->n_paths >= 2 && p->n_paths <= (1 << 30), "n_paths %d outside [2, 2^30]", p->n_paths);
   FSMC_REQUIRE(p->L0 > 0.0, "L0 must be > 0");
   FSMC_REQUIRE(p->lambda >= 0.0, "lambda must be >= 0");
+  FSMC_REQUIRE(p->checkpoint_every == 0 || p->checkpoint_every == kCkptEvery,
+               "checkpoint_every %d: 0 or %d", p->checkpoint_every, kCkptEvery);
   return FSMC_OK;
 }
 
 template <typename Real, bool B, bool Q>
-void launch(const LiborArgs& a, hipStream_t s) {
+void launch(const LiborArgs& a, bool ckpt, hipStream_t s) {
   const dim3 grid((a.n_paths + kBlock - 1) / kBlock);
-  if (sizeof(Real) == 4 && a.lambda <= kRebuildMaxLambda)
-    hipLaunchKernelGGL((libor_path_rc_kernel<B, Q>), grid, dim3(kBlock), 0, s, a);
-  else
+  if (sizeof(Real) == 4 && a.lambda <= kRebuildMaxLambda) {
+    if (ckpt)
+      hipLaunchKernelGGL((libor_path_rc_kernel<B, Q, true>), grid, dim3(kBlock), 0, s, a);
+    else
+      hipLaunchKernelGGL((libor_path_rc_kernel<B, Q, false>), grid, dim3(kBlock), 0, s, a);
+  } else {
     hipLaunchKernelGGL((libor_path_kernel<Real, B, Q>), grid, dim3(kBlock), 0, s, a);
+  }
 }
 
 template <typename Real>
-void launch_all(const LiborArgs& a, bool buffer, bool q, hipStream_t s) {
-  if (buffer) q ? launch<Real, true, true>(a, s) : launch<Real, true, false>(a, s);
-  else q ? launch<Real, false, true>(a, s) : launch<Real, false, false>(a, s);
+void launch_all(const LiborArgs& a, bool buffer, bool q, bool ckpt, hipStream_t s) {
+  if (buffer) q ? launch<Real, true, true>(a, ckpt, s) : launch<Real, true, false>(a, ckpt, s);
+  else q ? launch<Real, false, true>(a, ckpt, s) : launch<Real, false, false>(a, ckpt, s);
 }
 
 }  // namespace
@@ -479,8 +530,9 @@ int fsmc_libor_swaption_greeks_async(const fsmc_libor_params* p, const fsmc_mode
   double* part = (double*)(ws + l.traj_bytes + l.out_bytes);
   const bool q = (m->quirks & FSMC_QUIRK_LIBOR_INPLACE_ADJ) != 0;
   const bool f64 = m->precision == FSMC_FP64;
-  if (f64) launch_all<double>(a, buffer, q, s);
-  else launch_all<float>(a, buffer, q, s);
+  const bool ckpt = p->checkpoint_every != 0;
+  if (f64) launch_all<double>(a, buffer, q, ckpt, s);
+  else launch_all<float>(a, buffer, q, ckpt, s);
   FSMC_HIP(hipGetLastError());
   if (f64)
     hipLaunchKernelGGL(libor_reduce_kernel<double>, dim3(kReduceChunks, NN + 1), dim3(256), 0, s, p->n_paths,

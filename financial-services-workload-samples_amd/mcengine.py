@@ -53,7 +53,8 @@ class GbmParams(ctypes.Structure):
 
 
 class LiborParams(ctypes.Structure):
-    _fields_ = [("n_paths", ctypes.c_int), ("L0", ctypes.c_double), ("lam", ctypes.c_double)]
+    _fields_ = [("n_paths", ctypes.c_int), ("L0", ctypes.c_double), ("lam", ctypes.c_double),
+                ("checkpoint_every", ctypes.c_int)]
 
 
 class LsmcRefParams(ctypes.Structure):
@@ -264,9 +265,9 @@ class Engine:
 
 
     # -- LIBOR swaption portfolio -------------------------------------------
-    def libor(self, n_paths, m, L0=0.05, lam=0.2, z=None):
+    def libor(self, n_paths, m, L0=0.05, lam=0.2, z=None, checkpoint_every=0):
         """Returns (value, value_se, deltas[80], deltas_se[80])."""
-        p = LiborParams(n_paths, L0, lam)
+        p = LiborParams(n_paths, L0, lam, checkpoint_every)
         pz = None
         if z is not None:
             z, pz = _d(np.asarray(z).reshape(-1))
@@ -692,18 +693,18 @@ def heston_asian_noalloc(params, n_paths, n_steps, r, rho, m, out_res, out_conf,
                                            _stream()), where)
 
 
-def libor_workspace(n_paths, m, L0=0.05, lam=0.2):
-    p = LiborParams(n_paths, L0, lam)
+def libor_workspace(n_paths, m, L0=0.05, lam=0.2, checkpoint_every=0):
+    p = LiborParams(n_paths, L0, lam, checkpoint_every)
     return lib().fsmc_libor_workspace(ctypes.byref(p), ctypes.byref(m))
 
 
-def libor_async(n_paths, m, out_value, out_deltas, workspace, L0=0.05, lam=0.2, z=None):
+def libor_async(n_paths, m, out_value, out_deltas, workspace, L0=0.05, lam=0.2, z=None, checkpoint_every=0):
     """out_value: 2 doubles, out_deltas: 160 doubles (device tensors)."""
     where = "fsmc_libor_swaption_greeks_async"
     _need(out_value, "float64", 2, "out_value", where)
     _need(out_deltas, "float64", 160, "out_deltas", where)
     _need(z, "float64", n_paths * 40, "z", where)
-    p = LiborParams(n_paths, L0, lam)
+    p = LiborParams(n_paths, L0, lam, checkpoint_every)
     ws_bytes = _workspace(workspace, lib().fsmc_libor_workspace(ctypes.byref(p), ctypes.byref(m)), where)
     _check(lib().fsmc_libor_swaption_greeks_async(ctypes.byref(p), ctypes.byref(m), _ptr(z), _ptr(out_value),
                                                   _ptr(out_deltas), _ptr(workspace), ws_bytes, _stream()), where)

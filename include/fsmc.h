@@ -184,11 +184,19 @@ int fsmc_heston_asian_noalloc(const fsmc_heston_params* p, const fsmc_mode* m, c
  * Buffer mode consumes z laid out [n_paths][40] (z[path*40 + n], :386).
  * Outputs: value[2] = {mean discounted portfolio value, its standard error};
  * deltas[160] = {dV/dL0_i for i = 0..79, then their standard errors}.  The
- * means are per path (the sample prints 10x these, NLOOPS = 10, :355-356). */
+ * means are per path (the sample prints 10x these, NLOOPS = 10, :355-356).
+ * Trajectory (the sample stores all 41 x 80 rates of a path, :76,106-107):
+ * FP64 stores the 2380 updated rates per path in the workspace; FP32 (lambda
+ * <= 0.3) rebuilds row n from row n + 1 in the reverse sweep by inverting the
+ * step (truncation <= 1.5e-8 relative), with checkpoint_every = 8 from rows
+ * 8, 16, 24 and 32 stored by the forward sweep (each segment recomputed from
+ * the exact row at its top; 240 floats per path of workspace), with 0 from
+ * the final rates alone. */
 typedef struct fsmc_libor_params {
   int n_paths;
   double L0;
   double lambda;
+  int checkpoint_every; /* 0 or 8: see above */
 } fsmc_libor_params;
 
 /* m->rng = FSMC_RNG_MRG32K3A (synchronous entry point only): z is not read;

@@ -95,6 +95,15 @@ def test_workspace_sizes(lib):
         ws = lib.fsmc_libor_workspace(ctypes.byref(lp), ctypes.byref(mcengine.mode(prec)))
         real = 8 if prec == mcengine.FP64 else 4
         assert ws > 0 and (ws >= 2380 * 4096 * real) == stored
+    # checkpoints every 8 steps: rows 8, 16, 24, 32 (240 floats per path) on
+    # top of the outputs; any other period is refused
+    base = lib.fsmc_libor_workspace(ctypes.byref(mcengine.LiborParams(4096, 0.05, 0.2)),
+                                    ctypes.byref(mcengine.mode(mcengine.FP32)))
+    ck = lib.fsmc_libor_workspace(ctypes.byref(mcengine.LiborParams(4096, 0.05, 0.2, 8)),
+                                  ctypes.byref(mcengine.mode(mcengine.FP32)))
+    assert 240 * 4096 * 4 <= ck - base < 2380 * 4096 * 4
+    assert lib.fsmc_libor_workspace(ctypes.byref(mcengine.LiborParams(4096, 0.05, 0.2, 4)),
+                                    ctypes.byref(mcengine.mode(mcengine.FP32))) == 0
 
 
 def test_bench_refuses_more_than_one_gpu():

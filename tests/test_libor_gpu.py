@@ -117,6 +117,33 @@ def test_portfolio_262144_paths_fp32_vs_fp64(engine):
     assert np.sum(d64) == pytest.approx((up - dn) / (2 * h), rel=1e-5)
 
 
+@pytest.mark.parametrize("lam", [0.2, 0.3])
+def test_fp32_checkpointed_every_8_steps(engine, lam):
+    # checkpoint_every = 8: rows 8, 16, 24, 32 stored by the forward sweep and
+    # each segment rebuilt from the exact row at its top.  Against the
+    # oracle on the same normals (FP64 reference arithmetic) and against the
+    # kernel without checkpoints, which differs only by the inversion's
+    # truncation carried across segments
+    n = 2048
+    z = np.random.default_rng(2).standard_normal((n, 40))
+    m = mc.mode(mc.FP32, mc.RNG_BUFFER)
+    v, vse, d, dse = engine.libor(n, m, L0=0.051, lam=lam, z=z, checkpoint_every=8)
+    ov, ovse, od, odse = orc.libor(n, L0=0.051, lam=lam, z=z)
+    assert abs(v - ov) <= 1e-5 * abs(ov)
+    assert np.all(np.abs(d - od) <= 1e-5 * np.abs(od) + 1e-3 * odse)
+    v0, _, d0, _ = engine.libor(n, m, L0=0.051, lam=lam, z=z)
+    assert v == v0  # the value is the forward's, untouched by the reverse sweep
+    assert np.all(np.abs(d - d0) <= 2e-6 * np.abs(od) + 1e-4 * odse)
+    # and at full size on the Philox stream against FP64
+    n = 262144
+    v32, _, d32, _ = engine.libor(n, mc.mode(mc.FP32, seed=1), L0=0.05, lam=lam, checkpoint_every=8)
+    v64, s64, d64, ds64 = engine.libor(n, mc.mode(mc.FP64, seed=1), L0=0.05, lam=lam)
+    assert abs(v32 - v64) < 0.01 * s64
+    assert np.all(np.abs(d32 - d64) < 0.01 * ds64)
+    with pytest.raises(mc.FsmcError):
+        engine.libor(64, m, z=z[:64], checkpoint_every=4)
+
+
 def test_deterministic_and_async(engine):
     import torch
     n = 3000
