@@ -160,29 +160,35 @@ __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(BatchArgs a, c
 // indices: the loop ends for every workgroup whatever the others do.  Nothing waits on another workgroup and
 // nothing needs a fence: the items' inputs are written before the launch
 // and their partials are read by the next kernel.  *head is zeroed by
-// batch_reset_kernel earlier on the same stream.
-template <typename Real>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 3 : 1))) void
-batch_queue_kernel(BatchArgs a, const int* order, int n_items, int* head) {  // FP32: <= 168 registers, three
-                                                                             // waves per SIMD as the grouped
-                                                                             // Heston grid gets
+// batch_reset_kernel earlier on the same stream.  One queue per request
+// kind (kHeston): a kernel that could run either kind carries the union of
+// their registers (100 VGPRs, four waves per SIMD, where the Heston items
+// alone need 68 and run at seven), so a micro-batch runs its Heston queue
+// and then its GBM queue.
+template <typename Real, bool kHeston>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 7 : 2))) void
+batch_queue_kernel(BatchArgs a, const int* order, int n_items, int* head) {  // FP32: <= 72 VGPRs, seven waves
+                                                                             // per SIMD as the grouped Heston
+                                                                             // grid gets
   __shared__ double lds[kNV * (kBlock / 64)];
   __shared__ int next;
   if (threadIdx.x == 0) next = atomicAdd(head, 1);
   __syncthreads();
-  int j = next;
+  // the index is uniform: readfirstlane keeps it (and everything the item
+  // derives from it: addresses, request parameters) in scalar registers
+  int j = __builtin_amdgcn_readfirstlane(next);
   __syncthreads();  // every thread has read next before thread 0 replaces it
   while (j < n_items) {  // uniform per workgroup
     // thread 0 takes the following index now; its latency overlaps the item
     const int ahead = threadIdx.x == 0 ? atomicAdd(head, 1) : 0;
-    const int item = order[j];
-    if (a.req[a.items[item].x].kind == FSMC_REQ_GBM_EUROPEAN)  // uniform per workgroup
-      gbm_item<Real>(a, item, lds);
-    else
+    const int item = __builtin_amdgcn_readfirstlane(order[j]);
+    if (kHeston)
       heston_item<Real>(a, item, lds);
+    else
+      gbm_item<Real>(a, item, lds);
     if (threadIdx.x == 0) next = ahead;
     __syncthreads();  // next is written, and thread 0's reads of lds in block_sum precede the next item's writes
-    j = next;
+    j = __builtin_amdgcn_readfirstlane(next);
     __syncthreads();  // every thread has read next before thread 0 replaces it
   }
 }
@@ -224,7 +230,8 @@ struct MicroBatch {
   int gbm_first, gbm_count;  // items
   int hes_first, hes_count;
   int off_first;             // persistent mode: this micro-batch's G + 1 schedule offsets
-  int q_first, q_count;      // work-queue mode: its items' longest-first order in qorder
+  int q_first, q_count;      // work-queue mode: its items' order in qorder (Heston items, then GBM items)
+  int q_hes;                 // the Heston items among them
 };
 
 struct BatchImpl {
@@ -367,10 +374,15 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
         cost.push_back({(double)paths * (r.kind == FSMC_REQ_GBM_EUROPEAN ? 1.0 : 1.7 * r.n_steps), it});
       }
       std::stable_sort(cost.begin(), cost.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
-      // the work queue hands the items out in this order
+      // the work queues hand the items out in this order, one queue per kind:
+      // the Heston items longest first, then the GBM items longest first
       mb.q_first = (int)qorder.size();
       mb.q_count = (int)cost.size();
-      for (const auto& c : cost) qorder.push_back(c.second);
+      for (const auto& c : cost)
+        if (b.req[items[c.second].x].kind != FSMC_REQ_GBM_EUROPEAN) qorder.push_back(c.second);
+      mb.q_hes = (int)qorder.size() - mb.q_first;
+      for (const auto& c : cost)
+        if (b.req[items[c.second].x].kind == FSMC_REQ_GBM_EUROPEAN) qorder.push_back(c.second);
       {
         // the order is a permutation of the micro-batch's items (checked
         // here, so the kernel's reads order[j], j < q_count, stay in bounds)
@@ -435,7 +447,7 @@ static int batch_configure(fsmc_ctx* ctx, BatchImpl& b, fsmc_batch_tuning t) {
   b.off_sched = take(sizeof(int) * std::max<size_t>(1, sched.size()));
   b.off_soff = take(sizeof(int) * std::max<size_t>(1, soff.size()));
   b.off_qorder = take(sizeof(int) * std::max<size_t>(1, qorder.size()));
-  b.off_heads = take(sizeof(int) * b.waves.size());
+  b.off_heads = take(sizeof(int) * 2 * b.waves.size());  // a Heston and a GBM queue per micro-batch
   FSMC_TRY(b.ws.reserve(off));
   char* ws = (char*)b.ws.ptr;
   hipStream_t s = ctx->stream;
@@ -473,8 +485,8 @@ static int batch_enqueue(BatchImpl& b, hipStream_t s, bool stamps) {
   a.w_heston = counter_w(kTagHeston, b.mode.stream);
   unsigned long long* stamp = (unsigned long long*)(ws + b.off_stamp);
   int* heads = (int*)(ws + b.off_heads);
-  if (b.tuning.work_queue) {  // one counter per micro-batch, zeroed by a kernel (no memset node in a graph)
-    const int nw = (int)b.waves.size();
+  if (b.tuning.work_queue) {  // two counters per micro-batch, zeroed by a kernel (no memset node in a graph)
+    const int nw = 2 * (int)b.waves.size();
     hipLaunchKernelGGL(batch_reset_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, heads, nw);
   }
   if (stamps)
@@ -485,8 +497,13 @@ static int batch_enqueue(BatchImpl& b, hipStream_t s, bool stamps) {
     const MicroBatch& mb = b.waves[w];
     if (b.tuning.workgroups_per_cu > 0 && b.tuning.work_queue) {
       const int G = b.cus * b.tuning.workgroups_per_cu;
-      hipLaunchKernelGGL(batch_queue_kernel<Real>, dim3(G), dim3(kBlock), 0, s, a,
-                         (const int*)(ws + b.off_qorder) + mb.q_first, mb.q_count, heads + w);
+      const int* q = (const int*)(ws + b.off_qorder) + mb.q_first;
+      if (mb.q_hes > 0)
+        hipLaunchKernelGGL((batch_queue_kernel<Real, true>), dim3(G), dim3(kBlock), 0, s, a, q, mb.q_hes,
+                           heads + 2 * w);
+      if (mb.q_count > mb.q_hes)
+        hipLaunchKernelGGL((batch_queue_kernel<Real, false>), dim3(G), dim3(kBlock), 0, s, a, q + mb.q_hes,
+                           mb.q_count - mb.q_hes, heads + 2 * w + 1);
     } else if (b.tuning.workgroups_per_cu > 0) {
       const int G = b.cus * b.tuning.workgroups_per_cu;
       hipLaunchKernelGGL(batch_persistent_kernel<Real>, dim3(G), dim3(kBlock), 0, s, a,
@@ -685,7 +702,7 @@ int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen) {
   if (st != FSMC_OK) return st;
   const int chunk = best.chunk_paths, per = best.requests_per_launch;
   for (int queue : {0, 1}) {
-    for (int wpc : {1, 2, 3, 4}) {  // the persistent kernels at this chunk and micro-batch size
+    for (int wpc : {1, 2, 3, 4, 6, 8}) {  // the persistent kernels at this chunk and micro-batch size
       float ms = 0.0f;
       const fsmc_batch_tuning p = {chunk, per, 0, wpc, queue};
       FSMC_TRY(batch_time(b, p, &ms));

@@ -102,7 +102,7 @@ def test_mixed_batch_reruns_and_tuning(engine):
     b2.close()
     assert np.allclose(other["price"], first["price"], rtol=1e-9, atol=1e-12)
     chosen = b.autotune()
-    assert chosen[0] % 1024 == 0 and chosen[1] >= 0 and chosen[2] in (0, 1) and chosen[3] in (0, 1, 2, 3, 4)
+    assert chosen[0] % 1024 == 0 and chosen[1] >= 0 and chosen[2] in (0, 1) and chosen[3] in (0, 1, 2, 3, 4, 6, 8)
     assert chosen[4] in (0, 1) and (chosen[4] == 0 or chosen[3] >= 1)
     assert b.tuning() == chosen
     b.run()
