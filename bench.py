@@ -44,6 +44,9 @@ R, RHO = 0.03, -0.7
 # Box-Muller (2 T + 4 FLOP per normal).  Philox integer work is not counted.
 FLOP_PER_PATH_STEP = 17 + 2 * 6
 FP32_VECTOR_PEAK_TFLOPS = 157.3      # datasheet, packed v_pk_fma_f32
+# Without packed FP32 (compiled out on these boxes): one v_fma_f32 per lane
+# and cycle, half the packed figure
+FP32_NONPACKED_PEAK_TFLOPS = FP32_VECTOR_PEAK_TFLOPS / 2
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 # Device sources of the Heston path kernel; with the device compile flags
 # (the COMMON list of build.py) they fix its code object.  The committed
@@ -363,7 +366,8 @@ def _libor_config(np, torch, mc, eng, precisions, steps):
     if "fp32" in res:
         achieved = LIBOR_FLOP_PER_PATH * lp / (res["fp32"][0] * 1e-3) / 1e12
         out.update({"checkpoint_every_fp32": 8, "ms_fp32_no_checkpoints": nock,
-                    "achieved_tflops_fp32": achieved, "frac_fp32_peak": achieved / FP32_VECTOR_PEAK_TFLOPS})
+                    "achieved_tflops_fp32": achieved, "frac_fp32_peak": achieved / FP32_VECTOR_PEAK_TFLOPS,
+                    "frac_fp32_nonpacked_peak": achieved / FP32_NONPACKED_PEAK_TFLOPS})
         # The same algorithm as the FP64 kernel (trajectory stored in HBM):
         # the FP32 kernel stores it above lambda = 0.3 instead of rebuilding
         # rows, and lambda does not change its instruction count.
@@ -591,6 +595,7 @@ def main():
                    "steps": N_STEPS, "rng": "philox4x32-10+box-muller", "parallelism": "single-gpu"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "traffic": prof["bytes"],
+                     "frac_nonpacked_peak": achieved / FP32_NONPACKED_PEAK_TFLOPS,
                      "kernel": "heston_paths_kernel<float> (+finalize)", "kernel_ms": kern_ms,
                      "flop_per_path_step": FLOP_PER_PATH_STEP,
                      # the FP32 FLOP rate of back-to-back three-operand v_fma_f32, measured now
