@@ -566,7 +566,10 @@ __global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
     // (the barrier in the terminal sum below orders the copy before the dates' reads)
     auto date = [&](int i) -> DateConst { return dates_in_lds ? dl[i] : a.dates[ro.date0 + i]; };
     const Real K = (Real)m.K, invK = (Real)ro.invK, S0 = (Real)m.S0, sg = (Real)(m.sigma * O::kE);
-    Real W[NG][4], C[NG][4];
+    // per path: bridge state W, discounted cash flow C, and the spot S of the
+    // date awaiting its decision (computed by the bridge step, so the
+    // decision and the regression see the same bits)
+    Real W[NG][4], C[NG][4], SS[NG][4];
     {
       const Real sqT = (Real)ro.sqT, muT = (Real)(ro.muT * O::kE), discT = (Real)ro.discT;
       double eu[1] = {0.0};
@@ -596,12 +599,12 @@ __global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
       if (pass >= 1 && beta_lds[4] != 0.0) {  // exercise decisions of date d (uniform; per path by selects)
         const Real b0 = (Real)beta_lds[0], b1 = (Real)beta_lds[1], b2 = (Real)beta_lds[2], b3 = (Real)beta_lds[3];
         const DateConst cd = date(d);
-        const Real mu_d = (Real)(cd.mu * O::kE), isw_d = (Real)cd.isw, disc_d = (Real)cd.disc;
+        const Real isw_d = (Real)cd.isw, disc_d = (Real)cd.disc;
 #pragma unroll
         for (int h = 0; h < NG; ++h) {
 #pragma unroll
           for (int g = 0; g < 4; ++g) {
-            const Real Sd = S0 * O::ex(mu_d + sg * W[h][g]);
+            const Real Sd = SS[h][g];
             const Real x = Real(1) + (Sd * invK - Real(1)) * isw_d;
             const Real cont = b0 + x * (b1 + x * (b2 + x * b3));
             const Real ex = (K - Sd) * disc_d;
@@ -653,6 +656,7 @@ __global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
             const Real Wk = ba * W[h][g] + bb * z[g];
             W[h][g] = live ? Wk : Real(0);
             const Real Sk = S0 * O::ex(mu_k + sg * Wk);
+            SS[h][g] = Sk;
             const bool itm = live && K - Sk > Real(0);
             const Real x = itm ? Real(1) + (Sk * invK - Real(1)) * isw_k : Real(0);
             accum_moments(sm, itm, x, C[h][g]);
