@@ -550,7 +550,28 @@ __global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
   // device memory per date would put its latency on every date's critical path
   constexpr int kDatesLds = 1024;
   __shared__ DateConst dl[kDatesLds];
+  // the next bridge step's normals of each of this thread's path groups
+  // (thread-private slots: no barrier orders them), generated while the
+  // part waits for its partners' records, off the date's critical path
+  typedef typename Quad<Real>::V V4;
+  __shared__ V4 znext[NG][kThreads];
   const int lane = threadIdx.x & 63;
+  auto gen_normals = [&](int idx, int first, int o) {  // Philox step index idx of every group
+#pragma unroll
+    for (int h = 0; h < NG; ++h) {
+      int l0 = 4 * (h * kThreads + (int)threadIdx.x);
+      __asm__ volatile("" : "+v"(l0));
+      Real z[4];
+      normals4<Real>(U4{(uint32_t)((first + l0) >> 2), (uint32_t)idx, (uint32_t)o, a.w}, a.k0, a.k1, z[0], z[1], z[2],
+                     z[3]);
+      V4 v;
+      v[0] = z[0];
+      v[1] = z[1];
+      v[2] = z[2];
+      v[3] = z[3];
+      znext[h][threadIdx.x] = v;
+    }
+  };
   const int t_end = a.slot_begin[blockIdx.x + 1];
   for (int ti = a.slot_begin[blockIdx.x]; ti < t_end; ++ti) {
     const int2 task = a.tasks[ti];
@@ -592,6 +613,7 @@ __global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
       const double t = res_block_sum<kThreads, double, 1>(eu, lds);
       if (threadIdx.x == 0) a.eur[rec] = t;
       __syncthreads();  // lds is reused by the first pass
+      if (m.steps >= 2) gen_normals(m.steps - 2, first, o);  // the first bridge step's (N -> N - 1)
     }
     for (int pass = 0; pass < m.steps; ++pass) {
       const int d = m.steps - pass;  // decision date
@@ -643,13 +665,11 @@ __global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
 #pragma unroll
         for (int h = 0; h < NG; ++h) {
           // l0 is made opaque to the optimiser: otherwise it hoists every
-          // group's path index and first Philox product out of the date loop
-          // and keeps them live (spilled) beside the paths' state
+          // group's path index out of the date loop and keeps them live
           int l0 = 4 * (h * kThreads + (int)threadIdx.x);
           __asm__ volatile("" : "+v"(l0));
-          Real z[4];
-          normals4<Real>(U4{(uint32_t)((first + l0) >> 2), (uint32_t)(k - 1), (uint32_t)o, a.w}, a.k0, a.k1, z[0],
-                         z[1], z[2], z[3]);
+          const V4 zv = znext[h][threadIdx.x];  // Philox step index k - 1, generated last date
+          const Real z[4] = {zv[0], zv[1], zv[2], zv[3]};
 #pragma unroll
           for (int g = 0; g < 4; ++g) {
             const bool live = l0 + g < np;
@@ -664,7 +684,8 @@ __global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
           __builtin_amdgcn_sched_barrier(0);
         }
       }
-      // publish this part's moments, then gather every part's
+      // publish this part's moments, generate the next bridge step's normals,
+      // then gather every part's moments
       constexpr int NW = RecWords<Real>::kN;
       const unsigned long long tag = (unsigned long long)(pass + 1) << 32;
       {
@@ -682,6 +703,7 @@ __global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
           }
         }
       }
+      if (k >= 2) gen_normals(k - 2, first, o);  // the bridge k -> k - 1 of the next date
       if (threadIdx.x < 64) {
         // lanes l < parts poll and stage part l's record (rounds of 64 parts);
         // lanes q < kNM then add moment q over the staged parts in part order
