@@ -189,6 +189,18 @@ def test_resident_fp64_matches_oracle(engine):
     assert np.allclose(eu, oeu, rtol=1e-12, atol=0)
 
 
+def test_resident_long_option_reads_dates_from_memory(engine):
+    # more dates than the kernel stages in LDS (1024): that option's date
+    # constants are read from device memory; same results as the oracle
+    P = dict(S0=[100.0, 90.0], K=[100.0] * 2, sigma=[.25, .3], r=[.03] * 2, T=[2.0, 1.0], steps=[1100, 40],
+             paths=[3000, 5000])
+    got, (p, se, eu) = _plan_prices(engine, P, mc.mode(mc.FP64, seed=31), mc.LSMC_RESIDENT)
+    assert got == mc.LSMC_RESIDENT
+    op, ose, oeu = orc.lsmc(P["S0"], P["K"], P["sigma"], P["r"], P["T"], P["steps"], P["paths"], seed=31)
+    assert np.allclose(p, op, rtol=1e-12, atol=0)
+    assert np.allclose(eu, oeu, rtol=1e-12, atol=0)
+
+
 def test_resident_and_streamed_sweeps_agree_256_puts(engine):
     # the 256-put workload: both sweeps price the same paths; FP64 differs only
     # by the order the moment sums are combined in, FP32 by the rounding of
