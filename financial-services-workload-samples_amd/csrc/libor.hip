@@ -59,24 +59,36 @@ __device__ __forceinline__ Real normal_at(const LiborArgs& a, int p, int n, Real
 }
 
 // portfolio_b (liborSwaptionGreeks.cpp:120-211) on the final rates: returns
-// the discounted portfolio value and sets Lb = dV/dL(40, .).  The sample
-// keeps the bond prices B[m] and the adjoints B_b[m], S_b[m] in arrays of
-// 40; here the adjoints are running scalars (S_b and B_b of date m only feed
-// date m - 1), the swaption payoffs' contributions to them are five pairs of
-// scalars (one per maturity), and the reverse sweep recomputes B[m] forward
-// from a checkpoint every 8 dates -- the same divisions in the same order as
-// the forward sweep, so every value equals the array formulation's bit for
-// bit while the section needs ~20 registers besides the rates and adjoints.
+// the discounted portfolio value and hands each dV/dL(40, n) to put(n, lb).
+// The sample keeps the bond prices B[m] and the adjoints B_b[m], S_b[m] in
+// arrays of 40; here the adjoints are running scalars (S_b and B_b of date m
+// only feed date m - 1), the swaption payoffs' contributions to them are five
+// pairs of scalars (one per maturity), and the reverse sweep recomputes B[m]
+// forward from a checkpoint every 8 dates -- the same divisions in the same
+// order as the forward sweep, so every value equals the array formulation's
+// bit for bit while the section needs ~20 registers besides the rates and
+// adjoints.  The discount b to date 40 is taken first (the same divisions in
+// the same order as the sample's, :200-206), so each adjoint leaves in its
+// final, discounted form the moment it is made: no array of unscaled
+// adjoints waits for b.  The phases are kept apart by scheduling barriers, so
+// the compiler cannot hoist the next phase's loads into this one's registers.
 // lo(n) returns rate n < 40 (registers or LDS, as the caller keeps them).
 constexpr int kSeg = 8;
 constexpr int kMats = 5;  // distinct maturities of params.h:38
 __host__ __device__ constexpr int mat_slot(int m) {  // m = maturity - 1 -> its slot, or -1
   return m == 3 ? 0 : m == 7 ? 1 : m == 19 ? 2 : m == 27 ? 3 : m == 39 ? 4 : -1;
 }
-template <typename Real, bool kInplace, typename Lo>
-__device__ __forceinline__ Real portfolio_adjoint(const Real (&L)[NN], Lo lo, Real (&Lb)[NN]) {
+template <typename Real, bool kInplace, typename Lo, typename Put>
+__device__ __forceinline__ Real portfolio_adjoint(const Real (&L)[NN], Lo lo, Put put) {
   using O = MathOps<Real>;
   const Real delta = Real(0.25);
+  Real disc = 1;  // 1 / prod_{n < 40} (1 + delta L_n)
+#pragma unroll
+  for (int n = 0; n < NMAT; ++n) disc = O::div(disc, Real(1) + delta * lo(n));
+  // disc is made here (not sunk to its first use with its 40 operands held
+  // live), and lo(n) is read again at the end rather than kept
+  asm volatile("" : "+v"(disc)::"memory");
+  __builtin_amdgcn_sched_barrier(0);
   Real bck[NMAT / kSeg];  // B at the start of each segment (B[-1] = 1 for the first)
   Real injS[kMats], injB[kMats];
   Real b = 1, s = 0, val = 0;
@@ -103,14 +115,22 @@ __device__ __forceinline__ Real portfolio_adjoint(const Real (&L)[NN], Lo lo, Re
       injB[mat_slot(m)] = bb;
     }
   }
+  __builtin_amdgcn_sched_barrier(0);
   Real Sb = 0, Bb = 0;  // S_b[m], B_b[m] of the date being processed
 #pragma unroll
   for (int g = NMAT / kSeg - 1; g >= 0; --g) {
     Real B[kSeg];  // B[g kSeg .. g kSeg + kSeg - 1], recomputed as in the forward sweep
+    Real Lq[kSeg];  // opaque copies of the segment's rates: the forward sweep's 40
+                    // reciprocals are recomputed here, not kept live for reuse
+#pragma unroll
+    for (int q = 0; q < kSeg; ++q) {
+      Lq[q] = L[g * kSeg + q + NMAT];
+      asm volatile("" : "+v"(Lq[q]));
+    }
     Real bb0 = bck[g];
 #pragma unroll
     for (int q = 0; q < kSeg; ++q) {
-      bb0 = O::div(bb0, Real(1) + delta * L[g * kSeg + q + NMAT]);
+      bb0 = O::div(bb0, Real(1) + delta * Lq[q]);
       B[q] = bb0;
     }
 #pragma unroll
@@ -121,20 +141,17 @@ __device__ __forceinline__ Real portfolio_adjoint(const Real (&L)[NN], Lo lo, Re
         Bb += injB[mat_slot(m)];
       }
       Bb += delta * Sb;
-      const Real dn = Real(1) + delta * L[n];
-      Lb[n] = O::div(-Bb * B[q] * delta, dn);
-      if (m > 0) Bb = O::div(Bb, kInplace ? Real(1) + delta * Lb[n] : dn);
+      const Real dn = Real(1) + delta * Lq[q];
+      const Real lb = O::div(-Bb * B[q] * delta, dn);
+      put(n, disc * lb);
+      if (m > 0) Bb = O::div(Bb, kInplace ? Real(1) + delta * lb : dn);
     }
   }
-  b = 1;
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_sched_barrier(0);
+  val = disc * val;
 #pragma unroll
-  for (int n = 0; n < NMAT; ++n) b = O::div(b, Real(1) + delta * lo(n));
-  val = b * val;
-#pragma unroll
-  for (int n = 0; n < NMAT; ++n) Lb[n] = O::div(-val * delta, Real(1) + delta * lo(n));
-#pragma unroll
-  for (int n = NMAT; n < NN; ++n) Lb[n] = b * Lb[n];
-
+  for (int n = 0; n < NMAT; ++n) put(n, O::div(-val * delta, Real(1) + delta * lo(n)));
   return val;
 }
 
@@ -175,7 +192,8 @@ __global__ __launch_bounds__(kBlock) void libor_path_kernel(LiborArgs a) {
 
   // ---- portfolio_b ---------------------------------------------------------------
   Real Lb[NN];
-  const Real val = portfolio_adjoint<Real, kInplace>(L, [&](int n) { return L[n]; }, Lb);
+  const Real val = portfolio_adjoint<Real, kInplace>(
+      L, [&](int n) { return L[n]; }, [&](int n, Real lb) { Lb[n] = lb; });
 
   // ---- adjoint: path_calc_b2 (L holds row n+1, Lb the adjoints) ---------------------
   for (int n = NMAT - 1; n >= 0; --n) {
@@ -310,10 +328,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   asm volatile("" ::: "memory");
   float M[NN];
   const float L0r = L[0];
+  auto lo = [&](int n) { return n == 0 ? L0r : Llo[n][threadIdx.x]; };
   const float val = portfolio_adjoint<float, kInplace>(
-      L, [&](int n) { return n == 0 ? L0r : Llo[n][threadIdx.x]; }, M);
-#pragma unroll
-  for (int i = 1; i < NN; ++i) M[i] *= i < NMAT ? Llo[i][threadIdx.x] : L[i];
+      L, lo, [&](int n, float lb) { M[n] = n == 0 ? lb : lb * (n < NMAT ? lo(n) : L[n]); });
 
   // ---- adjoint: path_calc_b2, rows rebuilt in registers ------------------------
   // v1(i) lambda faci^2 = con1^2 (S - P) / (1 + delta L(n,i))^2, with P the
