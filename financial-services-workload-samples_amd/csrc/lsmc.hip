@@ -546,23 +546,33 @@ __global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
   __shared__ double sums[kNS];
   __shared__ double beta_lds[5];
   __shared__ int stop_lds;
-  // the option's date constants, copied once per task: a scalar load from
-  // device memory per date would put its latency on every date's critical path
-  constexpr int kDatesLds = 1024;
+  // a window of the option's date constants, dates [wbase, wbase +
+  // kDatesLds): a load from device memory per date would put its latency on
+  // every date's critical path.  Options with more dates slide the window
+  // down when the sweep leaves it.
+  constexpr int kDatesLds = 512;
   __shared__ DateConst dl[kDatesLds];
-  // the next bridge step's normals of each of this thread's path groups
-  // (thread-private slots: no barrier orders them), generated while the
-  // part waits for its partners' records, off the date's critical path
+  // thread-private slots (no barrier orders them): the next bridge step's
+  // normals of each of this thread's path groups, generated while the part
+  // waits for its partners' records, off the date's critical path; and the
+  // spots of the date awaiting its decision, made by the bridge step (so the
+  // decision and the regression see the same bits) and kept here rather than
+  // in registers beside the bridge states and cash flows
   typedef typename Quad<Real>::V V4;
   __shared__ V4 znext[NG][kThreads];
+  __shared__ V4 spot[NG][kThreads];
   const int lane = threadIdx.x & 63;
-  auto gen_normals = [&](int idx, int first, int o) {  // Philox step index idx of every group
+  // Group h of this thread holds paths 4 (h kThreads + threadIdx.x) + g of
+  // the part: its Philox counter word is ctr0 + h kThreads.  The base is
+  // passed through an empty asm at each use, so the optimiser recomputes the
+  // groups' counters there instead of keeping NG of them live in the loops.
+  auto gen_normals = [&](int idx, int ctr0, int o) __attribute__((always_inline)) {  // Philox step index idx
 #pragma unroll
     for (int h = 0; h < NG; ++h) {
-      int l0 = 4 * (h * kThreads + (int)threadIdx.x);
-      __asm__ volatile("" : "+v"(l0));
+      int c = ctr0;
+      __asm__ volatile("" : "+v"(c));
       Real z[4];
-      normals4<Real>(U4{(uint32_t)((first + l0) >> 2), (uint32_t)idx, (uint32_t)o, a.w}, a.k0, a.k1, z[0], z[1], z[2],
+      normals4<Real>(U4{(uint32_t)(c + h * kThreads), (uint32_t)idx, (uint32_t)o, a.w}, a.k0, a.k1, z[0], z[1], z[2],
                      z[3]);
       V4 v;
       v[0] = z[0];
@@ -579,30 +589,33 @@ __global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
     const OptMeta m = a.meta[o];
     const int first = part * a.part_paths;
     const int np = min(a.part_paths, m.paths - first);
+    // path 4 (h kThreads + threadIdx.x) + g is live iff 4 h kThreads + g < rem
+    const int rem = np - 4 * (int)threadIdx.x;
+    const int ctr0 = (first >> 2) + (int)threadIdx.x;  // part_paths is a multiple of 4
     const int rec = m.first_block + part;
     const ResOpt ro = a.ropt[o];
-    const bool dates_in_lds = m.steps < kDatesLds;  // uniform
-    if (dates_in_lds)
-      for (int i = threadIdx.x; i <= m.steps; i += kThreads) dl[i] = a.dates[ro.date0 + i];
+    int wbase = max(0, m.steps + 1 - kDatesLds);  // uniform
+    auto load_window = [&]() {
+      for (int i = threadIdx.x; i < kDatesLds && wbase + i <= m.steps; i += kThreads) dl[i] = a.dates[ro.date0 + wbase + i];
+    };
+    load_window();
     // (the barrier in the terminal sum below orders the copy before the dates' reads)
-    auto date = [&](int i) -> DateConst { return dates_in_lds ? dl[i] : a.dates[ro.date0 + i]; };
+    auto date = [&](int i) -> DateConst { return dl[i - wbase]; };
     const Real K = (Real)m.K, invK = (Real)ro.invK, S0 = (Real)m.S0, sg = (Real)(m.sigma * O::kE);
-    // per path: bridge state W, discounted cash flow C, and the spot S of the
-    // date awaiting its decision (computed by the bridge step, so the
-    // decision and the regression see the same bits)
-    Real W[NG][4], C[NG][4], SS[NG][4];
+    // per path: bridge state W and discounted cash flow C (the spot of the
+    // date awaiting its decision is in spot[][])
+    Real W[NG][4], C[NG][4];
     {
       const Real sqT = (Real)ro.sqT, muT = (Real)(ro.muT * O::kE), discT = (Real)ro.discT;
       double eu[1] = {0.0};
 #pragma unroll
       for (int h = 0; h < NG; ++h) {
-        const int l0 = 4 * (h * kThreads + (int)threadIdx.x);
         Real z[4];  // computed for padding groups too (never used for them): no branch per group
-        normals4<Real>(U4{(uint32_t)((first + l0) >> 2), (uint32_t)(m.steps - 1), (uint32_t)o, a.w}, a.k0, a.k1,
+        normals4<Real>(U4{(uint32_t)(ctr0 + h * kThreads), (uint32_t)(m.steps - 1), (uint32_t)o, a.w}, a.k0, a.k1,
                        z[0], z[1], z[2], z[3]);
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-          const bool live = l0 + g < np;
+          const bool live = 4 * h * kThreads + g < rem;
           W[h][g] = live ? sqT * z[g] : Real(0);
           const Real ST = S0 * O::ex(muT + sg * W[h][g]);
           C[h][g] = live ? discT * (K - ST > Real(0) ? K - ST : Real(0)) : Real(0);
@@ -613,27 +626,32 @@ __global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
       const double t = res_block_sum<kThreads, double, 1>(eu, lds);
       if (threadIdx.x == 0) a.eur[rec] = t;
       __syncthreads();  // lds is reused by the first pass
-      if (m.steps >= 2) gen_normals(m.steps - 2, first, o);  // the first bridge step's (N -> N - 1)
+      if (m.steps >= 2) gen_normals(m.steps - 2, ctr0, o);  // the first bridge step's (N -> N - 1)
     }
     for (int pass = 0; pass < m.steps; ++pass) {
       const int d = m.steps - pass;  // decision date
       const int k = d - 1;           // regression date
+      if (k < wbase) {  // uniform: the window slides down to end at date d
+        // (every read of the previous window ended before the last pass's barrier)
+        wbase = max(0, d + 1 - kDatesLds);
+        load_window();
+        __syncthreads();
+      }
       if (pass >= 1 && beta_lds[4] != 0.0) {  // exercise decisions of date d (uniform; per path by selects)
         const Real b0 = (Real)beta_lds[0], b1 = (Real)beta_lds[1], b2 = (Real)beta_lds[2], b3 = (Real)beta_lds[3];
         const DateConst cd = date(d);
         const Real isw_d = (Real)cd.isw, disc_d = (Real)cd.disc;
 #pragma unroll
         for (int h = 0; h < NG; ++h) {
+          const V4 sv = spot[h][threadIdx.x];
 #pragma unroll
           for (int g = 0; g < 4; ++g) {
-            const Real Sd = SS[h][g];
+            const Real Sd = sv[g];
             const Real x = Real(1) + (Sd * invK - Real(1)) * isw_d;
             const Real cont = b0 + x * (b1 + x * (b2 + x * b3));
             const Real ex = (K - Sd) * disc_d;
             // padding paths never exercise
-            int l = 4 * (h * kThreads + (int)threadIdx.x) + g;
-            __asm__ volatile("" : "+v"(l));
-            const bool live = l < np;
+            const bool live = 4 * h * kThreads + g < rem;
             C[h][g] = live && K - Sd > Real(0) && ex > cont ? ex : C[h][g];
           }
           __builtin_amdgcn_sched_barrier(0);
@@ -664,23 +682,21 @@ __global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
         const Real ba = (Real)cd.ba, bb = (Real)cd.bb;
 #pragma unroll
         for (int h = 0; h < NG; ++h) {
-          // l0 is made opaque to the optimiser: otherwise it hoists every
-          // group's path index out of the date loop and keeps them live
-          int l0 = 4 * (h * kThreads + (int)threadIdx.x);
-          __asm__ volatile("" : "+v"(l0));
           const V4 zv = znext[h][threadIdx.x];  // Philox step index k - 1, generated last date
           const Real z[4] = {zv[0], zv[1], zv[2], zv[3]};
+          V4 sk;
 #pragma unroll
           for (int g = 0; g < 4; ++g) {
-            const bool live = l0 + g < np;
+            const bool live = 4 * h * kThreads + g < rem;
             const Real Wk = ba * W[h][g] + bb * z[g];
             W[h][g] = live ? Wk : Real(0);
             const Real Sk = S0 * O::ex(mu_k + sg * Wk);
-            SS[h][g] = Sk;
+            sk[g] = Sk;
             const bool itm = live && K - Sk > Real(0);
             const Real x = itm ? Real(1) + (Sk * invK - Real(1)) * isw_k : Real(0);
             accum_moments(sm, itm, x, C[h][g]);
           }
+          spot[h][threadIdx.x] = sk;
           __builtin_amdgcn_sched_barrier(0);
         }
       }
@@ -703,7 +719,7 @@ __global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
           }
         }
       }
-      if (k >= 2) gen_normals(k - 2, first, o);  // the bridge k -> k - 1 of the next date
+      if (k >= 2) gen_normals(k - 2, ctr0, o);  // the bridge k -> k - 1 of the next date
       if (threadIdx.x < 64) {
         // lanes l < parts poll and stage part l's record (rounds of 64 parts);
         // lanes q < kNM then add moment q over the staged parts in part order
