@@ -192,14 +192,24 @@ __device__ __forceinline__ void combine_partials(const double* partials, int fir
 }
 
 // Solves the 4x4 normal equations from the option's summed moments s.
+__device__ __forceinline__ double rsqrt_f64(double v) {
+  double y = __builtin_amdgcn_rsq(v);
+  const double hv = 0.5 * v;
+#pragma unroll
+  for (int it = 0; it < 2; ++it) y = fma(y, fma(-hv * y, y, 0.5), y);  // y += y (1/2 - v y^2 / 2)
+  return y;
+}
 __device__ void lsmc_solve(const double (&s)[kNS], double* beta_out) {
   // Cholesky G = L L^T of G_ij = m_{i+j}; a pivot that is not clearly
   // positive (fewer than 4 ITM paths, or a singular basis) disables exercise
   // at this date.  The diagonal enters only through divisions, so its four
   // reciprocals are formed once and the remaining steps multiply: four
   // square roots and four divisions on the serial path instead of four and
-  // fourteen (the solve sits on the critical path of every date).  Fully
-  // unrolled so L stays in registers.
+  // fourteen (the solve sits on the critical path of every date).  Each
+  // reciprocal square root is the hardware estimate refined by two Newton
+  // steps (quadratic convergence from its ~2^-23 error to FP64 rounding),
+  // not an IEEE square root followed by an IEEE division.  Fully unrolled so
+  // L stays in registers.
   double L[4][4] = {{0}}, inv[4];
   bool ok = s[0] >= 4.0;
 #pragma unroll
@@ -211,7 +221,7 @@ __device__ void lsmc_solve(const double (&s)[kNS], double* beta_out) {
       for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
       if (i == j) {
         ok = ok && v > 1e-13 * s[2 * i];
-        inv[i] = ok ? 1.0 / sqrt(v) : 1.0;
+        inv[i] = ok ? rsqrt_f64(v) : 1.0;
       } else {
         L[i][j] = v * inv[j];
       }
@@ -541,6 +551,7 @@ template <typename Real, int PPT, int kThreads>
 __global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
   using O = MathOps<Real>;
   constexpr int NG = PPT / 4;
+  static_assert(64 + 64 * NG <= kThreads, "waves 1 .. NG take wave 0's normals");
   __shared__ double lds[kNS * (kThreads / 64)];
   __shared__ double stage[64 * kNM];  // records of up to 64 parts, [part][moment]
   __shared__ double sums[kNS];
@@ -719,7 +730,30 @@ __global__ __launch_bounds__(kThreads) void lsmc_resident_kernel(ResArgs a) {
           }
         }
       }
-      if (k >= 2) gen_normals(k - 2, ctr0, o);  // the bridge k -> k - 1 of the next date
+      // the bridge k -> k - 1 of the next date.  FP32: wave 0 goes straight
+      // to the poll and the solve below, the other waves generate its normals
+      // as well as their own (waves 1 .. NG one (thread, group) slot of wave 0
+      // per thread), so the generation stays off the date's critical path.
+      // FP64 generation (software log and sincos) outlasts the poll and the
+      // solve: there the extra share would lengthen the date, and every wave
+      // generates its own.
+      constexpr bool kOffload = sizeof(Real) == 4;
+      if (k >= 2 && (!kOffload || threadIdx.x >= 64)) {  // uniform per wave
+        gen_normals(k - 2, ctr0, o);
+        const int j = (int)threadIdx.x - 64;
+        if (kOffload && j < 64 * NG) {
+          const int h = j >> 6, t = j & 63;
+          Real z[4];
+          normals4<Real>(U4{(uint32_t)((first >> 2) + t + h * kThreads), (uint32_t)(k - 2), (uint32_t)o, a.w}, a.k0,
+                         a.k1, z[0], z[1], z[2], z[3]);
+          V4 v;
+          v[0] = z[0];
+          v[1] = z[1];
+          v[2] = z[2];
+          v[3] = z[3];
+          znext[h][t] = v;
+        }
+      }
       if (threadIdx.x < 64) {
         // lanes l < parts poll and stage part l's record (rounds of 64 parts);
         // lanes q < kNM then add moment q over the staged parts in part order
