@@ -441,6 +441,7 @@ def _batch_config(np, torch, mc, eng, precisions, steps):
         tlat = b.results(mc._stream())["latency_us"] / 1e3
         b.close()
         lb = mc.Batch(eng, reqs, m, tuning=(tuning[0], 256, tuning[2], tuning[3], tuning[4]))
+        ltuning = lb.autotune(requests_per_launch=256)
         lms = _timed(torch, np, lambda: lb.run(mc._stream()), steps, 1)
         lat = lb.results(mc._stream())["latency_us"] / 1e3
         lb.close()
@@ -454,8 +455,11 @@ def _batch_config(np, torch, mc, eng, precisions, steps):
                     # request completion latency of the throughput configuration above
                     "latency_p50_ms_" + name: float(np.percentile(tlat, 50)),
                     "latency_p99_ms_" + name: float(np.percentile(tlat, 99)),
-                    # the same requests served in micro-batches of 256 (in arrival order)
+                    # the same requests served in micro-batches of 256 (in arrival order),
+                    # tuned for that micro-batch size
                     "microbatch256_ms_" + name: lms,
+                    "microbatch256_tuning_" + name: {"chunk_paths": ltuning[0], "replay_graph": ltuning[2],
+                                                     "workgroups_per_cu": ltuning[3], "work_queue": ltuning[4]},
                     "microbatch256_requests_per_s_" + name: len(reqs) / (lms * 1e-3),
                     "microbatch256_latency_p50_ms_" + name: float(np.percentile(lat, 50)),
                     "microbatch256_latency_p99_ms_" + name: float(np.percentile(lat, 99))})

@@ -19,6 +19,7 @@
 // A request's normals and per-path arithmetic are those of the standalone
 // entry point with the request index as the option index.
 #include <algorithm>
+#include <initializer_list>
 #include <map>
 #include <vector>
 
@@ -677,18 +678,18 @@ static int batch_time(fsmc_batch* b, fsmc_batch_tuning t, float* best) {
 
 extern "C" {
 
-int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen) {
-  FSMC_REQUIRE(b != nullptr, "batch is NULL");
+// The search of both autotune entry points over the micro-batch sizes pers
+// (0 = the whole batch in one launch).
+static int batch_autotune(fsmc_batch* b, std::initializer_list<int> pers, fsmc_batch_tuning* chosen) {
   BatchImpl& bi = b->impl;
   fsmc_ctx* ctx = b->ctx;
   const int chunks[] = {1024, 2048, 4096, 8192, 16384};
-  const int pers[] = {0, 1024, 256};
   fsmc_batch_tuning best = bi.tuning;
   float best_ms = 1e30f;
   int st = FSMC_OK;
   for (int c : chunks) {
     for (int per : pers) {
-      if (per >= bi.n) continue;
+      if (pers.size() > 1 && per != 0 && per >= bi.n) continue;  // the same as one launch
       float ms = 0.0f;
       st = batch_time(b, fsmc_batch_tuning{c, per, 0, 0, 0}, &ms);
       if (st != FSMC_OK) break;
@@ -729,6 +730,17 @@ int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen) {
   }
   if (chosen) *chosen = best;
   return FSMC_OK;
+}
+
+int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen) {
+  FSMC_REQUIRE(b != nullptr, "batch is NULL");
+  return batch_autotune(b, {0, 1024, 256}, chosen);
+}
+
+int fsmc_batch_autotune_microbatch(fsmc_batch* b, int requests_per_launch, fsmc_batch_tuning* chosen) {
+  FSMC_REQUIRE(b != nullptr, "batch is NULL");
+  FSMC_REQUIRE(requests_per_launch > 0, "requests_per_launch must be positive");
+  return batch_autotune(b, {requests_per_launch}, chosen);
 }
 
 int fsmc_batch_tuning_get(const fsmc_batch* b, fsmc_batch_tuning* out) {

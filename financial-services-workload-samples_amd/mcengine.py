@@ -178,6 +178,7 @@ def lib():
         L.fsmc_batch_run.argtypes = [_VP, _VP]
         L.fsmc_batch_results.argtypes = [_VP, _VP, ctypes.POINTER(Result)]
         L.fsmc_batch_autotune.argtypes = [_VP, ctypes.POINTER(BatchTuning)]
+        L.fsmc_batch_autotune_microbatch.argtypes = [_VP, ctypes.c_int, ctypes.POINTER(BatchTuning)]
         L.fsmc_batch_tuning_get.argtypes = [_VP, ctypes.POINTER(BatchTuning)]
         L.fsmc_batch_destroy.argtypes = [_VP]
         _lib = L
@@ -477,9 +478,15 @@ class Batch:
         _check(lib().fsmc_batch_results(self._b, stream, out), "fsmc_batch_results")
         return {f: np.array([getattr(r, f) for r in out]) for f in self.FIELDS}
 
-    def autotune(self):
+    def autotune(self, requests_per_launch=None):
+        """Tunes the batch for throughput, or with requests_per_launch the
+        fastest configuration at that fixed micro-batch size."""
         t = BatchTuning()
-        _check(lib().fsmc_batch_autotune(self._b, ctypes.byref(t)), "fsmc_batch_autotune")
+        if requests_per_launch is None:
+            _check(lib().fsmc_batch_autotune(self._b, ctypes.byref(t)), "fsmc_batch_autotune")
+        else:
+            _check(lib().fsmc_batch_autotune_microbatch(self._b, int(requests_per_launch), ctypes.byref(t)),
+                   "fsmc_batch_autotune_microbatch")
         return t.chunk_paths, t.requests_per_launch, t.replay_graph, t.workgroups_per_cu, t.work_queue
 
     def tuning(self):

@@ -449,10 +449,16 @@ int fsmc_batch_run(fsmc_batch* b, void* stream);
  * run was enqueued on), copies the results and synchronises its stream. */
 int fsmc_batch_results(fsmc_batch* b, void* stream, fsmc_result* out);
 /* Times the batch over chunk sizes x micro-batch sizes (grouped grids), then
- * the winner as a persistent grid of 1 to 4 workgroups per CU with static
- * lists and with the work queue, then the best with graph replay; keeps the
- * fastest and caches it for batches of the same shape in this process. */
+ * the winner as a persistent grid of 1, 2, 3, 4, 6 or 8 workgroups per CU
+ * with static lists and with the work queue, then the best with graph
+ * replay; keeps the fastest and caches it for batches of the same shape in
+ * this process. */
 int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen);
+/* The same search with the micro-batch size fixed at requests_per_launch
+ * (> 0): for latency-bound serving, where each launch's requests complete
+ * together and the micro-batch size is set by the caller's latency target.
+ * No counterpart in the samples (their drivers price one option per call). */
+int fsmc_batch_autotune_microbatch(fsmc_batch* b, int requests_per_launch, fsmc_batch_tuning* chosen);
 int fsmc_batch_tuning_get(const fsmc_batch* b, fsmc_batch_tuning* out);
 int fsmc_batch_destroy(fsmc_batch* b);
 

@@ -107,8 +107,15 @@ def test_mixed_batch_reruns_and_tuning(engine):
     assert b.tuning() == chosen
     b.run()
     tuned = b.results()
-    b.close()
     assert np.allclose(tuned["price"], first["price"], rtol=1e-9, atol=1e-12)
+    # tuned at a fixed micro-batch size: the size is kept, requests still complete in order
+    micro = b.autotune(requests_per_launch=32)
+    assert micro[1] == 32 and b.tuning() == micro
+    b.run()
+    mt = b.results()
+    b.close()
+    assert np.allclose(mt["price"], first["price"], rtol=1e-9, atol=1e-12)
+    assert np.all(np.diff(mt["latency_us"][::32]) >= 0)
 
 
 def test_results_wait_for_a_run_on_another_stream(engine):
@@ -190,6 +197,10 @@ def test_batch_errors(engine):
         mc.Batch(engine, [mc.gbm_request(100, 10, 10, 1, .02, .3)], mc.mode(), tuning=(1024, 0, 0, 17))
     with pytest.raises(mc.FsmcError):  # the work queue runs on the persistent grid only
         mc.Batch(engine, [mc.gbm_request(100, 10, 10, 1, .02, .3)], mc.mode(), tuning=(1024, 0, 0, 0, 1))
+    b = mc.Batch(engine, [mc.gbm_request(100, 10, 10, 1, .02, .3)], mc.mode())
+    with pytest.raises(mc.FsmcError):  # a micro-batch holds at least one request
+        b.autotune(requests_per_launch=0)
+    b.close()
 
 
 def test_serving_batch_4096_requests(engine):
