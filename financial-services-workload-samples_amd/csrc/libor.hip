@@ -333,16 +333,29 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
       L, lo, [&](int n, float lb) { M[n] = n == 0 ? lb : lb * (n < NMAT ? lo(n) : L[n]); });
 
   // ---- adjoint: path_calc_b2, rows rebuilt in registers ------------------------
-  // v1(i) lambda faci^2 = con1^2 (S - P) / (1 + delta L(n,i))^2, with P the
-  // prefix of the row total below i.
+  // v1(i) lambda faci^2 = con1^2 Q / (1 + delta L(n,i))^2, with Q the row total
+  // of M from i up.  The sweep keeps its rates scaled by sig = con1^2 (a floor
+  // keeps sig a normal number as lambda -> 0, where every con1^2 term
+  // vanishes): with x' = sig x the series variable is s = x' r0 and the
+  // adjoint's factor con1^2 x r^2 is x' r^2, one multiply each fewer, and the
+  // running sum vs' = sig vs enters the exponent as kk / sig.
   const float con1sq = con1 * con1;
+  const float sig = con1sq > 1e-30f ? con1sq : 1e-30f;
+  const float kks = kk / sig, dls = delta / sig;
+#pragma unroll
+  for (int i = 1; i < NN; ++i) {
+    if (i < NMAT)
+      Llo[i][threadIdx.x] *= sig;
+    else
+      L[i] *= sig;
+  }
   for (int n = NMAT - 1; n >= 0; --n) {
     if (kCkpt && ckpt_row(n + 1)) {  // uniform: the stored row n + 1 replaces the rebuilt one
       const int cb = ckpt_base(n + 1) - (n + 1);
 #pragma unroll
       for (int i = 1; i < NN; ++i) {
         if (i > n) {
-          const float y = ckpt[(size_t)(cb + i) * P];
+          const float y = sig * ckpt[(size_t)(cb + i) * P];
           if (i < NMAT)
             Llo[i][threadIdx.x] = y;
           else
@@ -362,28 +375,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
       z = j == 0 ? zr[0] : j == 1 ? zr[1] : j == 2 ? zr[2] : zr[3];
     }
     const float c2 = lam * (sqrt_delta * z - 0.5f * con1) * O::kE;
-    float S = 0.f;
+    float Q = 0.f;
 #pragma unroll
     for (int i = 1; i < NN; ++i)
-      if (i > n) S += M[i];
-    float vs = 0.f, P0 = 0.f;
+      if (i > n) Q += M[i];
+    float vs = 0.f;
 #pragma unroll
     for (int i = 1; i < NN; ++i) {
       if (i > n) {
         const float y = i < NMAT ? Llo[i][threadIdx.x] : L[i];
-        const float w = y * O::ex(-__builtin_fmaf(kk, vs, c2));
-        const float r0 = __builtin_amdgcn_rcpf(__builtin_fmaf(delta, w, 1.f));
+        const float w = y * O::ex(-__builtin_fmaf(kks, vs, c2));
+        const float r0 = __builtin_amdgcn_rcpf(__builtin_fmaf(dls, w, 1.f));
         // x = w (1 + e), e = -s + c2 s^2 + c3 s^3, s = con1^2 w / (1 + delta w)
-        const float s = con1sq * (w * r0);
+        const float s = w * r0;
         const float c3 = __builtin_fmaf(r0, __builtin_fmaf(-2.f, r0, -0.5f), -1.f / 6.f);
         const float h = __builtin_fmaf(s, __builtin_fmaf(s, c3, r0 + 0.5f), -1.f);
         const float x = __builtin_fmaf(w, s * h, w);
-        const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(delta, x, 1.f));
+        const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(dls, x, 1.f));
         const float xr = x * r;
         vs += xr;
-        const float d = S - P0;
-        P0 += M[i];
-        M[i] = __builtin_fmaf(xr * r, d * con1sq, M[i]);
+        const float m = M[i];
+        M[i] = __builtin_fmaf(xr * r, Q, m);
+        Q -= m;
         if (i < NMAT)
           Llo[i][threadIdx.x] = x;
         else
