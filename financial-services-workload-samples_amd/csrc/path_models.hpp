@@ -240,6 +240,10 @@ __device__ __forceinline__ double heston_adjoint_path(const HestonConsts<Real>& 
 // linear in r (dS_k/dr = S_k k dt), so those two need no recursion.  Then
 //   dP/dS0 = e^{-rT} A / S0,  dP/dsqrt(v0) = e^{-rT} (1/N) sum_k dS_k,
 //   dP/dr = e^{-rT} (1/N) sum_k S_k k dt - T P   (A > K; all zero else).
+// The time-weighted sum comes from the running sums A_j = sum_{i<=j} S_i the
+// average already forms: sum_{k=1..N} k S_k = (N + 1) A_N - sum_{j=1..N} A_j,
+// so a step adds its running sum (in FP64) instead of carrying k dt and a
+// product.
 // The path itself is stepped exactly as heston_philox_pair steps it, so the
 // price is bit for bit the forward kernel's.  Three Greeks cost a reciprocal
 // and ~12 instructions per step here, against a segment recompute and a
@@ -247,8 +251,8 @@ __device__ __forceinline__ double heston_adjoint_path(const HestonConsts<Real>& 
 // adjoint's.
 template <typename Real>
 struct HestonTangent {
-  Real dS = 0, dv = 0, t = 0;  // dS_k/dsqrt(v0), dv_k/dsqrt(v0), k dt
-  PathSum dA, St;                // sum_k dS_k, sum_k S_k k dt
+  Real dS = 0, dv = 0;  // dS_k/dsqrt(v0), dv_k/dsqrt(v0)
+  PathSum dA, B;        // sum_k dS_k, sum_j A_j (the running sums of the spots)
 };
 
 // One step of heston_step_polar (FP32, polar normals) or heston_step (FP64,
@@ -257,7 +261,7 @@ struct HestonTangent {
 // terms through tg; the caller adds the pair sums.
 template <typename Real>
 __device__ __forceinline__ void heston_step_tangent(const HestonConsts<Real>& c, Real& S, Real& v, Real r2, Real cs,
-                                                    Real sn, HestonTangent<Real>& tg, Real& dS_out, Real& St_out) {
+                                                    Real sn, HestonTangent<Real>& tg, Real& dS_out) {
 #pragma clang fp contract(off)
   const Real vp = positive_part(v);
   Real q, u;  // q: the step's sqrt(vp) times the radius; u = radius / (2 sqrt(vp))
@@ -276,9 +280,7 @@ __device__ __forceinline__ void heston_step_tangent(const HestonConsts<Real>& c,
   const Real dvdv = vp > Real(0) ? ffma(zz, u, c.one_kdt) : Real(1);
   tg.dS = ffma(tg.dS, growth, Sn * (dedv * tg.dv));
   tg.dv = tg.dv * dvdv;
-  tg.t = tg.t + c.dt_n;
   dS_out = tg.dS;
-  St_out = Sn * tg.t;
   S = Sn;
   v = ffma(q, zz, ffma(c.kdt, c.theta - vp, v));
 }
@@ -303,18 +305,20 @@ __device__ __forceinline__ double heston_tangent_path(const HestonConsts<Real>& 
       normals4<Real>(U4{p, (uint32_t)(k >> 1), c2, w}, k0, k1, ca, sa, cb, sb);
       ra = rb = Real(1);
     }
-    Real d0, s0, d1, s1;
-    heston_step_tangent(c, S, v, ra, ca, sa, tg, d0, s0);
+    Real d0, d1;
+    heston_step_tangent(c, S, v, ra, ca, sa, tg, d0);
+    const double Aprev = avg.value();
     if (second) {
       const Real sa0 = S;
-      heston_step_tangent(c, S, v, rb, cb, sb, tg, d1, s1);
+      heston_step_tangent(c, S, v, rb, cb, sb, tg, d1);
       avg.add(sa0 + S);
       tg.dA.add(d0 + d1);
-      tg.St.add(s0 + s1);
+      // A_{k+1} + A_{k+2} = 2 A_k + 2 S_{k+1} + S_{k+2}
+      tg.B.add(fma(2.0, Aprev, (double)ffma(Real(2), sa0, S)));
     } else {
       avg.add(S);
       tg.dA.add(d0);
-      tg.St.add(s0);
+      tg.B.add(Aprev + (double)S);
     }
   }
   g[0] = g[1] = g[2] = 0.0;
@@ -324,7 +328,8 @@ __device__ __forceinline__ double heston_tangent_path(const HestonConsts<Real>& 
   const double disc = exp(-r * Td);
   g[0] = disc * A / S0d;
   g[1] = disc * tg.dA.value() / (double)N;
-  g[2] = disc * tg.St.value() / (double)N - Td * disc * x;
+  const double St = Td / N * ((double)(N + 1) * avg.value() - tg.B.value());  // sum_k S_k k dt
+  g[2] = disc * St / (double)N - Td * disc * x;
   return x;
 }
 
