@@ -679,8 +679,10 @@ static int batch_time(fsmc_batch* b, fsmc_batch_tuning t, float* best) {
 extern "C" {
 
 // The search of both autotune entry points over the micro-batch sizes pers
-// (0 = the whole batch in one launch).
-static int batch_autotune(fsmc_batch* b, std::initializer_list<int> pers, fsmc_batch_tuning* chosen) {
+// (0 = the whole batch in one launch); cache: remember the winner for
+// batches of this shape (the throughput search only, so that a later
+// fsmc_batch_create with no tuning gets the throughput choice).
+static int batch_autotune(fsmc_batch* b, std::initializer_list<int> pers, bool cache, fsmc_batch_tuning* chosen) {
   BatchImpl& bi = b->impl;
   fsmc_ctx* ctx = b->ctx;
   const int chunks[] = {1024, 2048, 4096, 8192, 16384};
@@ -724,7 +726,7 @@ static int batch_autotune(fsmc_batch* b, std::initializer_list<int> pers, fsmc_b
     std::lock_guard<std::mutex> lock(ctx->mu);
     FSMC_TRY(batch_configure(ctx, bi, best));
   }
-  {
+  if (cache) {
     std::lock_guard<std::mutex> cl(g_cache_mu);
     tuning_cache()[batch_key(bi)] = best;
   }
@@ -734,13 +736,13 @@ static int batch_autotune(fsmc_batch* b, std::initializer_list<int> pers, fsmc_b
 
 int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen) {
   FSMC_REQUIRE(b != nullptr, "batch is NULL");
-  return batch_autotune(b, {0, 1024, 256}, chosen);
+  return batch_autotune(b, {0, 1024, 256}, true, chosen);
 }
 
 int fsmc_batch_autotune_microbatch(fsmc_batch* b, int requests_per_launch, fsmc_batch_tuning* chosen) {
   FSMC_REQUIRE(b != nullptr, "batch is NULL");
   FSMC_REQUIRE(requests_per_launch > 0, "requests_per_launch must be positive");
-  return batch_autotune(b, {requests_per_launch}, chosen);
+  return batch_autotune(b, {requests_per_launch}, false, chosen);
 }
 
 int fsmc_batch_tuning_get(const fsmc_batch* b, fsmc_batch_tuning* out) {

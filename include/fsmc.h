@@ -457,7 +457,9 @@ int fsmc_batch_autotune(fsmc_batch* b, fsmc_batch_tuning* chosen);
 /* The same search with the micro-batch size fixed at requests_per_launch
  * (> 0): for latency-bound serving, where each launch's requests complete
  * together and the micro-batch size is set by the caller's latency target.
- * No counterpart in the samples (their drivers price one option per call). */
+ * Configures this batch only; the per-shape cache keeps the throughput
+ * choice.  No counterpart in the samples (their drivers price one option per
+ * call). */
 int fsmc_batch_autotune_microbatch(fsmc_batch* b, int requests_per_launch, fsmc_batch_tuning* chosen);
 int fsmc_batch_tuning_get(const fsmc_batch* b, fsmc_batch_tuning* out);
 int fsmc_batch_destroy(fsmc_batch* b);

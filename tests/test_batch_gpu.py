@@ -111,6 +111,10 @@ def test_mixed_batch_reruns_and_tuning(engine):
     # tuned at a fixed micro-batch size: the size is kept, requests still complete in order
     micro = b.autotune(requests_per_launch=32)
     assert micro[1] == 32 and b.tuning() == micro
+    # the shape's cached tuning stays the throughput choice
+    b3 = mc.Batch(engine, reqs, m)
+    assert b3.tuning() == chosen
+    b3.close()
     b.run()
     mt = b.results()
     b.close()
