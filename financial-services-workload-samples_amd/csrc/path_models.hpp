@@ -117,6 +117,32 @@ __device__ __forceinline__ void heston_philox_pair(const HestonConsts<Real>& c, 
   }
 }
 
+// One step of heston_step_polar (FP32, polar normals) or heston_step (FP64,
+// normals z1 = cs, z2 = sn, r2 = 1) that also returns the step's derivatives
+// de/dv and dv'/dv at its starting variance (0 and 1 where the variance is
+// clamped), formed as heston_step_tangent forms them: the adjoint's segment
+// recompute keeps these instead of the step's inputs.
+template <typename Real>
+__device__ __forceinline__ void heston_step_derivs(const HestonConsts<Real>& c, Real& S, Real& v, Real r2, Real cs,
+                                                   Real sn, Real& dedv, Real& dvdv) {
+#pragma clang fp contract(off)
+  const Real vp = positive_part(v);
+  Real q, u;  // q: the step's sqrt(vp) times the radius; u = radius / (2 sqrt(vp))
+  if constexpr (sizeof(Real) == 4) {
+    q = fsqrt(vp * r2);
+    u = vp > Real(0) ? Real(0.5) * q * frcp(vp) : Real(0);
+  } else {
+    q = fsqrt(vp);
+    u = vp > Real(0) ? Real(0.5) / q : Real(0);
+  }
+  const Real e = ffma(c.c_sdt * q, cs, ffma(-c.c_hdt, vp, c.c_rdt));
+  const Real zz = ffma(c.xr, cs, c.xs * sn);
+  S = S * fexp2(e);
+  dedv = vp > Real(0) ? ffma(c.sdt_n * u, cs, -c.hdt_n) : Real(0);
+  dvdv = vp > Real(0) ? ffma(zz, u, c.one_kdt) : Real(1);
+  v = ffma(q, zz, ffma(c.kdt, c.theta - vp, v));
+}
+
 // Adjoint (reverse sweep) of one Heston-Asian path.  Pathwise derivatives of
 //   P = e^{-rT} max(A - K, 0),  A = (1/N) sum_{k=1..N} S_k,
 //   S_{k+1} = S_k exp(e_k),  e_k = r dt - vp_k dt / 2 + sqrt(vp_k dt) z1_k,
@@ -139,7 +165,6 @@ __device__ __forceinline__ double heston_adjoint_path(const HestonConsts<Real>& 
                                                       double (&g)[3]) {
   const int nseg = (N + kHestonSeg - 1) / kHestonSeg;
   const double dtd = Td / N;
-  const Real hdt = (Real)(0.5 * dtd), sdt = (Real)sqrt(dtd);
   Real S = (Real)S0d, v = (Real)v0d;
   PathSum avg;
   for (int seg = 0; seg < nseg; ++seg) {
@@ -157,9 +182,11 @@ __device__ __forceinline__ double heston_adjoint_path(const HestonConsts<Real>& 
   const double disc = exp(-r * Td);
   const Real abar = (Real)(disc / N);
   Real U = 0, vbar = 0, ebar_sum = 0;
-  Real Snext = S;  // S_N
   for (int seg = nseg - 1; seg >= 0; --seg) {
-    Real Ss[kHestonSeg], vs[kHestonSeg], z1s[kHestonSeg], zcs[kHestonSeg];
+    // per step of the segment: the spot after it and the step's derivatives
+    // de/dv, dv'/dv at its starting variance (formed in the recompute, so the
+    // reverse step is three multiply-adds)
+    Real Sa[kHestonSeg], de[kHestonSeg], dv[kHestonSeg];
     Real Sr = ck[(size_t)(2 * seg) * stride];
     Real vr = ck[(size_t)(2 * seg + 1) * stride];
 #pragma unroll
@@ -167,35 +194,20 @@ __device__ __forceinline__ double heston_adjoint_path(const HestonConsts<Real>& 
       const int k = seg * kHestonSeg + j;
       if (k < N) {
         // recompute exactly as the forward sweep stepped (heston_philox_pair)
+        Real ra, ca, sa, rb, cb, sb;  // two steps' (squared radius, cos, sin) or (1, z1, z2)
         if constexpr (sizeof(Real) == 4) {
           const U4 q = philox4x32_10(U4{p, (uint32_t)(k >> 1), c2, w}, k0, k1);
-          float q0, c0, s0, q1, c1, s1;  // q0, q1: the squared radii
-          box_muller_polar(q.x, q.y, q0, c0, s0);
-          box_muller_polar(q.z, q.w, q1, c1, s1);
-          const float r0 = fsqrt(q0), r1 = fsqrt(q1);
-          Ss[j] = Sr;
-          vs[j] = vr;
-          z1s[j] = r0 * c0;
-          zcs[j] = r0 * (c.rho * c0 + c.srho * s0);
-          heston_step_polar(c, Sr, vr, q0, c0, s0);
-          Ss[j + 1] = Sr;
-          vs[j + 1] = vr;
-          z1s[j + 1] = r1 * c1;
-          zcs[j + 1] = r1 * (c.rho * c1 + c.srho * s1);
-          if (k + 1 < N) heston_step_polar(c, Sr, vr, q1, c1, s1);
+          box_muller_polar(q.x, q.y, ra, ca, sa);
+          box_muller_polar(q.z, q.w, rb, cb, sb);
         } else {
-          Real n0, n1, n2, n3;
-          normals4<Real>(U4{p, (uint32_t)(k >> 1), c2, w}, k0, k1, n0, n1, n2, n3);
-          Ss[j] = Sr;
-          vs[j] = vr;
-          z1s[j] = n0;
-          zcs[j] = c.rho * n0 + c.srho * n1;
-          heston_step<Real, false>(c, Sr, vr, n0, n1);
-          Ss[j + 1] = Sr;
-          vs[j + 1] = vr;
-          z1s[j + 1] = n2;
-          zcs[j + 1] = c.rho * n2 + c.srho * n3;
-          if (k + 1 < N) heston_step<Real, false>(c, Sr, vr, n2, n3);
+          normals4<Real>(U4{p, (uint32_t)(k >> 1), c2, w}, k0, k1, ca, sa, cb, sb);
+          ra = rb = Real(1);
+        }
+        heston_step_derivs(c, Sr, vr, ra, ca, sa, de[j], dv[j]);
+        Sa[j] = Sr;
+        if (k + 1 < N) {
+          heston_step_derivs(c, Sr, vr, rb, cb, sb, de[j + 1], dv[j + 1]);
+          Sa[j + 1] = Sr;
         }
       }
     }
@@ -203,25 +215,11 @@ __device__ __forceinline__ double heston_adjoint_path(const HestonConsts<Real>& 
     for (int j = kHestonSeg - 1; j >= 0; --j) {
       const int k = seg * kHestonSeg + j;
       if (k < N) {
-        const Real Skp1 = (j + 1 < kHestonSeg && k + 1 < N) ? Ss[j + 1] : Snext;
-        U += abar * Skp1;
+        U += abar * Sa[j];
         ebar_sum += U;
-        const Real vk = vs[j];
-        if (vk > Real(0)) {
-          // FP32: the hardware reciprocal square root (the IEEE quotient costs
-          // a nine-instruction sequence); FP64 keeps the exact quotient
-          Real h;
-          if constexpr (sizeof(Real) == 4)
-            h = Real(0.5) * frsqrt(vk);
-          else
-            h = Real(0.5) / fsqrt(vk);
-          const Real dedv = sdt * z1s[j] * h - hdt;
-          const Real dvdv = Real(1) - c.kdt + c.xsdt * zcs[j] * h;
-          vbar = vbar * dvdv + U * dedv;
-        }
+        vbar = vbar * dv[j] + U * de[j];
       }
     }
-    Snext = Ss[0];
   }
   g[0] = (double)U / S0d;
   g[1] = (double)vbar * 2.0 * sqrt(v0d);
