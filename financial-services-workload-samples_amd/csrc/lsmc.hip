@@ -527,16 +527,60 @@ __device__ __forceinline__ float wave_total(float v) {
 // wave 0 for value q.  The total of value q is returned in thread q < NV
 // (other threads: unspecified); the caller separates reuses of lds by a
 // barrier.
+// Wave totals of up to 12 FP32 values per lane, transposed: v_permlane32_swap
+// pairs value j with value j + 6 (each half-wave then sums one of them over
+// both halves), v_permlane16_swap pairs j with j + 3 within the halves, and
+// four DPP steps finish each of the 3 remaining values within its row of 16
+// lanes.  Row r (lanes 16 r .. 16 r + 15) ends with the totals of values
+// 3 r + j in t[j], 35 instructions where one total per value costs ~9.
+template <int NV>
+__device__ __forceinline__ void wave_totals_transposed(const float (&v)[NV], float (&t)[3]) {
+  static_assert(NV <= 12, "four rows of three values");
+  auto bits = [](float x) { return __builtin_bit_cast(unsigned, x); };
+  auto flt = [](unsigned x) { return __builtin_bit_cast(float, x); };
+  float u[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const auto r = __builtin_amdgcn_permlane32_swap(bits(v[j]), bits(j + 6 < NV ? v[j + 6] : 0.f), false, false);
+    u[j] = flt(r[0]) + flt(r[1]);
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const auto r = __builtin_amdgcn_permlane16_swap(bits(u[j]), bits(u[j + 3]), false, false);
+    t[j] = flt(r[0]) + flt(r[1]);
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    t[j] += dpp_f32<0xB1>(t[j]);   // lane ^ 1
+    t[j] += dpp_f32<0x4E>(t[j]);   // lane ^ 2
+    t[j] += dpp_f32<0x141>(t[j]);  // row half mirror
+    t[j] += dpp_f32<0x140>(t[j]);  // row mirror
+  }
+}
+
 template <int kThreads, typename T, int NV>
 __device__ __forceinline__ double res_block_sum(const T (&v)[NV], double* lds) {
   constexpr int kWaves = kThreads / 64;
   const int wave = threadIdx.x >> 6;
-  double w[NV];
+  if constexpr (sizeof(T) == 4 && NV > 4) {
+    float t[3];
+    wave_totals_transposed(v, t);
+    const int lane = threadIdx.x & 63;
+    if ((lane & 15) == 0) {  // row r = lane / 16 holds values 3 r .. 3 r + 2
 #pragma unroll
-  for (int q = 0; q < NV; ++q) w[q] = (double)wave_total(v[q]);
-  if ((threadIdx.x & 63) == 0) {
+      for (int j = 0; j < 3; ++j) {
+        const int q = 3 * (lane >> 4) + j;
+        if (q < NV) lds[q * kWaves + wave] = (double)t[j];
+      }
+    }
+  } else {
+    double w[NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) lds[q * kWaves + wave] = w[q];
+    for (int q = 0; q < NV; ++q) w[q] = (double)wave_total(v[q]);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+      for (int q = 0; q < NV; ++q) lds[q * kWaves + wave] = w[q];
+    }
   }
   __syncthreads();
   double t = 0.0;
