@@ -311,9 +311,9 @@ int fsmc_lsmc_plan_create(fsmc_ctx* ctx, int n_options, const fsmc_mode* m, cons
 /* Sweeps of a plan (no counterpart in the sample, whose sweep is one loop over
  * a stored path matrix, MonteCarloAmericanOptions/main.cpp:131-266):
  *   FSMC_LSMC_SWEEP_RESIDENT: the whole backward sweep of every option is one
- *     cooperative launch; an option's paths are cut into parts of 8192 (FP32)
- *     or 4096 (FP64) paths, one workgroup each, whose bridge state and cash
- *     flows stay in registers; the parts of an option meet once per date
+ *     cooperative launch; an option's paths are cut into parts of 16384
+ *     (FP32) or 8192 (FP64) paths, one workgroup of a CU each, whose bridge
+ *     state and cash flows stay in registers; the parts of an option meet once per date
  *     through device-memory records (no grid-wide barrier).  Needs every
  *     option's parts to fit the resident grid at once;
  *   FSMC_LSMC_SWEEP_STREAMED: one launch per date, state streamed through
